@@ -1,0 +1,145 @@
+/*
+ * rrtl_hip.h -- C ABI of the MI355X batch-simulation backend for RRTL.
+ *
+ * A drop-in for the reference's batch engines behind `trait SimBackend`
+ * (crates/rrtl-sim-ir/src/lib.rs:5258-5270) and the runtime dispatch surface
+ * `RuntimeEngine` (crates/rrtl-runtime/src/lib.rs:7785-8100).  A caller hands
+ * over a packed machine program (the reference's PackedMachineProgram
+ * serialised in the blob format documented in rrtl_amd/program.py) and a lane
+ * count; the library owns all device memory.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every call returns RRTL_OK or an error status; the message, prefixed with
+ *     an "E_..." diagnostic code like the reference's ErrorReport
+ *     (crates/rrtl-ir/src/lib.rs:632-691), is available from
+ *     rrtl_hip_last_error() (or rrtl_hip_create_error() when create failed).
+ *   - Lane values are u32 limbs laid out [limbs][lanes] (limb-major), the
+ *     per-signal slice of PackedSimulatorStorage.values (lib.rs:10045).
+ *   - Memory values are [lanes][depth][limbs], the lane-major order of
+ *     PackedSimulatorStorage.memories (lib.rs:10049-10054).
+ *   - A handle is single-owner and not thread-safe; separate handles may be
+ *     used from separate threads.  Calls are synchronous unless noted.
+ */
+#ifndef RRTL_HIP_H
+#define RRTL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rrtl_status {
+  RRTL_OK = 0,
+  RRTL_E_INVALID = 1,     /* bad argument (null pointer, index or lane out of range) */
+  RRTL_E_PROGRAM = 2,     /* malformed program blob */
+  RRTL_E_UNSUPPORTED = 3, /* design outside this engine's scope (caller falls back) */
+  RRTL_E_HIP = 4,         /* HIP runtime failure */
+  RRTL_E_STATE = 5        /* call not valid in the current state (e.g. no run yet) */
+} rrtl_status;
+
+typedef struct rrtl_hip_engine rrtl_hip_engine;
+
+/* Engine kinds (the new SimBackendKind arm; reference kinds lib.rs:3407-3425). */
+enum { RRTL_HIP_AUTO = 0, RRTL_HIP_WORD = 1, RRTL_HIP_GATE = 2 };
+
+/* Options (the HipOptions arm of RuntimeBackend, crates/rrtl-runtime/src/lib.rs:29-46). */
+typedef struct rrtl_hip_options {
+  uint32_t struct_size;      /* sizeof(rrtl_hip_options) */
+  uint32_t engine;           /* RRTL_HIP_AUTO picks GATE for all-1-bit bitwise designs */
+  int32_t device;            /* HIP device ordinal */
+  uint32_t block_threads;    /* word engine workgroup size, 0 = default (256) */
+  uint32_t tile_words;       /* gate engine u32 lane-words per workgroup (1,2,4,8), 0 = auto */
+  uint32_t ticks_per_launch; /* cycles per kernel launch in runs, 0 = whole run in one launch */
+} rrtl_hip_options;
+
+/* SimdCpuSimulator::new (lib.rs:5529): lanes == 0 is E_SIM_IR_LANES; the
+ * constructor settles combinational logic (lib.rs:9058). */
+rrtl_status rrtl_hip_create(const void* program_blob, size_t blob_bytes, uint32_t lanes,
+                            const rrtl_hip_options* options, rrtl_hip_engine** out);
+void rrtl_hip_destroy(rrtl_hip_engine* engine);
+const char* rrtl_hip_last_error(const rrtl_hip_engine* engine);
+const char* rrtl_hip_create_error(void); /* thread-local message of the last failed create */
+
+uint32_t rrtl_hip_kind(const rrtl_hip_engine* engine);   /* SimBackend::kind */
+uint32_t rrtl_hip_lanes(const rrtl_hip_engine* engine);  /* SimBackend::lanes */
+
+/* set_signal_limbs / set_packed_signal_limbs (lib.rs:9909-9943): writes every
+ * lane and re-settles combinational logic. */
+rrtl_status rrtl_hip_set_signal_limbs(rrtl_hip_engine* e, uint32_t signal, const uint32_t* lane_limbs);
+/* set_signals_replicated_raw (lib.rs:9118): writes without settling. */
+rrtl_status rrtl_hip_set_signal_limbs_raw(rrtl_hip_engine* e, uint32_t signal, const uint32_t* lane_limbs);
+/* get_signal_limbs (lib.rs:9177). */
+rrtl_status rrtl_hip_get_signal_limbs(rrtl_hip_engine* e, uint32_t signal, uint32_t* out_lane_limbs);
+/* Gate engine only: a 1-bit signal's lanes packed 32 per u32 word (ceil(lanes/32) words). */
+rrtl_status rrtl_hip_get_signal_bits(rrtl_hip_engine* e, uint32_t signal, uint32_t* out_words);
+/* set_memory_limbs / get_memory_limbs (lib.rs:10136-10180, RuntimeEngine :7960). */
+rrtl_status rrtl_hip_set_memory_limbs(rrtl_hip_engine* e, uint32_t memory, const uint32_t* values);
+rrtl_status rrtl_hip_get_memory_limbs(rrtl_hip_engine* e, uint32_t memory, uint32_t* out_values);
+
+rrtl_status rrtl_hip_eval_combinational(rrtl_hip_engine* e);   /* lib.rs:5735 */
+rrtl_status rrtl_hip_tick(rrtl_hip_engine* e);                 /* lib.rs:5741 */
+rrtl_status rrtl_hip_tick_from_evaluated(rrtl_hip_engine* e);  /* tick_from_evaluated_no_post_eval, lib.rs:5748 */
+rrtl_status rrtl_hip_tick_many(rrtl_hip_engine* e, uint64_t steps); /* comb-fused tick_many, lib.rs:5784-5803 */
+
+/* snapshot_storage / restore_storage (lib.rs:9070-9103): values are
+ * total_signal_words*lanes u32, memories total_memory_words*lanes u32. */
+rrtl_status rrtl_hip_snapshot(rrtl_hip_engine* e, uint32_t* values, uint32_t* memories);
+rrtl_status rrtl_hip_restore(rrtl_hip_engine* e, const uint32_t* values, const uint32_t* memories);
+
+/* Stuck-at fault injection, BitParallelSimulator::set_force / clear_forces
+ * (crates/rrtl-sim-ir/src/bitparallel.rs:96-101); gate engine only. */
+rrtl_status rrtl_hip_set_force(rrtl_hip_engine* e, uint32_t signal, uint32_t lane, int value);
+rrtl_status rrtl_hip_clear_forces(rrtl_hip_engine* e);
+
+rrtl_status rrtl_hip_synchronize(rrtl_hip_engine* e);         /* InterpGpuSimulator::synchronize, interp.rs:2044 */
+
+/* ---- device-resident runs: the per-cycle replay loop of the reference's
+ * harnesses (PackedSimulator replay, lib.rs:9500-9570; fault_sim.rs:66-97)
+ * moved on chip.  Per cycle: drive inputs, settle, observe (PRE), edge
+ * (capture, memory writes, commit, forces), then either a settle + observe
+ * (POST) or nothing; the state after the run equals the reference's after its
+ * trailing settle. */
+enum { RRTL_IN_HOLD = 0, RRTL_IN_RANDOM = 1, RRTL_IN_CONST = 2, RRTL_IN_PULSE = 3, RRTL_IN_TRACE = 4 };
+enum { RRTL_OBSERVE_PRE = 0, RRTL_OBSERVE_POST = 1 };
+typedef struct rrtl_hip_run_spec {
+  uint32_t struct_size;
+  uint64_t seed;
+  uint64_t cycles;
+  uint32_t n_inputs;
+  const uint32_t* input_signal;
+  const uint32_t* input_mode;   /* RRTL_IN_*: RANDOM = counter hash of (seed, signal<<8|bit, cycle, lane/64) */
+  const uint64_t* input_arg;    /* CONST: value; PULSE: low 32 bits n, bit 32 value while cycle < n, bit 33 after */
+  const uint32_t* trace_inputs; /* RRTL_IN_TRACE values: [cycles][trace limbs][lanes] (word engine) */
+  uint32_t n_observe;
+  const uint32_t* observe_signal;
+  uint32_t observe_point;       /* RRTL_OBSERVE_PRE / RRTL_OBSERVE_POST */
+  uint32_t record_trace;        /* also keep every observed value: [cycles][...][...] */
+  uint32_t halt_signal;         /* word engine: a lane stops after the cycle this bit is 1; UINT32_MAX = none */
+} rrtl_hip_run_spec;
+
+typedef struct rrtl_hip_run_stats {
+  double kernel_ms;       /* device time of the run's launches (HIP events on the engine stream) */
+  uint64_t launches;
+  uint64_t lane_cycles;   /* cycles actually simulated summed over lanes (halted lanes stop counting) */
+} rrtl_hip_run_stats;
+
+rrtl_status rrtl_hip_run(rrtl_hip_engine* e, const rrtl_hip_run_spec* spec);
+rrtl_status rrtl_hip_last_run_stats(const rrtl_hip_engine* e, rrtl_hip_run_stats* out);
+/* Observation digest of the last run.  Gate engine: 64 x ceil(lanes/32) u32
+ * bit-sliced signature words; word engine: one u64 chain per lane. */
+size_t rrtl_hip_digest_bytes(const rrtl_hip_engine* e);
+rrtl_status rrtl_hip_get_digest(rrtl_hip_engine* e, void* out);
+size_t rrtl_hip_trace_bytes(const rrtl_hip_engine* e);
+rrtl_status rrtl_hip_get_trace(rrtl_hip_engine* e, void* out);
+/* Word engine runs with a halt signal: cycles simulated per lane. */
+rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out);
+/* Gate engine: [tile_words, live_slots, levels, gates] of the last compiled run. */
+rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RRTL_HIP_H */

@@ -1,0 +1,305 @@
+"""Python binding of the C ABI (``include/rrtl_hip.h``) via ctypes.
+
+``HipSimulator`` mirrors the reference's backend trait ``SimBackend``
+(crates/rrtl-sim-ir/src/lib.rs:5258-5270) and the ``RuntimeEngine`` surface
+(crates/rrtl-runtime/src/lib.rs:7785-8100): signal and memory access in u32
+limbs, ``eval_combinational``, ``tick``, ``tick_from_evaluated`` (the
+reference's ``tick_from_evaluated_no_post_eval``), ``tick_many``,
+``snapshot``/``restore`` in the ``PackedSimulatorStorage`` layout, stuck-at
+forces, and the device-resident ``run``.  Errors raise ``HipError`` carrying
+the library's ``E_*`` diagnostic code, as the reference returns an
+``ErrorReport``.
+
+There is no fallback: if ``librrtl_hip.so`` is missing the import fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from .program import Program
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librrtl_hip.so")
+
+RRTL_HIP_AUTO, RRTL_HIP_WORD, RRTL_HIP_GATE = 0, 1, 2
+IN_HOLD, IN_RANDOM, IN_CONST, IN_PULSE, IN_TRACE = 0, 1, 2, 3, 4
+OBSERVE_PRE, OBSERVE_POST = 0, 1
+NO_HALT = 0xFFFFFFFF
+ENGINE_NAMES = {"auto": RRTL_HIP_AUTO, "word": RRTL_HIP_WORD, "gate": RRTL_HIP_GATE}
+
+# every symbol the header declares
+EXPORTS = [
+    "rrtl_hip_create", "rrtl_hip_destroy", "rrtl_hip_last_error", "rrtl_hip_create_error", "rrtl_hip_kind",
+    "rrtl_hip_lanes", "rrtl_hip_set_signal_limbs", "rrtl_hip_set_signal_limbs_raw", "rrtl_hip_get_signal_limbs",
+    "rrtl_hip_get_signal_bits", "rrtl_hip_set_memory_limbs", "rrtl_hip_get_memory_limbs",
+    "rrtl_hip_eval_combinational", "rrtl_hip_tick", "rrtl_hip_tick_from_evaluated", "rrtl_hip_tick_many",
+    "rrtl_hip_snapshot", "rrtl_hip_restore", "rrtl_hip_set_force", "rrtl_hip_clear_forces",
+    "rrtl_hip_synchronize", "rrtl_hip_run", "rrtl_hip_last_run_stats", "rrtl_hip_digest_bytes",
+    "rrtl_hip_get_digest", "rrtl_hip_trace_bytes", "rrtl_hip_get_trace", "rrtl_hip_get_halt_cycles",
+    "rrtl_hip_gate_tile_info",
+]
+
+
+class HipError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+        self.code = message.split(":", 1)[0] if message else ""
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("engine", ctypes.c_uint32), ("device", ctypes.c_int32),
+                ("block_threads", ctypes.c_uint32), ("tile_words", ctypes.c_uint32),
+                ("ticks_per_launch", ctypes.c_uint32)]
+
+
+class RunSpec(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("cycles", ctypes.c_uint64),
+                ("n_inputs", ctypes.c_uint32), ("input_signal", ctypes.c_void_p), ("input_mode", ctypes.c_void_p),
+                ("input_arg", ctypes.c_void_p), ("trace_inputs", ctypes.c_void_p), ("n_observe", ctypes.c_uint32),
+                ("observe_signal", ctypes.c_void_p), ("observe_point", ctypes.c_uint32),
+                ("record_trace", ctypes.c_uint32), ("halt_signal", ctypes.c_uint32)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("launches", ctypes.c_uint64), ("lane_cycles", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def load_library() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"E_HIP_LIBRARY: {LIB_PATH} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t
+    L.rrtl_hip_create.argtypes = [ctypes.c_char_p, sz, u32, ctypes.POINTER(Options), ctypes.POINTER(vp)]
+    L.rrtl_hip_destroy.argtypes = [vp]
+    L.rrtl_hip_last_error.restype = ctypes.c_char_p
+    L.rrtl_hip_last_error.argtypes = [vp]
+    L.rrtl_hip_create_error.restype = ctypes.c_char_p
+    L.rrtl_hip_kind.restype = u32
+    L.rrtl_hip_kind.argtypes = [vp]
+    L.rrtl_hip_lanes.restype = u32
+    L.rrtl_hip_lanes.argtypes = [vp]
+    for fn in ("rrtl_hip_set_signal_limbs", "rrtl_hip_set_signal_limbs_raw", "rrtl_hip_get_signal_limbs",
+               "rrtl_hip_get_signal_bits", "rrtl_hip_set_memory_limbs", "rrtl_hip_get_memory_limbs"):
+        getattr(L, fn).argtypes = [vp, u32, vp]
+    for fn in ("rrtl_hip_eval_combinational", "rrtl_hip_tick", "rrtl_hip_tick_from_evaluated",
+               "rrtl_hip_clear_forces", "rrtl_hip_synchronize"):
+        getattr(L, fn).argtypes = [vp]
+    L.rrtl_hip_tick_many.argtypes = [vp, u64]
+    L.rrtl_hip_snapshot.argtypes = [vp, vp, vp]
+    L.rrtl_hip_restore.argtypes = [vp, vp, vp]
+    L.rrtl_hip_set_force.argtypes = [vp, u32, u32, ctypes.c_int]
+    L.rrtl_hip_run.argtypes = [vp, ctypes.POINTER(RunSpec)]
+    L.rrtl_hip_last_run_stats.argtypes = [vp, ctypes.POINTER(RunStats)]
+    L.rrtl_hip_digest_bytes.restype = sz
+    L.rrtl_hip_digest_bytes.argtypes = [vp]
+    L.rrtl_hip_trace_bytes.restype = sz
+    L.rrtl_hip_trace_bytes.argtypes = [vp]
+    L.rrtl_hip_get_digest.argtypes = [vp, vp]
+    L.rrtl_hip_get_trace.argtypes = [vp, vp]
+    L.rrtl_hip_get_halt_cycles.argtypes = [vp, vp]
+    L.rrtl_hip_gate_tile_info.argtypes = [vp, vp]
+    for fn in EXPORTS:
+        if fn != "rrtl_hip_destroy" and getattr(L, fn).restype is ctypes.c_int:
+            getattr(L, fn).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+SignalRef = Union[int, str]
+
+
+class HipSimulator:
+    """A batch of ``lanes`` independent instances of one program on one GPU."""
+
+    def __init__(self, program: Program, lanes: int, engine: str = "auto", device: int = 0,
+                 tile_words: int = 0, ticks_per_launch: int = 0, block_threads: int = 0):
+        self.L = load_library()
+        self.program = program
+        blob = program.to_blob()
+        opt = Options(ctypes.sizeof(Options), ENGINE_NAMES[engine], device, block_threads, tile_words,
+                      ticks_per_launch)
+        h = ctypes.c_void_p()
+        st = self.L.rrtl_hip_create(blob, len(blob), lanes, ctypes.byref(opt), ctypes.byref(h))
+        if st != 0:
+            raise HipError(st, self.L.rrtl_hip_create_error().decode())
+        self.h = h
+        self.lanes = lanes
+        self.words32 = (lanes + 31) // 32
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.rrtl_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ helpers
+    def _check(self, st: int) -> None:
+        if st != 0:
+            raise HipError(st, self.L.rrtl_hip_last_error(self.h).decode())
+
+    def sig(self, s: SignalRef) -> int:
+        return self.program.signal_index(s) if isinstance(s, str) else int(s)
+
+    def mem(self, m: Union[int, str]) -> int:
+        return self.program.memory_index(m) if isinstance(m, str) else int(m)
+
+    @property
+    def kind(self) -> str:
+        return {1: "word", 2: "gate"}[self.L.rrtl_hip_kind(self.h)]
+
+    # ------------------------------------------------------------ signals
+    def set_signal_limbs(self, s: SignalRef, limbs: np.ndarray, settle: bool = True) -> None:
+        i = self.sig(s)
+        n = self.program.signals[i].limbs
+        a = np.ascontiguousarray(np.asarray(limbs, dtype=np.uint32).reshape(n, self.lanes))
+        fn = self.L.rrtl_hip_set_signal_limbs if settle else self.L.rrtl_hip_set_signal_limbs_raw
+        self._check(fn(self.h, i, _ptr(a)))
+
+    def set_signal(self, s: SignalRef, values, settle: bool = True) -> None:
+        i = self.sig(s)
+        n = self.program.signals[i].limbs
+        a = np.zeros((n, self.lanes), dtype=np.uint32)
+        if np.isscalar(values):
+            for l in range(n):
+                a[l, :] = (int(values) >> (32 * l)) & 0xFFFFFFFF
+        else:
+            vals = list(values)
+            for l in range(n):
+                a[l, :] = [(int(v) >> (32 * l)) & 0xFFFFFFFF for v in vals]
+        self.set_signal_limbs(i, a, settle)
+
+    def get_signal_limbs(self, s: SignalRef) -> np.ndarray:
+        i = self.sig(s)
+        out = np.zeros((self.program.signals[i].limbs, self.lanes), dtype=np.uint32)
+        self._check(self.L.rrtl_hip_get_signal_limbs(self.h, i, _ptr(out)))
+        return out
+
+    def get_signal(self, s: SignalRef) -> List[int]:
+        a = self.get_signal_limbs(s)
+        if a.shape[0] == 1:
+            return [int(x) for x in a[0]]
+        return [sum(int(a[l, k]) << (32 * l) for l in range(a.shape[0])) for k in range(self.lanes)]
+
+    def get_signal_bits(self, s: SignalRef) -> np.ndarray:
+        out = np.zeros(self.words32, dtype=np.uint32)
+        self._check(self.L.rrtl_hip_get_signal_bits(self.h, self.sig(s), _ptr(out)))
+        return out
+
+    def set_memory_limbs(self, m, values: np.ndarray) -> None:
+        i = self.mem(m)
+        pm = self.program.memories[i]
+        a = np.ascontiguousarray(np.asarray(values, dtype=np.uint32).reshape(self.lanes, pm.depth, pm.limbs))
+        self._check(self.L.rrtl_hip_set_memory_limbs(self.h, i, _ptr(a)))
+
+    def get_memory_limbs(self, m) -> np.ndarray:
+        i = self.mem(m)
+        pm = self.program.memories[i]
+        out = np.zeros((self.lanes, pm.depth, pm.limbs), dtype=np.uint32)
+        self._check(self.L.rrtl_hip_get_memory_limbs(self.h, i, _ptr(out)))
+        return out
+
+    # ------------------------------------------------------------ ticking
+    def eval_combinational(self) -> None:
+        self._check(self.L.rrtl_hip_eval_combinational(self.h))
+
+    def tick(self) -> None:
+        self._check(self.L.rrtl_hip_tick(self.h))
+
+    def tick_from_evaluated(self) -> None:
+        self._check(self.L.rrtl_hip_tick_from_evaluated(self.h))
+
+    def tick_many(self, steps: int) -> None:
+        self._check(self.L.rrtl_hip_tick_many(self.h, steps))
+
+    def synchronize(self) -> None:
+        self._check(self.L.rrtl_hip_synchronize(self.h))
+
+    def snapshot(self) -> Tuple[np.ndarray, np.ndarray]:
+        v = np.zeros(self.program.total_signal_words * self.lanes, dtype=np.uint32)
+        m = np.zeros(max(1, self.program.total_memory_words_per_lane * self.lanes), dtype=np.uint32)
+        self._check(self.L.rrtl_hip_snapshot(self.h, _ptr(v), _ptr(m)))
+        return v, m[: self.program.total_memory_words_per_lane * self.lanes]
+
+    def restore(self, values: np.ndarray, memories: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, dtype=np.uint32)
+        m = np.ascontiguousarray(memories, dtype=np.uint32)
+        if m.size == 0:
+            m = np.zeros(1, dtype=np.uint32)
+        self._check(self.L.rrtl_hip_restore(self.h, _ptr(v), _ptr(m)))
+
+    def set_force(self, s: SignalRef, lane: int, value: int) -> None:
+        self._check(self.L.rrtl_hip_set_force(self.h, self.sig(s), lane, int(value)))
+
+    def clear_forces(self) -> None:
+        self._check(self.L.rrtl_hip_clear_forces(self.h))
+
+    # ------------------------------------------------------------ runs
+    def run(self, cycles: int, seed: int = 0, inputs: Sequence[Tuple] = (), observe: Sequence[SignalRef] = (),
+            observe_post: bool = False, trace: bool = False, halt: Optional[SignalRef] = None,
+            trace_inputs: Optional[np.ndarray] = None) -> None:
+        """``inputs``: (signal, mode, arg) triples with modes IN_*; ``observe``: signals
+        folded into the digest (and the trace when ``trace``)."""
+        isig = np.array([self.sig(i[0]) for i in inputs] or [0], dtype=np.uint32)
+        imode = np.array([i[1] for i in inputs] or [0], dtype=np.uint32)
+        iarg = np.array([int(i[2]) if len(i) > 2 else 0 for i in inputs] or [0], dtype=np.uint64)
+        osig = np.array([self.sig(o) for o in observe] or [0], dtype=np.uint32)
+        tin = None if trace_inputs is None else np.ascontiguousarray(trace_inputs, dtype=np.uint32)
+        self._keep = (isig, imode, iarg, osig, tin)
+        spec = RunSpec(ctypes.sizeof(RunSpec), seed, cycles, len(inputs), isig.ctypes.data, imode.ctypes.data,
+                       iarg.ctypes.data, tin.ctypes.data if tin is not None else None, len(observe),
+                       osig.ctypes.data, OBSERVE_POST if observe_post else OBSERVE_PRE, int(trace),
+                       NO_HALT if halt is None else self.sig(halt))
+        self._check(self.L.rrtl_hip_run(self.h, ctypes.byref(spec)))
+
+    def stats(self) -> dict:
+        s = RunStats()
+        self._check(self.L.rrtl_hip_last_run_stats(self.h, ctypes.byref(s)))
+        return {"kernel_ms": s.kernel_ms, "launches": s.launches, "lane_cycles": s.lane_cycles}
+
+    def digest(self) -> np.ndarray:
+        n = self.L.rrtl_hip_digest_bytes(self.h)
+        if self.kind == "gate":
+            out = np.zeros((64, self.words32), dtype=np.uint32)
+        else:
+            out = np.zeros(self.lanes, dtype=np.uint64)
+        assert out.nbytes == n, (out.nbytes, n)
+        self._check(self.L.rrtl_hip_get_digest(self.h, _ptr(out)))
+        return out
+
+    def trace(self) -> np.ndarray:
+        n = self.L.rrtl_hip_trace_bytes(self.h)
+        out = np.zeros(n // 4, dtype=np.uint32)
+        self._check(self.L.rrtl_hip_get_trace(self.h, _ptr(out)))
+        return out
+
+    def halt_cycles(self) -> np.ndarray:
+        out = np.zeros(self.lanes, dtype=np.uint64)
+        self._check(self.L.rrtl_hip_get_halt_cycles(self.h, _ptr(out)))
+        return out
+
+    def tile_info(self) -> dict:
+        out = np.zeros(4, dtype=np.uint32)
+        self._check(self.L.rrtl_hip_gate_tile_info(self.h, _ptr(out)))
+        return {"tile_words": int(out[0]), "live_slots": int(out[1]), "levels": int(out[2]), "gates": int(out[3])}

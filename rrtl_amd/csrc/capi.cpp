@@ -1,0 +1,181 @@
+// C ABI over the engines (include/rrtl_hip.h).  Every entry point catches C++
+// exceptions and turns them into a status plus an "E_..." message.
+#include "../../include/rrtl_hip.h"
+#include "engine.hpp"
+#include <cstring>
+#include <memory>
+#include <new>
+
+namespace rrtl {
+void gate_engine_get_words(Engine* e, uint32_t sig, uint32_t* out);
+void gate_engine_tile_info(Engine* e, uint32_t* info);
+}
+
+struct rrtl_hip_engine {
+  std::unique_ptr<rrtl::Engine> impl;
+  std::string err;
+};
+
+namespace {
+thread_local std::string g_create_err;
+
+rrtl_status classify(const std::string& m) {
+  if (m.rfind("E_HIP_RUNTIME", 0) == 0) return RRTL_E_HIP;
+  if (m.rfind("E_PROGRAM", 0) == 0 || m.rfind("E_SIM_IR_MACHINE", 0) == 0) return RRTL_E_PROGRAM;
+  if (m.rfind("E_BITPARALLEL", 0) == 0 || m.rfind("E_HIP_UNSUPPORTED", 0) == 0 || m.rfind("E_HIP_WIDE", 0) == 0 ||
+      m.rfind("E_HIP_TOO_LARGE", 0) == 0 || m.rfind("E_HIP_COMB_ORDER", 0) == 0)
+    return RRTL_E_UNSUPPORTED;
+  if (m.rfind("E_HIP_NO_RUN", 0) == 0) return RRTL_E_STATE;
+  return RRTL_E_INVALID;
+}
+
+template <class F>
+rrtl_status guard(rrtl_hip_engine* e, F&& f) {
+  if (!e || !e->impl) return RRTL_E_INVALID;
+  try {
+    f(*e->impl);
+    e->err.clear();
+    return RRTL_OK;
+  } catch (const std::exception& ex) {
+    e->err = ex.what();
+    return classify(e->err);
+  }
+}
+
+rrtl::RunSpec to_spec(const rrtl_hip_run_spec* s) {
+  if (!s || s->struct_size < sizeof(rrtl_hip_run_spec)) throw rrtl::Error("E_HIP_RUN_SPEC: run spec is null or too small");
+  rrtl::RunSpec r;
+  r.seed = s->seed; r.cycles = s->cycles;
+  if (s->n_inputs && (!s->input_signal || !s->input_mode)) throw rrtl::Error("E_HIP_RUN_SPEC: input arrays are null");
+  if (s->n_observe && !s->observe_signal) throw rrtl::Error("E_HIP_RUN_SPEC: observe array is null");
+  r.in_sig.assign(s->input_signal, s->input_signal + s->n_inputs);
+  r.in_mode.assign(s->input_mode, s->input_mode + s->n_inputs);
+  if (s->input_arg) r.in_arg.assign(s->input_arg, s->input_arg + s->n_inputs);
+  else r.in_arg.assign(s->n_inputs, 0);
+  for (uint32_t m : r.in_mode) if (m > RRTL_IN_TRACE) throw rrtl::Error("E_HIP_RUN_SPEC: unknown input mode");
+  r.obs_sig.assign(s->observe_signal, s->observe_signal + s->n_observe);
+  r.observe_post = s->observe_point == RRTL_OBSERVE_POST;
+  r.trace = s->record_trace != 0;
+  r.halt_sig = s->halt_signal;
+  r.trace_inputs = s->trace_inputs;
+  return r;
+}
+}  // namespace
+
+extern "C" {
+
+rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, const rrtl_hip_options* options,
+                            rrtl_hip_engine** out) {
+  if (!out) return RRTL_E_INVALID;
+  *out = nullptr;
+  try {
+    if (lanes == 0) throw rrtl::Error("E_SIM_IR_LANES: packed simulator requires at least one lane");
+    rrtl::EngineOptions opt;
+    uint32_t kind = RRTL_HIP_AUTO;
+    if (options) {
+      if (options->struct_size < sizeof(rrtl_hip_options)) throw rrtl::Error("E_HIP_OPTIONS: options struct is too small");
+      kind = options->engine; opt.device = options->device; opt.block_threads = options->block_threads;
+      opt.tile_words = options->tile_words; opt.ticks_per_launch = options->ticks_per_launch;
+    }
+    if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 1024))
+      throw rrtl::Error("E_HIP_OPTIONS: block_threads must be a multiple of 64 up to 1024");
+    if (opt.tile_words && opt.tile_words != 1 && opt.tile_words != 2 && opt.tile_words != 4 && opt.tile_words != 8)
+      throw rrtl::Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
+    rrtl::Program prog = rrtl::Program::parse(blob, bytes);
+    if (kind == RRTL_HIP_AUTO) kind = prog.is_gate_level() ? RRTL_HIP_GATE : RRTL_HIP_WORD;
+    auto* h = new rrtl_hip_engine;
+    try {
+      if (kind == RRTL_HIP_GATE) h->impl.reset(rrtl::make_gate_engine(std::move(prog), lanes, opt));
+      else if (kind == RRTL_HIP_WORD) h->impl.reset(rrtl::make_word_engine(std::move(prog), lanes, opt));
+      else throw rrtl::Error("E_HIP_OPTIONS: unknown engine kind");
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    g_create_err.clear();
+    return RRTL_OK;
+  } catch (const std::exception& ex) {
+    g_create_err = ex.what();
+    return classify(g_create_err);
+  }
+}
+
+void rrtl_hip_destroy(rrtl_hip_engine* e) { delete e; }
+const char* rrtl_hip_last_error(const rrtl_hip_engine* e) { return e ? e->err.c_str() : "E_HIP_NULL: null engine"; }
+const char* rrtl_hip_create_error(void) { return g_create_err.c_str(); }
+uint32_t rrtl_hip_kind(const rrtl_hip_engine* e) { return e && e->impl ? e->impl->kind() : 0; }
+uint32_t rrtl_hip_lanes(const rrtl_hip_engine* e) { return e && e->impl ? e->impl->lanes() : 0; }
+
+rrtl_status rrtl_hip_set_signal_limbs(rrtl_hip_engine* e, uint32_t s, const uint32_t* v) {
+  return guard(e, [&](rrtl::Engine& x) { if (!v) throw rrtl::Error("E_HIP_NULL: null buffer"); x.set_signal_limbs(s, v, true); });
+}
+rrtl_status rrtl_hip_set_signal_limbs_raw(rrtl_hip_engine* e, uint32_t s, const uint32_t* v) {
+  return guard(e, [&](rrtl::Engine& x) { if (!v) throw rrtl::Error("E_HIP_NULL: null buffer"); x.set_signal_limbs(s, v, false); });
+}
+rrtl_status rrtl_hip_get_signal_limbs(rrtl_hip_engine* e, uint32_t s, uint32_t* out) {
+  return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_signal_limbs(s, out); });
+}
+rrtl_status rrtl_hip_get_signal_bits(rrtl_hip_engine* e, uint32_t s, uint32_t* out) {
+  return guard(e, [&](rrtl::Engine& x) {
+    if (!out) throw rrtl::Error("E_HIP_NULL: null buffer");
+    if (x.kind() != RRTL_HIP_GATE) throw rrtl::Error("E_HIP_UNSUPPORTED: packed bits need the gate engine");
+    rrtl::gate_engine_get_words(&x, s, out);
+  });
+}
+rrtl_status rrtl_hip_set_memory_limbs(rrtl_hip_engine* e, uint32_t m, const uint32_t* v) {
+  return guard(e, [&](rrtl::Engine& x) { if (!v) throw rrtl::Error("E_HIP_NULL: null buffer"); x.set_memory_limbs(m, v, true); });
+}
+rrtl_status rrtl_hip_get_memory_limbs(rrtl_hip_engine* e, uint32_t m, uint32_t* out) {
+  return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_memory_limbs(m, out); });
+}
+rrtl_status rrtl_hip_eval_combinational(rrtl_hip_engine* e) { return guard(e, [](rrtl::Engine& x) { x.eval_combinational(); }); }
+rrtl_status rrtl_hip_tick(rrtl_hip_engine* e) { return guard(e, [](rrtl::Engine& x) { x.tick(); }); }
+rrtl_status rrtl_hip_tick_from_evaluated(rrtl_hip_engine* e) { return guard(e, [](rrtl::Engine& x) { x.tick_from_evaluated(); }); }
+rrtl_status rrtl_hip_tick_many(rrtl_hip_engine* e, uint64_t n) { return guard(e, [&](rrtl::Engine& x) { x.tick_many(n); }); }
+rrtl_status rrtl_hip_snapshot(rrtl_hip_engine* e, uint32_t* v, uint32_t* m) {
+  return guard(e, [&](rrtl::Engine& x) {
+    if (!v || (!m && x.program().total_memory_words)) throw rrtl::Error("E_HIP_NULL: null buffer");
+    x.snapshot(v, m);
+  });
+}
+rrtl_status rrtl_hip_restore(rrtl_hip_engine* e, const uint32_t* v, const uint32_t* m) {
+  return guard(e, [&](rrtl::Engine& x) {
+    if (!v || (!m && x.program().total_memory_words)) throw rrtl::Error("E_HIP_NULL: null buffer");
+    x.restore(v, m);
+  });
+}
+rrtl_status rrtl_hip_set_force(rrtl_hip_engine* e, uint32_t s, uint32_t lane, int value) {
+  return guard(e, [&](rrtl::Engine& x) { x.set_force(s, lane, value); });
+}
+rrtl_status rrtl_hip_clear_forces(rrtl_hip_engine* e) { return guard(e, [](rrtl::Engine& x) { x.clear_forces(); }); }
+rrtl_status rrtl_hip_synchronize(rrtl_hip_engine* e) { return guard(e, [](rrtl::Engine& x) { x.synchronize(); }); }
+rrtl_status rrtl_hip_run(rrtl_hip_engine* e, const rrtl_hip_run_spec* spec) {
+  return guard(e, [&](rrtl::Engine& x) { x.run(to_spec(spec)); });
+}
+rrtl_status rrtl_hip_last_run_stats(const rrtl_hip_engine* e, rrtl_hip_run_stats* out) {
+  if (!e || !e->impl || !out) return RRTL_E_INVALID;
+  const rrtl::RunStats& s = e->impl->last_stats();
+  out->kernel_ms = s.kernel_ms; out->launches = s.launches; out->lane_cycles = s.lane_cycles;
+  return RRTL_OK;
+}
+size_t rrtl_hip_digest_bytes(const rrtl_hip_engine* e) { return e && e->impl ? e->impl->digest_bytes() : 0; }
+rrtl_status rrtl_hip_get_digest(rrtl_hip_engine* e, void* out) {
+  return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_digest(out); });
+}
+size_t rrtl_hip_trace_bytes(const rrtl_hip_engine* e) { return e && e->impl ? e->impl->trace_bytes() : 0; }
+rrtl_status rrtl_hip_get_trace(rrtl_hip_engine* e, void* out) {
+  return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_trace(out); });
+}
+rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out) {
+  return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_halt_cycles(out); });
+}
+rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4) {
+  return guard(e, [&](rrtl::Engine& x) {
+    if (!out4) throw rrtl::Error("E_HIP_NULL: null buffer");
+    if (x.kind() != RRTL_HIP_GATE) throw rrtl::Error("E_HIP_UNSUPPORTED: not a gate engine");
+    rrtl::gate_engine_tile_info(&x, out4);
+  });
+}
+
+}  // extern "C"

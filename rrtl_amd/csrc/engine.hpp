@@ -1,0 +1,69 @@
+// Engine interface: the C++ mirror of the reference's backend trait
+// `trait SimBackend` (crates/rrtl-sim-ir/src/lib.rs:5258-5270) plus the runtime
+// dispatch surface of `RuntimeEngine` (crates/rrtl-runtime/src/lib.rs:7785-8100):
+// signal/memory access in limbs, eval_combinational, tick,
+// tick_from_evaluated_no_post_eval, tick_many, snapshot/restore in the
+// PackedSimulatorStorage layout, stuck-at forces, and the device-resident run
+// (stimulus in, observations out) that replaces per-cycle host round trips.
+#pragma once
+#include "common.hpp"
+
+namespace rrtl {
+
+enum EngineKind : uint32_t { ENGINE_AUTO = 0, ENGINE_WORD = 1, ENGINE_GATE = 2 };
+
+struct EngineOptions {
+  int device = 0;
+  uint32_t block_threads = 0;      // 0: engine default
+  uint32_t tile_words = 0;         // gate engine: u32 lane-words per workgroup tile (0: autotune)
+  uint32_t ticks_per_launch = 0;   // 0: whole run in one launch
+};
+
+class Engine {
+ public:
+  Engine(Program prog, uint32_t lanes, const EngineOptions& opt) : P(std::move(prog)), lanes_(lanes), opt_(opt) {}
+  virtual ~Engine() = default;
+  virtual uint32_t kind() const = 0;
+  uint32_t lanes() const { return lanes_; }
+  const Program& program() const { return P; }
+
+  // Writes lane values ([limbs][lanes] u32); `settle` re-evaluates comb like
+  // set_packed_signal_limbs (lib.rs:9909-9943).
+  virtual void set_signal_limbs(uint32_t sig, const uint32_t* lane_limbs, bool settle) = 0;
+  virtual void get_signal_limbs(uint32_t sig, uint32_t* out) = 0;
+  // Memory values are [lanes][depth][limbs] (set_memory_limbs, lib.rs:10136).
+  virtual void set_memory_limbs(uint32_t mem, const uint32_t* vals, bool settle) = 0;
+  virtual void get_memory_limbs(uint32_t mem, uint32_t* out) = 0;
+  virtual void eval_combinational() = 0;
+  virtual void tick() = 0;
+  virtual void tick_from_evaluated() = 0;
+  virtual void tick_many(uint64_t steps) = 0;
+  // values: word-major (offset+limb)*lanes+lane; memories: lane-major (lib.rs:10045-10054)
+  virtual void snapshot(uint32_t* values, uint32_t* memories) = 0;
+  virtual void restore(const uint32_t* values, const uint32_t* memories) = 0;
+  virtual void set_force(uint32_t sig, uint32_t lane, int value) = 0;
+  virtual void clear_forces() = 0;
+  virtual void run(const RunSpec& spec) = 0;
+  // Results of the last run: digest (gate engine [64][words32] u32 bit-sliced,
+  // word engine [lanes] u64), trace, per-lane halt cycles.
+  virtual size_t digest_bytes() const = 0;
+  virtual void get_digest(void* out) = 0;
+  virtual size_t trace_bytes() const = 0;
+  virtual void get_trace(void* out) = 0;
+  virtual void get_halt_cycles(uint64_t* out) { (void)out; throw Error("E_HIP_UNSUPPORTED: engine has no halting"); }
+  virtual void synchronize() { RRTL_HIP_CHECK(hipStreamSynchronize(stream_)); }
+  const RunStats& last_stats() const { return stats_; }
+  hipStream_t stream() const { return stream_; }
+
+ protected:
+  Program P;
+  uint32_t lanes_;
+  EngineOptions opt_;
+  hipStream_t stream_ = nullptr;
+  RunStats stats_;
+};
+
+Engine* make_gate_engine(Program prog, uint32_t lanes, const EngineOptions& opt);
+Engine* make_word_engine(Program prog, uint32_t lanes, const EngineOptions& opt);
+
+}  // namespace rrtl

@@ -1,0 +1,913 @@
+// Bit-sliced gate engine: 32 lanes per u32 word (two words per reference u64),
+// the MI355X counterpart of rrtl_sim_ir::bitparallel::BitParallelSimulator
+// (crates/rrtl-sim-ir/src/bitparallel.rs:31-355).
+//
+// Semantics (BitParallelSimulator): settle = async-reset-comb then comb, each
+// packet's stores visible to later packets; tick = settle, capture tick_next with
+// the reset mux (a & rv) | (~a & v), commit, re-apply stuck-at forces, settle;
+// a forced (signal, lane) is re-clamped after every store of that signal and
+// after the commit (set_force/clamp_signal, :96-131).  On async-reset stores the
+// register keeps its value on lanes whose reset is not asserted (the reference's
+// :281-287 writes the literal there; PackedSimulator and the scalar Simulator
+// keep the register, and so does this engine).
+//
+// Device layout.  HBM holds every node row-major: node n's lanes are
+// state[n * nwords + w], one u32 per 32 lanes (node = signal index, then the two
+// constants, then compiler temporaries).  The timed path is `tile_kernel`: one
+// workgroup owns a tile of W u32 lane-words for the whole run, keeps only the
+// live values of the levelized gate list in LDS (slots reused by liveness, dead
+// logic dropped), generates its primary-input bits on chip from a counter hash,
+// folds observed outputs into a per-lane signature, and touches HBM only at the
+// start and end of the run.  Gates of one level run in parallel over
+// (gate, word) pairs; a workgroup barrier separates levels.
+#include "engine.hpp"
+#include <algorithm>
+#include <map>
+#include <unordered_set>
+#include <unordered_map>
+
+namespace rrtl {
+namespace {
+
+// 4-bit gate op: bit0 invert a, bit1 invert b, bit2 invert result, bit3 xor.
+// r = (x ? a ^ b : (a ^ ia) & (b ^ ib)) ^ io covers every 2-input gate kind.
+enum : uint32_t { GI_A = 1, GI_B = 2, GI_O = 4, GI_X = 8 };
+constexpr uint32_t G_AND = 0, G_OR = GI_A | GI_B | GI_O, G_XOR = GI_X, G_ANDN = GI_B, G_ORN = GI_A | GI_O;
+constexpr uint32_t G_NOT = GI_A;  // b = ones
+constexpr uint32_t G_COPY = 0;    // b = ones
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t CLAMP_COMMIT = 0xFFFFFFFEu;
+
+struct CGate { uint32_t op, a, b, dst; };
+struct CCommit { uint32_t reg, val, rsig, pol, rbit, has_reset; };
+
+__host__ __device__ inline uint32_t gate_eval(uint32_t op, uint32_t a, uint32_t b) {
+  uint32_t ia = 0u - (op & 1u), ib = 0u - ((op >> 1) & 1u), io = 0u - ((op >> 2) & 1u);
+  uint32_t r = (op & GI_X) ? (a ^ b) : ((a ^ ia) & (b ^ ib));
+  return r ^ io;
+}
+
+// ------------------------------------------------------------------ compile
+struct GateNet {
+  uint32_t nsig = 0, n_nodes = 0, c0 = 0, c1 = 0;
+  std::vector<CGate> settle;     // async-reset gates then comb gates, reference order
+  std::vector<CGate> next;       // tick_next value gates
+  std::vector<CCommit> commits;
+  std::vector<uint8_t> written;  // signal written by a settle gate
+};
+
+GateNet build_net(const Program& P) {
+  GateNet N;
+  N.nsig = uint32_t(P.signals.size());
+  N.c0 = N.nsig; N.c1 = N.nsig + 1;
+  uint32_t next_temp = N.nsig + 2;
+  auto lower_block = [&](const Block& bk, int kind, std::vector<CGate>& out) {
+    std::vector<uint32_t> node(bk.n_values, NONE), uses(bk.n_values, 0);
+    std::vector<int64_t> gate_of(bk.n_values, -1);
+    size_t k0 = 0;
+    for (size_t k = 0; k < bk.packets.size(); k++) {
+      if (bk.packets[k].effects.empty() && k + 1 < bk.packets.size()) continue;
+      std::unordered_set<uint32_t> read_sig;
+      for (size_t q = k0; q <= k; q++) {
+        for (const Instr& in : bk.packets[q].instrs) {
+          if (in.op == OP_SIGNAL) read_sig.insert(in.args[0]);
+          else if (in.op != OP_LIT) for (uint32_t a : in.args) uses[a]++;
+        }
+        for (const Effect& ef : bk.packets[q].effects) uses[ef.value]++;
+      }
+      const size_t gstart = out.size();
+      auto temp = [&]() { return next_temp++; };
+      for (size_t q = k0; q <= k; q++) {
+        for (const Instr& in : bk.packets[q].instrs) {
+          switch (in.op) {
+            case OP_SIGNAL: node[in.dst] = in.args[0]; break;
+            case OP_LIT: node[in.dst] = (!in.args.empty() && (in.args[0] & 1)) ? N.c1 : N.c0; break;
+            case OP_NOT: {
+              int64_t g = gate_of[in.args[0]];
+              if (g >= int64_t(gstart) && uses[in.args[0]] == 1 && out[g].b != N.c1) {
+                out[g].op ^= GI_O;
+                node[in.dst] = out[g].dst; gate_of[in.dst] = g;
+              } else {
+                uint32_t t = temp();
+                out.push_back({G_NOT, node[in.args[0]], N.c1, t});
+                node[in.dst] = t; gate_of[in.dst] = int64_t(out.size()) - 1;
+              }
+              break;
+            }
+            case OP_AND: case OP_OR: case OP_XOR: {
+              uint32_t t = temp();
+              out.push_back({in.op == OP_AND ? G_AND : in.op == OP_OR ? G_OR : G_XOR, node[in.args[0]], node[in.args[1]], t});
+              node[in.dst] = t; gate_of[in.dst] = int64_t(out.size()) - 1;
+              break;
+            }
+            case OP_MUX: {  // (c & t) | (e & ~c)
+              uint32_t c = node[in.args[0]], tv = node[in.args[1]], ev = node[in.args[2]];
+              uint32_t t1 = temp(), t2 = temp(), t3 = temp();
+              out.push_back({G_AND, c, tv, t1});
+              out.push_back({G_ANDN, ev, c, t2});
+              out.push_back({G_OR, t1, t2, t3});
+              node[in.dst] = t3; gate_of[in.dst] = int64_t(out.size()) - 1;
+              break;
+            }
+            default: throw Error("E_BITPARALLEL: op is not a gate-level bitwise op");
+          }
+        }
+      }
+      for (const Effect& ef : bk.packets[k].effects) {
+        if (kind != 2 && ef.kind == EF_STORE) {
+          int64_t g = gate_of[ef.value];
+          bool is_temp = node[ef.value] >= N.nsig + 2;
+          if (g >= int64_t(gstart) && is_temp && uses[ef.value] == 1 && !read_sig.count(ef.dst)) out[g].dst = ef.dst;
+          else out.push_back({G_COPY, node[ef.value], N.c1, ef.dst});
+        } else if (kind == 0 && ef.kind == EF_CAPTURE) {
+          if (!ef.has_reset || ef.reset.kind != 1) continue;
+          // reg = asserted ? value : reg, asserted = rsig (active high) or ~rsig
+          uint32_t r = ef.dst, v = node[ef.value], rs = ef.reset.signal;
+          uint32_t t1 = temp(), t2 = temp();
+          if (ef.reset.polarity == 0) { out.push_back({G_AND, rs, v, t1}); out.push_back({G_ANDN, r, rs, t2}); }
+          else { out.push_back({G_ANDN, v, rs, t1}); out.push_back({G_AND, r, rs, t2}); }
+          out.push_back({G_OR, t1, t2, r});
+        } else if (kind == 2 && ef.kind == EF_CAPTURE) {
+          uint32_t rbit = (!ef.reset.value.empty() && (ef.reset.value[0] & 1)) ? 1u : 0u;
+          N.commits.push_back({ef.dst, node[ef.value], ef.has_reset ? ef.reset.signal : NONE, ef.reset.polarity, rbit,
+                               ef.has_reset ? 1u : 0u});
+        }
+      }
+      for (size_t q = k0; q <= k; q++) for (const Instr& in : bk.packets[q].instrs) uses[in.dst] = 0;
+      k0 = k + 1;
+    }
+  };
+  lower_block(P.blocks[B_ASYNC], 0, N.settle);
+  lower_block(P.blocks[B_COMB], 1, N.settle);
+  lower_block(P.blocks[B_NEXT], 2, N.next);
+  N.n_nodes = next_temp;
+  N.written.assign(N.nsig, 0);
+  // Read-before-write inside one settle would make the result depend on the
+  // packet order (a combinational cycle): the levelized engines reject it.
+  std::vector<int64_t> first_w(N.n_nodes, -1);
+  for (size_t i = 0; i < N.settle.size(); i++) {
+    uint32_t d = N.settle[i].dst;
+    if (d < N.nsig) {
+      if (first_w[d] >= 0 && P.signals[d].kind != K_REG) throw Error("E_HIP_COMB_ORDER: signal `" + P.signals[d].name + "` is driven twice");
+      if (first_w[d] < 0) first_w[d] = int64_t(i);
+      N.written[d] = 1;
+    }
+  }
+  for (size_t i = 0; i < N.settle.size(); i++)
+    for (uint32_t s : {N.settle[i].a, N.settle[i].b})
+      if (s < N.nsig && first_w[s] > int64_t(i))
+        throw Error("E_HIP_COMB_ORDER: signal `" + P.signals[s].name + "` is read before it is driven (combinational cycle)");
+  return N;
+}
+
+// One compiled tile program: levels of gates over LDS slots.
+struct TileProgram {
+  std::vector<uint32_t> desc;          // 2 words per gate: a | b<<16, dst | op<<16
+  std::vector<uint32_t> level_start;   // gate ranges per level (cycle levels, then post levels)
+  uint32_t n_cycle_levels = 0, n_post_levels = 0;
+  std::vector<uint32_t> load_slot, load_node;     // persistent slots loaded at start, stored at end
+  std::vector<uint32_t> stim_slot, stim_mode, stim_stream;
+  std::vector<uint64_t> stim_arg;
+  std::vector<uint32_t> commit;        // 4 words: reg_slot, val_slot, rsig_slot|NONE, flags(pol | rbit<<1 | has<<2)
+  std::vector<uint32_t> obs_slot;
+  uint32_t n_slots = 0, c0 = 0, c1 = 0;
+  // per node: persistent slot, or NONE
+  std::unordered_map<uint32_t, uint32_t> persist;
+  // forced-node clamp positions: node -> (timeline level, slot) for each instance
+  std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> clamp_sites;
+};
+
+struct Timeline {
+  // positions: cycle gate levels 1..L, PRE observe L+1, commit L+2, post levels L+3.., POST observe after
+  uint32_t L = 0, L2 = 0;
+};
+
+TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec, const std::vector<uint8_t>& forced) {
+  TileProgram T;
+  const uint32_t nsig = N.nsig;
+  // writer gate per node (within settle + next), levels
+  std::vector<const CGate*> gl;
+  for (auto& g : N.settle) gl.push_back(&g);
+  for (auto& g : N.next) gl.push_back(&g);
+  std::vector<int64_t> writer(N.n_nodes, -1);
+  std::vector<uint32_t> lvl(gl.size(), 0);
+  for (size_t i = 0; i < gl.size(); i++) {
+    uint32_t l = 0;
+    for (uint32_t s : {gl[i]->a, gl[i]->b}) if (writer[s] >= 0) l = std::max(l, lvl[writer[s]]);
+    // a gate that rewrites a register (async reset) also orders after earlier readers of it
+    lvl[i] = l + 1;
+    writer[gl[i]->dst] = int64_t(i);
+  }
+  // async-reset rewrites: every comb reader of that register must see the new
+  // value; readers after the rewrite already depend on it through `writer`.
+  std::vector<uint8_t> is_obs(N.n_nodes, 0);
+  for (uint32_t s : spec.obs_sig) is_obs[s] = 1;
+  // liveness roots
+  std::vector<uint8_t> need(gl.size(), 0);
+  std::vector<uint32_t> stack;
+  auto mark_node = [&](uint32_t n) { if (writer[n] >= 0 && !need[writer[n]]) { need[writer[n]] = 1; stack.push_back(uint32_t(writer[n])); } };
+  for (auto& c : N.commits) { mark_node(c.val); if (c.rsig != NONE) mark_node(c.rsig); }
+  if (!spec.observe_post) for (uint32_t s : spec.obs_sig) mark_node(s);
+  for (uint32_t s = 0; s < nsig; s++) if (forced[s] && P.signals[s].kind == K_REG) mark_node(s);
+  while (!stack.empty()) {
+    uint32_t i = stack.back(); stack.pop_back();
+    mark_node(gl[i]->a); mark_node(gl[i]->b);
+    // an async-reset rewrite of a register reads the register's prior writer too
+  }
+  // compact levels among needed gates
+  uint32_t L = 0;
+  for (size_t i = 0; i < gl.size(); i++) if (need[i]) L = std::max(L, lvl[i]);
+  // post program: cone of observed signals, re-evaluated after the commit
+  std::vector<uint8_t> need2(gl.size(), 0);
+  uint32_t L2 = 0;
+  if (spec.observe_post) {
+    std::vector<uint32_t> st2;
+    auto mark2 = [&](uint32_t n) { if (writer[n] >= 0 && size_t(writer[n]) < N.settle.size() && !need2[writer[n]]) { need2[writer[n]] = 1; st2.push_back(uint32_t(writer[n])); } };
+    for (uint32_t s : spec.obs_sig) mark2(s);
+    while (!st2.empty()) { uint32_t i = st2.back(); st2.pop_back(); mark2(gl[i]->a); mark2(gl[i]->b); }
+    for (size_t i = 0; i < N.settle.size(); i++) if (need2[i]) L2 = std::max(L2, lvl[i]);
+  }
+  // persistent nodes: signals not written by the settle (inputs, registers,
+  // host-driven wires) that are read, committed, observed or stimulated
+  std::vector<uint8_t> pers(N.n_nodes, 0);
+  auto want_persistent = [&](uint32_t n) { if (n < nsig && (!N.written[n] || P.signals[n].kind == K_REG)) pers[n] = 1; };
+  for (size_t i = 0; i < gl.size(); i++) if (need[i] || need2[i]) { want_persistent(gl[i]->a); want_persistent(gl[i]->b); want_persistent(gl[i]->dst); }
+  for (auto& c : N.commits) { pers[c.reg] = 1; want_persistent(c.val); if (c.rsig != NONE) want_persistent(c.rsig); }
+  for (uint32_t s : spec.obs_sig) want_persistent(s);
+  for (uint32_t s : spec.in_sig) pers[s] = 1;
+  // slot 0 / 1: constants
+  T.c0 = 0; T.c1 = 1;
+  uint32_t next_slot = 2;
+  for (uint32_t n = 0; n < nsig; n++) if (pers[n]) T.persist[n] = next_slot++;
+  const uint32_t first_free = next_slot;
+  // linear-scan allocation of gate outputs over the timeline
+  const uint32_t POS_OBS_PRE = L + 1, POS_COMMIT = L + 2, POS_POST0 = L + 2;
+  struct Inst { uint32_t gate; uint32_t pos; uint32_t slot; };
+  std::vector<Inst> insts;
+  std::vector<uint32_t> inst_of_cycle(gl.size(), NONE), inst_of_post(gl.size(), NONE);
+  for (size_t i = 0; i < gl.size(); i++) if (need[i]) { inst_of_cycle[i] = uint32_t(insts.size()); insts.push_back({uint32_t(i), lvl[i], NONE}); }
+  for (size_t i = 0; i < gl.size(); i++) if (need2[i]) { inst_of_post[i] = uint32_t(insts.size()); insts.push_back({uint32_t(i), POS_POST0 + lvl[i], NONE}); }
+  // last use of each instance
+  std::vector<uint32_t> last(insts.size(), 0);
+  auto src_inst = [&](uint32_t node, bool post) -> uint32_t {
+    if (writer[node] < 0) return NONE;
+    return post ? inst_of_post[writer[node]] : inst_of_cycle[writer[node]];
+  };
+  for (auto& in : insts) {
+    bool post = in.pos > POS_POST0;
+    for (uint32_t s : {gl[in.gate]->a, gl[in.gate]->b}) {
+      if (pers.size() > s && pers[s]) continue;
+      uint32_t si = src_inst(s, post);
+      if (si != NONE) last[si] = std::max(last[si], in.pos);
+    }
+  }
+  for (auto& c : N.commits) {
+    for (uint32_t s : {c.val, c.rsig}) {
+      if (s == NONE || pers[s]) continue;
+      uint32_t si = src_inst(s, false);
+      if (si != NONE) last[si] = std::max(last[si], POS_COMMIT);
+    }
+  }
+  for (uint32_t s : spec.obs_sig) {
+    if (pers[s]) continue;
+    uint32_t si = src_inst(s, spec.observe_post);
+    if (si != NONE) last[si] = std::max(last[si], spec.observe_post ? POS_POST0 + L2 + 1 : POS_OBS_PRE);
+  }
+  // allocate: process instances by position; free slots whose last use < position
+  std::vector<uint32_t> order(insts.size());
+  for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return insts[x].pos < insts[y].pos; });
+  std::multimap<uint32_t, uint32_t> releases;  // last-use pos -> slot
+  std::vector<uint32_t> free_slots;
+  uint32_t max_slot = first_free;
+  for (uint32_t ii : order) {
+    Inst& in = insts[ii];
+    while (!releases.empty() && releases.begin()->first < in.pos) { free_slots.push_back(releases.begin()->second); releases.erase(releases.begin()); }
+    uint32_t d = gl[in.gate]->dst;
+    if (d < nsig && pers[d]) { in.slot = T.persist.at(d); continue; }
+    if (!free_slots.empty()) { in.slot = free_slots.back(); free_slots.pop_back(); }
+    else in.slot = max_slot++;
+    releases.emplace(std::max(last[ii], in.pos), in.slot);
+  }
+  T.n_slots = max_slot;
+  if (T.n_slots > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
+  auto slot_of = [&](uint32_t node, bool post) -> uint32_t {
+    if (node == N.c0) return T.c0;
+    if (node == N.c1) return T.c1;
+    if (node < nsig && pers[node]) return T.persist.at(node);
+    uint32_t si = src_inst(node, post);
+    if (si == NONE) throw Error("E_HIP_INTERNAL: node has no producer in the tile program");
+    return insts[si].slot;
+  };
+  // emit levels: cycle levels 1..L (plus one staging level when needed), then post levels
+  std::vector<std::vector<uint32_t>> lv_desc(L + 1), lv2_desc(L2 + 1);
+  std::vector<std::vector<uint32_t>> by_level(L + 1), by_level2(L2 + 1);
+  for (uint32_t ii = 0; ii < insts.size(); ii++) {
+    if (insts[ii].pos <= L) by_level[insts[ii].pos].push_back(ii);
+    else by_level2[insts[ii].pos - POS_POST0].push_back(ii);
+  }
+  std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> forced_cycle, forced_post;  // (signal, (level, slot))
+  auto emit = [&](const std::vector<uint32_t>& v, bool post, std::vector<uint32_t>& out, uint32_t level) {
+    for (uint32_t ii : v) {
+      const CGate& g = *gl[insts[ii].gate];
+      uint32_t a = slot_of(g.a, post), b = slot_of(g.b, post), d = insts[ii].slot;
+      out.push_back(a | (b << 16));
+      out.push_back(d | (g.op << 16));
+      if (g.dst < nsig && forced[g.dst]) (post ? forced_post : forced_cycle).push_back({g.dst, {level, d}});
+    }
+  };
+  for (uint32_t l = 1; l <= L; l++) emit(by_level[l], false, lv_desc[l], l);
+  for (uint32_t l = 1; l <= L2; l++) emit(by_level2[l], true, lv2_desc[l], l);
+  // commits: a value or reset signal that is itself a committed register would
+  // be overwritten mid-commit; stage it through a copy in an extra level that
+  // runs after every cycle level.
+  std::unordered_set<uint32_t> committed;
+  for (auto& c : N.commits) committed.insert(c.reg);
+  std::vector<uint32_t> staging;
+  for (auto& c : N.commits) {
+    uint32_t vs = slot_of(c.val, false);
+    uint32_t rs = c.rsig == NONE ? NONE : slot_of(c.rsig, false);
+    auto stage = [&](uint32_t slot) {
+      uint32_t t = T.n_slots++;
+      staging.push_back(slot | (T.c1 << 16));
+      staging.push_back(t | (G_COPY << 16));
+      return t;
+    };
+    if (committed.count(c.val)) vs = stage(vs);
+    if (c.rsig != NONE && committed.count(c.rsig)) rs = stage(rs);
+    T.commit.insert(T.commit.end(), {T.persist.at(c.reg), vs, rs == NONE ? 0u : rs, c.pol | (c.rbit << 1) | (c.has_reset << 2)});
+  }
+  if (T.n_slots > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
+  uint32_t n_cycle = L + (staging.empty() ? 0 : 1);
+  T.level_start.push_back(0);
+  auto flush = [&](const std::vector<uint32_t>& d) {
+    T.desc.insert(T.desc.end(), d.begin(), d.end());
+    T.level_start.push_back(uint32_t(T.desc.size() / 2));
+  };
+  for (uint32_t l = 1; l <= L; l++) flush(lv_desc[l]);
+  if (!staging.empty()) flush(staging);
+  for (uint32_t l = 1; l <= L2; l++) flush(lv2_desc[l]);
+  T.n_cycle_levels = n_cycle; T.n_post_levels = L2;
+  // clamp sites in kernel numbering: cycle level l -> l, post level l -> n_cycle + l,
+  // commit -> CLAMP_COMMIT
+  for (auto& fc : forced_cycle) T.clamp_sites.push_back(fc);
+  for (auto& fp : forced_post) T.clamp_sites.push_back({fp.first, {n_cycle + fp.second.first, fp.second.second}});
+  // persistent load/store lists
+  for (auto& kv : T.persist) {
+    T.load_slot.push_back(kv.second); T.load_node.push_back(kv.first);
+    if (P.signals[kv.first].kind == K_REG && forced[kv.first]) T.clamp_sites.push_back({kv.first, {CLAMP_COMMIT, kv.second}});
+  }
+  // stimulus
+  for (size_t i = 0; i < spec.in_sig.size(); i++) {
+    if (spec.in_mode[i] == IN_HOLD) continue;
+    if (spec.in_mode[i] == IN_TRACE) throw Error("E_HIP_UNSUPPORTED: trace-driven inputs are not supported by the gate engine run");
+    T.stim_slot.push_back(T.persist.at(spec.in_sig[i]));
+    T.stim_mode.push_back(spec.in_mode[i]);
+    T.stim_stream.push_back(spec.in_sig[i] << 8);
+    T.stim_arg.push_back(spec.in_arg.empty() ? 0 : spec.in_arg[i]);
+  }
+  for (uint32_t s : spec.obs_sig) T.obs_slot.push_back(slot_of(s, spec.observe_post));
+  return T;
+}
+
+
+// ------------------------------------------------------------------ kernels
+struct TileArgs {
+  uint32_t* state; uint64_t nwords;             // HBM rows [node][nwords]
+  const uint2* desc; const uint32_t* level_start;
+  uint32_t n_cycle_levels, n_post_levels;
+  const uint32_t* load_slot; const uint32_t* load_node; uint32_t n_load;
+  const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
+  const uint4* commit; uint32_t n_commit;
+  const uint32_t* obs_slot; uint32_t n_obs;
+  const uint32_t* force; const uint32_t* force_start;  // per tile: 5-word entries (pos, slot, word, and, or)
+  uint32_t* digest;                                    // [64][nwords] or null
+  uint32_t* trace;                                     // [cycles][n_obs][nwords] or null
+  uint64_t seed, cycle_base, cycles;
+  uint32_t n_slots, c0, c1, observe_post;
+};
+
+constexpr uint32_t TILE_THREADS = 256;
+
+template <int W>
+__device__ __forceinline__ void run_level(const TileArgs& A, uint32_t* lds, uint32_t L) {
+  const uint32_t gb = A.level_start[L], ge = A.level_start[L + 1];
+  for (uint32_t p = gb * W + threadIdx.x; p < ge * W; p += TILE_THREADS) {
+    const uint32_t g = p / W, w = p % W;
+    const uint2 d = A.desc[g];
+    const uint32_t a = lds[(d.x & 0xFFFFu) * W + w], b = lds[(d.x >> 16) * W + w];
+    lds[(d.y & 0xFFFFu) * W + w] = gate_eval(d.y >> 16, a, b);
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void apply_clamps(const TileArgs& A, uint32_t* lds, uint32_t pos, uint32_t fb, uint32_t fe) {
+  for (uint32_t i = fb + threadIdx.x; i < fe; i += TILE_THREADS) {
+    const uint32_t* f = &A.force[size_t(i) * 5];
+    if (f[0] != pos) continue;
+    uint32_t* v = &lds[f[1] * W + f[2]];
+    *v = (*v & f[3]) | f[4];
+  }
+}
+
+// Signature of the observed outputs: per cycle c, S[(j + c) & 63] ^= v_j for every
+// observed word v_j, then one nonlinear round S'[p] = S[p] ^ (S[p+17] & ~S[p+41]).
+// Bitwise, so every lane keeps its own 64-bit signature.
+template <int W>
+__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32_t* sig, uint64_t c, uint64_t cyc, uint64_t w0) {
+  const uint32_t tid = threadIdx.x;
+  if (A.trace)
+    for (uint32_t i = tid; i < A.n_obs * W; i += TILE_THREADS) {
+      uint32_t j = i / W, w = i % W;
+      uint64_t gw = w0 + w;
+      if (gw < A.nwords) A.trace[(c * A.n_obs + j) * A.nwords + gw] = lds[A.obs_slot[j] * W + w];
+    }
+  if (A.digest) {
+    constexpr uint32_t PER = (64 * W + TILE_THREADS - 1) / TILE_THREADS;
+    for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
+      uint32_t p = i / W, w = i % W, x = sig[i];
+      for (uint32_t j = (p - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) x ^= lds[A.obs_slot[j] * W + w];
+      sig[i] = x;
+    }
+    __syncthreads();
+    uint32_t nv[PER];
+#pragma unroll
+    for (uint32_t k = 0; k < PER; k++) {
+      uint32_t i = tid + k * TILE_THREADS;
+      if (i < 64 * W) {
+        uint32_t p = i / W, w = i % W;
+        nv[k] = sig[i] ^ (sig[((p + 17) & 63) * W + w] & ~sig[((p + 41) & 63) * W + w]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < PER; k++) {
+      uint32_t i = tid + k * TILE_THREADS;
+      if (i < 64 * W) sig[i] = nv[k];
+    }
+  }
+  __syncthreads();
+}
+
+template <int W>
+__global__ void __launch_bounds__(TILE_THREADS) tile_kernel(TileArgs A) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* sig = lds + size_t(A.n_slots) * W;  // 64 x W signature words
+  const uint32_t tid = threadIdx.x;
+  const uint64_t w0 = uint64_t(blockIdx.x) * W;
+  const uint32_t fb = A.force_start ? A.force_start[blockIdx.x] : 0, fe = A.force_start ? A.force_start[blockIdx.x + 1] : 0;
+  for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
+    uint32_t s = i / W, w = i % W;
+    uint64_t gw = w0 + w;
+    lds[A.load_slot[s] * W + w] = gw < A.nwords ? A.state[A.load_node[s] * A.nwords + gw] : 0u;
+  }
+  for (uint32_t w = tid; w < W; w += TILE_THREADS) { lds[A.c0 * W + w] = 0u; lds[A.c1 * W + w] = ~0u; }
+  if (A.digest)
+    for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
+      uint32_t p = i / W, w = i % W;
+      uint64_t gw = w0 + w;
+      sig[i] = gw < A.nwords ? A.digest[p * A.nwords + gw] : 0u;
+    }
+  __syncthreads();
+  for (uint64_t c = 0; c < A.cycles; c++) {
+    const uint64_t cyc = A.cycle_base + c;
+    for (uint32_t i = tid; i < A.n_stim * W; i += TILE_THREADS) {
+      uint32_t s = i / W, w = i % W;
+      uint64_t gw = w0 + w, arg = A.stim_arg[s];
+      uint32_t v;
+      if (A.stim_mode[s] == IN_RANDOM) {
+        uint64_t word = stim_from_key(stim_key(A.seed, A.stim_stream[s], cyc), gw >> 1);
+        v = uint32_t(word >> (32 * (gw & 1)));
+      } else if (A.stim_mode[s] == IN_CONST) {
+        v = (arg & 1) ? ~0u : 0u;
+      } else {  // IN_PULSE
+        v = ((cyc < (arg & 0xFFFFFFFFull)) ? (arg >> 32) & 1 : (arg >> 33) & 1) ? ~0u : 0u;
+      }
+      lds[A.stim_slot[s] * W + w] = v;
+    }
+    __syncthreads();
+    for (uint32_t L = 0; L < A.n_cycle_levels; L++) {
+      run_level<W>(A, lds, L);
+      __syncthreads();
+      if (fe > fb) { apply_clamps<W>(A, lds, L + 1, fb, fe); __syncthreads(); }
+    }
+    if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+    for (uint32_t i = tid; i < A.n_commit * W; i += TILE_THREADS) {
+      uint32_t k = i / W, w = i % W;
+      const uint4 cm = A.commit[k];
+      uint32_t v = lds[cm.y * W + w];
+      if (cm.w & 4) {
+        uint32_t a = lds[cm.z * W + w];
+        if (cm.w & 1) a = ~a;
+        uint32_t rv = (cm.w & 2) ? ~0u : 0u;
+        v = (a & rv) | (~a & v);
+      }
+      lds[cm.x * W + w] = v;
+    }
+    __syncthreads();
+    if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
+    if (A.observe_post) {
+      for (uint32_t L = A.n_cycle_levels; L < A.n_cycle_levels + A.n_post_levels; L++) {
+        run_level<W>(A, lds, L);
+        __syncthreads();
+        if (fe > fb) { apply_clamps<W>(A, lds, L + 1, fb, fe); __syncthreads(); }
+      }
+      if (A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+    }
+  }
+  for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
+    uint32_t s = i / W, w = i % W;
+    uint64_t gw = w0 + w;
+    if (gw < A.nwords) A.state[A.load_node[s] * A.nwords + gw] = lds[A.load_slot[s] * W + w];
+  }
+  if (A.digest)
+    for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
+      uint32_t p = i / W, w = i % W;
+      uint64_t gw = w0 + w;
+      if (gw < A.nwords) A.digest[p * A.nwords + gw] = sig[i];
+    }
+}
+
+// Exact, unhurried kernels for the API-level operations (eval_combinational,
+// tick, tick_from_evaluated): one thread per u32 lane-word walks the whole gate
+// list in reference order over the HBM rows, clamping forced nodes as it goes.
+struct HbmArgs {
+  uint32_t* state; uint64_t nwords;
+  const uint4* gates; uint32_t n_gates;     // (op, a, b, dst) node ids
+  const uint4* commits; uint32_t n_commits; uint32_t stage_row;
+  const uint32_t* fstart; const uint32_t* fent;  // per node: entries (word, and, or)
+  uint32_t n_nodes;
+};
+
+__device__ __forceinline__ void hbm_clamp(const HbmArgs& A, uint32_t node, uint64_t w) {
+  if (!A.fstart) return;
+  for (uint32_t i = A.fstart[node]; i < A.fstart[node + 1]; i++) {
+    const uint32_t* f = &A.fent[size_t(i) * 3];
+    if (f[0] != w) continue;
+    uint32_t* v = &A.state[size_t(node) * A.nwords + w];
+    *v = (*v & f[1]) | f[2];
+  }
+}
+
+__global__ void hbm_gates_kernel(HbmArgs A) {
+  const uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (w >= A.nwords) return;
+  for (uint32_t i = 0; i < A.n_gates; i++) {
+    const uint4 g = A.gates[i];
+    uint32_t a = A.state[size_t(g.y) * A.nwords + w], b = A.state[size_t(g.z) * A.nwords + w];
+    A.state[size_t(g.w) * A.nwords + w] = gate_eval(g.x, a, b);
+    hbm_clamp(A, g.w, w);
+  }
+}
+
+__global__ void hbm_commit_kernel(HbmArgs A) {
+  const uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (w >= A.nwords) return;
+  for (uint32_t k = 0; k < A.n_commits; k++) {
+    const uint4 cm = A.commits[k];  // reg, val, rsig, flags
+    uint32_t v = A.state[size_t(cm.y) * A.nwords + w];
+    if (cm.w & 4) {
+      uint32_t a = A.state[size_t(cm.z) * A.nwords + w];
+      if (cm.w & 1) a = ~a;
+      uint32_t rv = (cm.w & 2) ? ~0u : 0u;
+      v = (a & rv) | (~a & v);
+    }
+    A.state[size_t(A.stage_row + k) * A.nwords + w] = v;
+  }
+  for (uint32_t k = 0; k < A.n_commits; k++)
+    A.state[size_t(A.commits[k].x) * A.nwords + w] = A.state[size_t(A.stage_row + k) * A.nwords + w];
+  // apply_forces: every forced node is re-clamped after the commit
+  if (A.fstart)
+    for (uint32_t n = 0; n < A.n_nodes; n++)
+      if (A.fstart[n + 1] != A.fstart[n]) hbm_clamp(A, n, w);
+}
+
+// ------------------------------------------------------------------ engine
+struct Force { uint32_t sig, lane, value; };
+
+class GateEngine final : public Engine {
+ public:
+  GateEngine(Program prog, uint32_t lanes, const EngineOptions& opt) : Engine(std::move(prog), lanes, opt) {
+    std::string why;
+    if (!P.is_gate_level(&why)) throw Error("E_BITPARALLEL: " + why);
+    N = build_net(P);
+    nwords_ = (uint64_t(lanes) + 31) / 32;
+    RRTL_HIP_CHECK(hipSetDevice(opt.device));
+    hipDeviceProp_t prop;
+    RRTL_HIP_CHECK(hipGetDeviceProperties(&prop, opt.device));
+    lds_max_ = prop.sharedMemPerBlock;
+    RRTL_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stage_row_ = N.n_nodes;
+    rows_ = N.n_nodes + uint32_t(N.commits.size());
+    state_.alloc(size_t(rows_) * nwords_);
+    state_.zero(stream_);
+    RRTL_HIP_CHECK(hipMemsetAsync(state_.p + size_t(N.c1) * nwords_, 0xFF, nwords_ * 4, stream_));
+    std::vector<uint4> sg, ng, cm;
+    for (auto& g : N.settle) sg.push_back(make_uint4(g.op, g.a, g.b, g.dst));
+    for (auto& g : N.next) ng.push_back(make_uint4(g.op, g.a, g.b, g.dst));
+    for (auto& c : N.commits) cm.push_back(make_uint4(c.reg, c.val, c.rsig == NONE ? 0 : c.rsig, c.pol | (c.rbit << 1) | (c.has_reset << 2)));
+    settle_g_.upload(sg, stream_); next_g_.upload(ng, stream_); commit_g_.upload(cm, stream_);
+    RRTL_HIP_CHECK(hipEventCreate(&ev0_)); RRTL_HIP_CHECK(hipEventCreate(&ev1_));
+    eval_combinational();
+    synchronize();
+  }
+  ~GateEngine() override {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    state_.release(); settle_g_.release(); next_g_.release(); commit_g_.release();
+    fstart_.release(); fent_.release(); tile_force_.release(); tile_fstart_.release(); digest_.release(); trace_.release();
+    for (auto& b : prog_bufs_) b.release();
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+  uint32_t kind() const override { return ENGINE_GATE; }
+
+  void set_signal_limbs(uint32_t sig, const uint32_t* lane_limbs, bool settle) override {
+    check_sig(sig);
+    refresh_if_stale();
+    std::vector<uint32_t> row(nwords_, 0);
+    for (uint32_t l = 0; l < lanes_; l++) if (lane_limbs[l] & 1) row[l / 32] |= 1u << (l % 32);
+    RRTL_HIP_CHECK(hipMemcpyAsync(state_.p + size_t(sig) * nwords_, row.data(), nwords_ * 4, hipMemcpyHostToDevice, stream_));
+    synchronize();
+    if (settle) eval_combinational();
+  }
+  void get_signal_limbs(uint32_t sig, uint32_t* out) override {
+    check_sig(sig);
+    refresh_if_stale();
+    std::vector<uint32_t> row(nwords_);
+    RRTL_HIP_CHECK(hipMemcpyAsync(row.data(), state_.p + size_t(sig) * nwords_, nwords_ * 4, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    for (uint32_t l = 0; l < lanes_; l++) out[l] = (row[l / 32] >> (l % 32)) & 1u;
+  }
+  void get_signal_words(uint32_t sig, uint32_t* out) {
+    check_sig(sig);
+    refresh_if_stale();
+    RRTL_HIP_CHECK(hipMemcpyAsync(out, state_.p + size_t(sig) * nwords_, nwords_ * 4, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+  }
+  void set_memory_limbs(uint32_t, const uint32_t*, bool) override { throw Error("E_BITPARALLEL: the gate engine has no memories"); }
+  void get_memory_limbs(uint32_t, uint32_t*) override { throw Error("E_BITPARALLEL: the gate engine has no memories"); }
+
+  void eval_combinational() override {
+    launch_hbm_gates(settle_g_, uint32_t(N.settle.size()));
+    wires_stale_ = false;
+  }
+  void tick() override { eval_combinational(); edge(); eval_combinational(); }
+  void tick_from_evaluated() override { refresh_if_stale(); edge(); }
+  void tick_many(uint64_t steps) override {
+    if (!steps) return;
+    RunSpec s;
+    s.cycles = steps;
+    run_impl(s, false);
+  }
+  void run(const RunSpec& spec) override { run_impl(spec, true); }
+
+  void snapshot(uint32_t* values, uint32_t*) override {
+    refresh_if_stale();
+    std::vector<uint32_t> rows(size_t(N.nsig) * nwords_);
+    RRTL_HIP_CHECK(hipMemcpyAsync(rows.data(), state_.p, rows.size() * 4, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    for (uint32_t s = 0; s < N.nsig; s++)
+      for (uint32_t l = 0; l < lanes_; l++)
+        values[size_t(P.signals[s].offset) * lanes_ + l] = (rows[size_t(s) * nwords_ + l / 32] >> (l % 32)) & 1u;
+  }
+  void restore(const uint32_t* values, const uint32_t*) override {
+    std::vector<uint32_t> rows(size_t(N.nsig) * nwords_, 0);
+    for (uint32_t s = 0; s < N.nsig; s++)
+      for (uint32_t l = 0; l < lanes_; l++)
+        if (values[size_t(P.signals[s].offset) * lanes_ + l] & 1) rows[size_t(s) * nwords_ + l / 32] |= 1u << (l % 32);
+    RRTL_HIP_CHECK(hipMemcpyAsync(state_.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, stream_));
+    synchronize();
+    wires_stale_ = false;
+  }
+  void set_force(uint32_t sig, uint32_t lane, int value) override {
+    check_sig(sig);
+    if (lane >= lanes_) throw Error("E_SIM_IR_LANE: lane is out of range");
+    if (P.signals[sig].kind == K_INPUT) throw Error("E_HIP_FORCE_INPUT: inputs are driven by the caller; force the net they feed instead");
+    forces_.push_back({sig, lane, value ? 1u : 0u});
+    forces_dirty_ = true;
+  }
+  void clear_forces() override { forces_.clear(); forces_dirty_ = true; }
+
+  size_t digest_bytes() const override { return has_digest_ ? size_t(64) * nwords_ * 4 : 0; }
+  void get_digest(void* out) override {
+    if (!has_digest_) throw Error("E_HIP_NO_RUN: no run has produced a digest");
+    RRTL_HIP_CHECK(hipMemcpyAsync(out, digest_.p, digest_bytes(), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+  }
+  size_t trace_bytes() const override { return trace_bytes_; }
+  void get_trace(void* out) override {
+    if (!trace_bytes_) throw Error("E_HIP_NO_RUN: no run has produced a trace");
+    RRTL_HIP_CHECK(hipMemcpyAsync(out, trace_.p, trace_bytes_, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+  }
+  uint32_t tile_words() const { return W_; }
+  uint32_t tile_slots() const { return last_slots_; }
+  uint32_t tile_levels() const { return last_levels_; }
+  uint32_t tile_gates() const { return last_gates_; }
+  uint64_t nwords() const { return nwords_; }
+
+ private:
+  void check_sig(uint32_t sig) const { if (sig >= N.nsig) throw Error("E_SIM_IR_SIGNAL: signal index out of range"); }
+
+  void refresh_if_stale() { if (wires_stale_) eval_combinational(); }
+
+  HbmArgs hbm_args() {
+    sync_forces();
+    HbmArgs A{};
+    A.state = state_.p; A.nwords = nwords_;
+    A.commits = commit_g_.p; A.n_commits = uint32_t(N.commits.size()); A.stage_row = stage_row_;
+    A.fstart = forces_.empty() ? nullptr : fstart_.p; A.fent = fent_.p; A.n_nodes = N.n_nodes;
+    return A;
+  }
+  void launch_hbm_gates(DevBuf<uint4>& g, uint32_t n) {
+    HbmArgs A = hbm_args();
+    A.gates = g.p; A.n_gates = n;
+    uint32_t bs = 64, grid = uint32_t((nwords_ + bs - 1) / bs);
+    if (n) hipLaunchKernelGGL(hbm_gates_kernel, dim3(grid), dim3(bs), 0, stream_, A);
+    RRTL_HIP_CHECK(hipGetLastError());
+    synchronize();
+  }
+  void edge() {
+    launch_hbm_gates(next_g_, uint32_t(N.next.size()));
+    HbmArgs A = hbm_args();
+    uint32_t bs = 64, grid = uint32_t((nwords_ + bs - 1) / bs);
+    hipLaunchKernelGGL(hbm_commit_kernel, dim3(grid), dim3(bs), 0, stream_, A);
+    RRTL_HIP_CHECK(hipGetLastError());
+    synchronize();
+  }
+
+  // per-node force lists for the HBM kernels
+  void sync_forces() {
+    if (!forces_dirty_) return;
+    std::vector<uint32_t> start(N.n_nodes + 1, 0);
+    std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> m;  // (node, word) -> (and, or)
+    for (auto& f : forces_) {
+      auto key = std::make_pair(f.sig, f.lane / 32);
+      auto it = m.find(key);
+      if (it == m.end()) it = m.emplace(key, std::make_pair(~0u, 0u)).first;
+      uint32_t bit = 1u << (f.lane % 32);
+      it->second.first &= ~bit;
+      if (f.value) it->second.second |= bit; else it->second.second &= ~bit;
+    }
+    std::vector<uint32_t> ent;
+    for (auto& kv : m) { start[kv.first.first + 1]++; ent.insert(ent.end(), {kv.first.second, kv.second.first, kv.second.second}); }
+    for (uint32_t n = 0; n < N.n_nodes; n++) start[n + 1] += start[n];
+    fstart_.upload(start, stream_);
+    fent_.upload(ent, stream_);
+    forces_dirty_ = false;
+    synchronize();
+  }
+
+  template <class T>
+  T* upload_prog(size_t idx, const std::vector<T>& v) {
+    auto& b = prog_bufs_[idx];
+    size_t bytes = std::max<size_t>(1, v.size() * sizeof(T));
+    b.alloc((bytes + 3) / 4);
+    if (!v.empty()) RRTL_HIP_CHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
+    return reinterpret_cast<T*>(b.p);
+  }
+
+  uint32_t choose_tile_words(uint32_t n_slots) const {
+    if (opt_.tile_words) return opt_.tile_words;
+    // largest tile that still leaves room for two workgroups per CU
+    for (uint32_t w : {8u, 4u, 2u, 1u}) {
+      size_t bytes = size_t(n_slots + 64) * w * 4;
+      if (bytes * 2 <= lds_max_ || w == 1) return w;
+    }
+    return 1;
+  }
+
+  void run_impl(const RunSpec& spec, bool with_outputs) {
+    for (uint32_t s : spec.obs_sig) check_sig(s);
+    for (uint32_t s : spec.in_sig) check_sig(s);
+    if (spec.halt_sig != NONE) throw Error("E_HIP_UNSUPPORTED: the gate engine does not halt lanes");
+    refresh_if_stale();
+    std::vector<uint8_t> forced(N.nsig, 0);
+    for (auto& f : forces_) forced[f.sig] = 1;
+    TileProgram T = compile_tile(N, P, spec, forced);
+    uint32_t W = choose_tile_words(T.n_slots);
+    if (W != 1 && W != 2 && W != 4 && W != 8) throw Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
+    size_t lds = size_t(T.n_slots + 64) * W * 4;
+    if (lds > lds_max_) {
+      W = 1; lds = size_t(T.n_slots + 64) * 4;
+      if (lds > lds_max_) throw Error("E_HIP_TOO_LARGE: live gate state does not fit in LDS for one 32-lane tile");
+    }
+    W_ = W; last_slots_ = T.n_slots; last_levels_ = T.n_cycle_levels + T.n_post_levels; last_gates_ = uint32_t(T.desc.size() / 2);
+    const uint32_t tiles = uint32_t((nwords_ + W - 1) / W);
+    // per-tile force clamp entries (pos, slot, word, and, or), merged per LDS word
+    std::vector<uint32_t> fent, fstart(tiles + 1, 0);
+    if (!forces_.empty()) {
+      std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> m;  // (tile,pos,slot,word)
+      std::unordered_map<uint32_t, std::vector<std::pair<uint32_t, uint32_t>>> sites;
+      for (auto& cs : T.clamp_sites) sites[cs.first].push_back(cs.second);
+      for (auto& f : forces_) {
+        auto it = sites.find(f.sig);
+        if (it == sites.end()) continue;
+        uint32_t gw = f.lane / 32, tile = gw / W, word = gw % W, bit = 1u << (f.lane % 32);
+        for (auto& ps : it->second) {
+          auto key = std::make_tuple(tile, ps.first, ps.second, word);
+          auto e = m.find(key);
+          if (e == m.end()) e = m.emplace(key, std::make_pair(~0u, 0u)).first;
+          e->second.first &= ~bit;
+          if (f.value) e->second.second |= bit; else e->second.second &= ~bit;
+        }
+      }
+      for (auto& kv : m) {
+        fstart[std::get<0>(kv.first) + 1]++;
+        fent.insert(fent.end(), {std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), kv.second.first, kv.second.second});
+      }
+      for (uint32_t t = 0; t < tiles; t++) fstart[t + 1] += fstart[t];
+    }
+    TileArgs A{};
+    A.state = state_.p; A.nwords = nwords_;
+    std::vector<uint2> desc(T.desc.size() / 2);
+    for (size_t g = 0; g < desc.size(); g++) desc[g] = make_uint2(T.desc[2 * g], T.desc[2 * g + 1]);
+    A.desc = upload_prog(0, desc);
+    A.level_start = upload_prog(1, T.level_start);
+    A.n_cycle_levels = T.n_cycle_levels; A.n_post_levels = T.n_post_levels;
+    A.load_slot = upload_prog(2, T.load_slot); A.load_node = upload_prog(3, T.load_node); A.n_load = uint32_t(T.load_slot.size());
+    A.stim_slot = upload_prog(4, T.stim_slot); A.stim_mode = upload_prog(5, T.stim_mode);
+    A.stim_stream = upload_prog(6, T.stim_stream); A.stim_arg = upload_prog(7, T.stim_arg); A.n_stim = uint32_t(T.stim_slot.size());
+    std::vector<uint4> cm(T.commit.size() / 4);
+    for (size_t k = 0; k < cm.size(); k++) cm[k] = make_uint4(T.commit[4 * k], T.commit[4 * k + 1], T.commit[4 * k + 2], T.commit[4 * k + 3]);
+    A.commit = upload_prog(8, cm); A.n_commit = uint32_t(cm.size());
+    A.obs_slot = upload_prog(9, T.obs_slot); A.n_obs = uint32_t(T.obs_slot.size());
+    A.force = fent.empty() ? nullptr : upload_prog(10, fent);
+    A.force_start = fent.empty() ? nullptr : upload_prog(11, fstart);
+    has_digest_ = with_outputs && !spec.obs_sig.empty();
+    if (has_digest_) { digest_.alloc(size_t(64) * nwords_); digest_.zero(stream_); }
+    A.digest = has_digest_ ? digest_.p : nullptr;
+    trace_bytes_ = 0;
+    if (with_outputs && spec.trace && !spec.obs_sig.empty()) {
+      trace_bytes_ = size_t(spec.cycles) * spec.obs_sig.size() * nwords_ * 4;
+      trace_.alloc(trace_bytes_ / 4);
+    }
+    A.trace = trace_bytes_ ? trace_.p : nullptr;
+    A.seed = spec.seed; A.n_slots = T.n_slots; A.c0 = T.c0; A.c1 = T.c1; A.observe_post = spec.observe_post ? 1 : 0;
+    const uint64_t chunk = opt_.ticks_per_launch ? opt_.ticks_per_launch : spec.cycles;
+    stats_ = RunStats{};
+    RRTL_HIP_CHECK(hipStreamSynchronize(stream_));
+    RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
+    for (uint64_t done = 0; done < spec.cycles; done += chunk) {
+      A.cycle_base = done;
+      A.cycles = std::min<uint64_t>(chunk, spec.cycles - done);
+      if (A.trace) A.trace = trace_.p + done * A.n_obs * nwords_;
+      launch_tile(W, tiles, lds, A);
+      stats_.launches++;
+    }
+    RRTL_HIP_CHECK(hipEventRecord(ev1_, stream_));
+    RRTL_HIP_CHECK(hipEventSynchronize(ev1_));
+    float ms = 0;
+    RRTL_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    stats_.kernel_ms = ms;
+    stats_.lane_cycles = uint64_t(lanes_) * spec.cycles;
+    wires_stale_ = spec.cycles > 0;
+  }
+
+  template <int W>
+  void launch_w(uint32_t tiles, size_t lds, const TileArgs& A) {
+    if (lds > 65536)
+      RRTL_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_kernel<W>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    hipLaunchKernelGGL(tile_kernel<W>, dim3(tiles), dim3(TILE_THREADS), lds, stream_, A);
+  }
+  void launch_tile(uint32_t W, uint32_t tiles, size_t lds, const TileArgs& A) {
+    switch (W) {
+      case 1: launch_w<1>(tiles, lds, A); break;
+      case 2: launch_w<2>(tiles, lds, A); break;
+      case 4: launch_w<4>(tiles, lds, A); break;
+      default: launch_w<8>(tiles, lds, A); break;
+    }
+    RRTL_HIP_CHECK(hipGetLastError());
+  }
+
+  GateNet N;
+  uint64_t nwords_ = 0;
+  uint32_t rows_ = 0, stage_row_ = 0;
+  size_t lds_max_ = 65536;
+  DevBuf<uint32_t> state_;
+  DevBuf<uint4> settle_g_, next_g_, commit_g_;
+  DevBuf<uint32_t> fstart_, fent_, tile_force_, tile_fstart_, digest_, trace_;
+  DevBuf<uint32_t> prog_bufs_[16];
+  std::vector<Force> forces_;
+  bool forces_dirty_ = true, wires_stale_ = false, has_digest_ = false;
+  size_t trace_bytes_ = 0;
+  uint32_t W_ = 0, last_slots_ = 0, last_levels_ = 0, last_gates_ = 0;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+}  // namespace
+
+Engine* make_gate_engine(Program prog, uint32_t lanes, const EngineOptions& opt) {
+  return new GateEngine(std::move(prog), lanes, opt);
+}
+
+// gate-engine specific accessors used by the C ABI
+void gate_engine_get_words(Engine* e, uint32_t sig, uint32_t* out) { static_cast<GateEngine*>(e)->get_signal_words(sig, out); }
+void gate_engine_tile_info(Engine* e, uint32_t* info) {
+  auto* g = static_cast<GateEngine*>(e);
+  info[0] = g->tile_words(); info[1] = g->tile_slots(); info[2] = g->tile_levels(); info[3] = g->tile_gates();
+}
+
+}  // namespace rrtl

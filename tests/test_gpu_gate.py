@@ -1,0 +1,185 @@
+"""Bit-sliced gate engine on the GPU against the bit-parallel oracle, the crc32
+known answers and the fault-simulation known answers
+(crates/rrtl-sim-ir/examples/fault_sim.rs; docs/rrtl-optimizations.md:1412-1416)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import stim
+from rrtl_amd import designs, lower, netlist
+from rrtl_amd.backend import IN_CONST, IN_PULSE, IN_RANDOM, HipSimulator
+from rrtl_amd.program import KIND_INPUT, KIND_OUTPUT, KIND_REG
+from conftest import flush_print
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def crc32_program():
+    with open(os.path.join(GOLDEN, "crc32_gates.json")) as f:
+        return lower.lower_module(netlist.import_yosys_netlist(f.read(), "crc32"))
+
+
+def read_crc(sim, p, lane=0):
+    return sum(int(sim.get_signal(p.signal_index(f"crc_{b}"))[lane]) << b for b in range(32))
+
+
+def test_crc32_known_answer():
+    """1 reset cycle then 200 cycles of din=0xAB give 0x53535102 (closed form of crc32.sv)."""
+    flush_print("case: crc32 KAT gate engine")
+    p = crc32_program()
+    sim = HipSimulator(p, 64, engine="gate")
+    assert sim.kind == "gate"
+    sim.set_signal("clk", 1)
+    sim.set_signal("rst", 1)
+    for i in range(8):
+        sim.set_signal(f"din_{i}", (0xAB >> i) & 1)
+    sim.tick()
+    sim.set_signal("rst", 0)
+    sim.tick_many(200)
+    v = 0xFFFFFFFF
+    for _ in range(200):
+        v = designs.crc32_step(v, 0xAB)
+    assert v == 0x53535102
+    assert read_crc(sim, p) == v
+    assert len(set(sim.get_signal("crc_5"))) == 1  # every lane identical
+
+
+def test_fault_sim_known_answer():
+    """64 stuck-at faults on the 32 flops, lane 0 golden: 64/64 detected, latency 1/6/2.1."""
+    flush_print("case: fault-sim KAT gate engine")
+    p = crc32_program()
+    regs = [i for i, s in enumerate(p.signals) if s.kind == KIND_REG]
+    faults = [(r, v) for r in regs for v in (0, 1)]
+    lanes = len(faults) + 1
+    sim = HipSimulator(p, lanes, engine="gate")
+    for f, (sig, v) in enumerate(faults):
+        sim.set_force(sig, f + 1, v)
+    crc = [p.signal_index(f"crc_{b}") for b in range(32)]
+    din = [p.signal_index(f"din_{i}") for i in range(8)]
+    sim.set_signal("clk", 1)
+    detected = [None] * len(faults)
+    gold = 0xFFFFFFFF
+    for cyc in range(96):
+        dv = ((cyc * 2654435761) >> 13) & 0xFF
+        sim.set_signal("rst", 1 if cyc < 1 else 0, settle=False)
+        for i in range(8):
+            sim.set_signal(din[i], (dv >> i) & 1, settle=False)
+        sim.tick()
+        words = np.stack([sim.get_signal_bits(c) for c in crc])  # [32, words32]
+        bits = ((words[:, :, None] >> np.arange(32, dtype=np.uint32)) & 1).reshape(32, -1)[:, :lanes]
+        g = bits[:, 0]
+        gold = 0xFFFFFFFF if cyc < 1 else designs.crc32_step(gold, dv)
+        assert sum(int(g[b]) << b for b in range(32)) == gold
+        for f in range(len(faults)):
+            if detected[f] is None and not np.array_equal(bits[:, f + 1], g):
+                detected[f] = cyc
+    lat = [d + 1 for d in detected if d is not None]
+    assert len(lat) == 64
+    assert (min(lat), max(lat), round(sum(lat) / len(lat), 1)) == (1, 6, 2.1)
+
+
+def test_fault_sim_device_run_post_observation():
+    """The same campaign as one device-resident run with POST observation and a trace."""
+    p = crc32_program()
+    regs = [i for i, s in enumerate(p.signals) if s.kind == KIND_REG]
+    faults = [(r, v) for r in regs for v in (0, 1)]
+    lanes = len(faults) + 1
+    crc = [p.signal_index(f"crc_{b}") for b in range(32)]
+    ins = [(p.signal_index("clk"), IN_CONST, 1), (p.signal_index("rst"), IN_PULSE, oracle.pulse_arg(1, 1, 0))]
+    ins += [(p.signal_index(f"din_{i}"), IN_RANDOM, 0) for i in range(8)]
+    sim = HipSimulator(p, lanes, engine="gate")
+    ref = oracle.BitParallelRef(p, lanes)
+    for f, (sig, v) in enumerate(faults):
+        sim.set_force(sig, f + 1, v)
+        ref.set_force(sig, f + 1, v)
+    sim.run(96, seed=9, inputs=ins, observe=crc, observe_post=True, trace=True)
+    dg, tr = ref.run(seed=9, cycles=96, inputs=ins, observe=crc, observe_post=True, digest=True, trace=True)
+    got = stim.u32_rows_to_u64(sim.trace().reshape(96, 32, -1))
+    assert np.array_equal(got, tr)
+    assert np.array_equal(stim.u32_rows_to_u64(sim.digest()), dg)
+
+
+def small_netlist_program(**kw):
+    m = designs.synthetic_netlist(**kw)
+    return lower.lower_module(m)
+
+
+def netlist_inputs(p):
+    return [(i, IN_RANDOM, 0) for i, s in enumerate(p.signals) if s.kind == KIND_INPUT]
+
+
+def netlist_outputs(p):
+    return [i for i, s in enumerate(p.signals) if s.kind == KIND_OUTPUT]
+
+
+@pytest.mark.parametrize("tile_words", [1, 2, 4, 8])
+def test_synthetic_netlist_trace_matches_oracle(tile_words):
+    flush_print(f"case: small synthetic netlist tile_words={tile_words}")
+    p = small_netlist_program(gates=3000, levels=24, dffs=300, inputs=48, outputs=32, seed=4)
+    lanes, cycles, seed = 1000, 40, 17
+    ins, obs = netlist_inputs(p), netlist_outputs(p)
+    sim = HipSimulator(p, lanes, engine="gate", tile_words=tile_words)
+    sim.run(cycles, seed=seed, inputs=ins, observe=obs, trace=True)
+    ref = oracle.BitParallelRef(p, lanes)
+    dg, tr = ref.run(seed=seed, cycles=cycles, inputs=ins, observe=obs, digest=True, trace=True)
+    got = stim.u32_rows_to_u64(sim.trace().reshape(cycles, len(obs), -1))
+    assert np.array_equal(got, tr)
+    assert np.array_equal(stim.u32_rows_to_u64(sim.digest()), dg)
+    regs = [i for i, s in enumerate(p.signals) if s.kind == KIND_REG]
+    for r in regs[::7] + [i for i, s in enumerate(p.signals) if s.name.startswith("g")][::97]:
+        assert np.array_equal(stim.u32_rows_to_u64(sim.get_signal_bits(r))[: ref.words], ref.get_words(r)), p.signals[r].name
+
+
+def test_gate_engine_api_matches_oracle():
+    """eval/tick/tick_many/tick_from_evaluated through the exact HBM kernels and the tiled kernel."""
+    p = lower.lower_module(designs.gate_chain_design(depth=16, cones=64))
+    lanes = 1024
+    sim = HipSimulator(p, lanes, engine="gate")
+    ref = oracle.BitParallelRef(p, lanes)
+    a = np.array([(l & 1) for l in range(lanes)], np.uint32).reshape(1, lanes)
+    b = np.array([((l >> 1) & 1) for l in range(lanes)], np.uint32).reshape(1, lanes)
+    sim.set_signal_limbs("a", a); sim.set_signal_limbs("b", b); sim.set_signal("clk", 1)
+    pa = np.zeros(ref.words, np.uint64)
+    pb = np.zeros(ref.words, np.uint64)
+    for l in range(lanes):
+        pa[l // 64] |= np.uint64(int(a[0, l]) << (l % 64))
+        pb[l // 64] |= np.uint64(int(b[0, l]) << (l % 64))
+    ref.set_words(p.signal_index("a"), pa); ref.set_words(p.signal_index("b"), pb)
+    ref.set_signal_all(p.signal_index("clk"), 1)
+    outs = [p.signal_index(f"o{c}") for c in range(64)]
+
+    def same():
+        for o in outs:
+            assert np.array_equal(stim.u32_rows_to_u64(sim.get_signal_bits(o))[: ref.words], ref.get_words(o))
+    for _ in range(3):
+        sim.tick(); ref.tick()
+    same()
+    sim.tick_many(37); ref.tick_many(37)
+    same()
+    sim.eval_combinational(); ref.settle()
+    sim.tick_from_evaluated(); ref.tick()  # reference tick = settle, edge, settle
+    sim.eval_combinational()
+    same()
+
+
+def test_full_size_netlist_sampled_words():
+    """The 20k-gate workload at full shape (65,536 lanes) for 300 cycles; four lane
+    words checked against the oracle, every lane against a second GPU run with a
+    different tile shape."""
+    flush_print("case: 20k-gate netlist 65536 lanes x 300 cycles")
+    p = small_netlist_program()
+    lanes, cycles, seed = 65536, 300, 1
+    ins, obs = netlist_inputs(p), netlist_outputs(p)
+    sim = HipSimulator(p, lanes, engine="gate")
+    sim.run(cycles, seed=seed, inputs=ins, observe=obs)
+    d1 = stim.u32_rows_to_u64(sim.digest())
+    sim2 = HipSimulator(p, lanes, engine="gate", tile_words=1)
+    sim2.run(cycles, seed=seed, inputs=ins, observe=obs)
+    assert np.array_equal(d1, stim.u32_rows_to_u64(sim2.digest()))
+    for w64 in (0, 313, 777, 1023):
+        ref = oracle.BitParallelRef(p, 64)
+        dg, _ = ref.run(seed=seed, cycles=cycles, inputs=ins, observe=obs, word_base=w64, digest=True)
+        assert np.array_equal(d1[:, w64], dg[:, 0]), w64

@@ -1,0 +1,88 @@
+"""Program format, lowering structure and the C-ABI library (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from rrtl_amd import backend, designs, ir, lower
+from rrtl_amd.program import OP_ADD, OP_MUX, OP_SIGNAL, Program
+from designgen import random_design
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_blob_roundtrip():
+    for seed in range(10):
+        p = lower.lower_module(random_design(seed))
+        q = Program.from_blob(p.to_blob())
+        assert q.to_blob() == p.to_blob()
+        assert [s.name for s in q.signals] == [s.name for s in p.signals]
+
+
+def test_counter_lowering_structure():
+    """Packets follow lower_packets_to_block: effects on the last machine packet,
+    tick_next captures the register with its reset (lib.rs:2997-3048, :12816)."""
+    p = lower.lower_module(designs.counter())
+    assert [s.name for s in p.signals] == ["clk", "rst", "en", "out", "count"]
+    assert [s.offset for s in p.signals] == [0, 1, 2, 3, 4]
+    assert p.total_signal_words == 5
+    comb = p.comb
+    assert len(comb.packets) == 1 and comb.packets[0].instrs[0].op == OP_SIGNAL
+    nxt = p.tick_next
+    last = nxt.packets[-1]
+    assert len(last.effects) == 1 and last.effects[0].reset is not None
+    ops = [i.op for pk in nxt.packets for i in pk.instrs]
+    assert OP_ADD in ops and OP_MUX in ops
+
+
+def test_levelization_with_chain():
+    m = ir.Module("Chain")
+    a = m.input("a", 4)
+    w1 = m.wire("w1", 4)
+    w2 = m.wire("w2", 4)
+    w3 = m.wire("w3", 4)
+    m.assign(w3, w2 + w1)   # declared first, depends on later wires
+    m.assign(w2, w1 ^ a)
+    m.assign(w1, ~a)
+    p = lower.lower_module(m)
+    stores = []
+    for pk in p.comb.packets:
+        for e in pk.effects:
+            stores.append(p.signals[e.dst].name)
+    assert stores == ["w1", "w2", "w3"]
+
+
+def test_design_rules():
+    m = ir.Module("Bad")
+    a = m.input("a", 4)
+    b = m.input("b", 5)
+    with pytest.raises(ir.DesignError):
+        _ = a + b
+    w = m.wire("w", 4)
+    m.assign(w, a)
+    with pytest.raises(ir.DesignError):
+        m.assign(w, a)
+    with pytest.raises(ir.DesignError):
+        ir.mux(a, a, a)
+
+
+def test_library_exports_every_header_symbol():
+    lib = backend.load_library()
+    hdr = open(os.path.join(ROOT, "include", "rrtl_hip.h")).read()
+    names = set(re.findall(r"\b(rrtl_hip_[a-z_0-9]+)\s*\(", hdr))
+    assert names, "no declarations parsed"
+    assert set(backend.EXPORTS) == names
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_create_rejects_bad_blob_without_gpu_work():
+    lib = backend.load_library()
+    h = ctypes.c_void_p()
+    st = lib.rrtl_hip_create(b"not a program blob at all......." * 2, 64, 4, None, ctypes.byref(h))
+    assert st == 2  # RRTL_E_PROGRAM
+    assert lib.rrtl_hip_create_error().decode().startswith("E_PROGRAM_MAGIC")
+    st = lib.rrtl_hip_create(b"x", 1, 0, None, ctypes.byref(h))
+    assert lib.rrtl_hip_create_error().decode().startswith("E_SIM_IR_LANES")
