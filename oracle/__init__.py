@@ -289,3 +289,66 @@ class BitParallelFast(_BPBase):
         spec, dg, tr = self._spec(**kw)
         secs = self.L.bpg_run(self.h, ctypes.byref(spec))
         return dg, tr, secs
+
+
+# ------------------------------------------------------------ host batch-SIMD
+_simd = None
+
+
+def _simd_lib():
+    global _simd
+    if _simd is None:
+        L = _lib("libsimd_cpu.so")
+        L.simd_create.restype = ctypes.c_void_p
+        L.simd_create.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32,
+                                  ctypes.c_char_p, ctypes.c_size_t]
+        L.simd_destroy.argtypes = [ctypes.c_void_p]
+        L.simd_run.restype = ctypes.c_double
+        L.simd_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int,
+                               ctypes.c_void_p]
+        L.simd_tick_many.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        L.simd_get_signal_limbs.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.simd_set_signal_limbs.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        _simd = L
+    return _simd
+
+
+class SimdCpu:
+    """Host batch-SIMD engine (``SimdCpuSimulator`` restated), lane-sharded over threads."""
+
+    def __init__(self, program, lanes: int, threads: int = 0, shard_lanes: int = 0):
+        L = _simd_lib()
+        blob = program.to_blob()
+        err = ctypes.create_string_buffer(512)
+        h = L.simd_create(blob, len(blob), lanes, threads, shard_lanes, err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        self.L, self.h, self.program, self.lanes = L, h, program, lanes
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.simd_destroy(self.h)
+            self.h = None
+
+    def run(self, seed, cycles, inputs, observe, observe_post=False):
+        """Returns (per-lane u64 digest, seconds)."""
+        isig = np.array([i[0] for i in inputs] or [0], dtype=np.uint32)
+        imode = np.array([i[1] for i in inputs] or [0], dtype=np.uint32)
+        iarg = np.array([i[2] for i in inputs] or [0], dtype=np.uint64)
+        osig = np.array(list(observe) or [0], dtype=np.uint32)
+        dg = np.zeros(self.lanes, dtype=np.uint64)
+        secs = self.L.simd_run(self.h, seed, cycles, len(inputs), _ptr(isig), _ptr(imode), _ptr(iarg), len(observe),
+                               _ptr(osig), int(observe_post), _ptr(dg))
+        return dg, secs
+
+    def tick_many(self, n): self.L.simd_tick_many(self.h, n)
+
+    def get_signal_limbs(self, sig) -> np.ndarray:
+        out = np.zeros((self.program.signals[sig].limbs, self.lanes), dtype=np.uint32)
+        self.L.simd_get_signal_limbs(self.h, sig, _ptr(out))
+        return out
+
+    def set_signal_limbs(self, sig, vals) -> None:
+        v = np.ascontiguousarray(vals, dtype=np.uint32).reshape(self.program.signals[sig].limbs, self.lanes)
+        self.L.simd_set_signal_limbs(self.h, sig, _ptr(v))

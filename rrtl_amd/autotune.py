@@ -1,0 +1,131 @@
+"""Autotuner for the HIP engines' launch shape.
+
+Mirrors the reference's GPU autotune recommendations
+(``GpuAutotuneRecommendation`` / ``find_gpu_autotune_recommendation_with_mode`` /
+``autotune_match_key``, crates/rrtl-gpu-sim/src/lib.rs:55-190): candidates are
+measured on the device, the fastest becomes a recommendation keyed by
+``(case, lanes, steps)``, recommendations are stored as JSON, and lookups match
+exactly or by the nearest (lanes, steps) with ties broken by the measured
+metric.
+
+Knobs: the gate engine's tile width (u32 lane-words per workgroup, i.e. how
+many lanes share one LDS-resident copy of the live gate values, which sets
+LDS per workgroup and so workgroups per CU) and the cycles per launch; the
+word engine's workgroup size and cycles per launch.
+"""
+from __future__ import annotations
+
+import json
+from dataclasses import asdict, dataclass, field
+from typing import Dict, Iterable, List, Optional, Sequence
+
+from .backend import HipError, HipSimulator
+from .program import Program
+
+
+@dataclass
+class HipAutotuneRecommendation:
+    case: str
+    lanes: int
+    steps: int
+    engine: str
+    tile_words: int = 0
+    block_threads: int = 0
+    ticks_per_launch: int = 0
+    autotune_metric: str = "kernel_ns_per_lane_cycle"
+    autotune_metric_ns: Optional[float] = None
+    candidates: List[dict] = field(default_factory=list)
+
+    def options(self) -> dict:
+        return {"engine": self.engine, "tile_words": self.tile_words, "block_threads": self.block_threads,
+                "ticks_per_launch": self.ticks_per_launch}
+
+
+def _candidates(engine: str, steps: int) -> List[dict]:
+    tpl = [0] if steps <= 4096 else [0, 2048]
+    if engine == "gate":
+        return [{"tile_words": w, "ticks_per_launch": t} for w in (1, 2, 4, 8) for t in tpl]
+    return [{"block_threads": b, "ticks_per_launch": t} for b in (64, 128, 256, 512) for t in tpl]
+
+
+def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = "auto",
+             run_kwargs: Optional[dict] = None, probe_steps: Optional[int] = None, repeats: int = 2,
+             device: int = 0) -> HipAutotuneRecommendation:
+    """Measures every candidate on ``probe_steps`` cycles (default min(steps, 512))
+    of the same run and returns the fastest as a recommendation."""
+    if engine == "auto":
+        engine = "gate" if program.is_gate_level() else "word"
+    probe = probe_steps or min(steps, 512)
+    run_kwargs = dict(run_kwargs or {})
+    results = []
+    best = None
+    for cand in _candidates(engine, steps):
+        try:
+            sim = HipSimulator(program, lanes, engine=engine, device=device, **cand)
+        except HipError as e:
+            results.append({**cand, "error": e.code})
+            continue
+        try:
+            sim.run(probe, **run_kwargs)  # warm
+            ms = min(_timed(sim, probe, run_kwargs) for _ in range(repeats))
+        except HipError as e:
+            results.append({**cand, "error": e.code})
+            sim.close()
+            continue
+        ns = ms * 1e6 / (lanes * probe)
+        row = {**cand, "ns_per_lane_cycle": ns}
+        if engine == "gate":
+            row.update(sim.tile_info())
+        results.append(row)
+        if best is None or ns < best[1]:
+            best = (cand, ns)
+        sim.close()
+    if best is None:
+        raise HipError(3, "E_HIP_AUTOTUNE: no candidate launch shape ran")
+    return HipAutotuneRecommendation(case=case, lanes=lanes, steps=steps, engine=engine,
+                                     tile_words=best[0].get("tile_words", 0),
+                                     block_threads=best[0].get("block_threads", 0),
+                                     ticks_per_launch=best[0].get("ticks_per_launch", 0),
+                                     autotune_metric_ns=best[1], candidates=results)
+
+
+def _timed(sim: HipSimulator, steps: int, run_kwargs: dict) -> float:
+    sim.run(steps, **run_kwargs)
+    return sim.stats()["kernel_ms"]
+
+
+def save_recommendations(path: str, items: Iterable[HipAutotuneRecommendation]) -> None:
+    with open(path, "w") as f:
+        json.dump([asdict(i) for i in items], f, indent=1)
+
+
+def load_recommendations(path: str) -> List[HipAutotuneRecommendation]:
+    try:
+        with open(path) as f:
+            raw = json.load(f)
+        return [HipAutotuneRecommendation(**r) for r in raw]
+    except (OSError, ValueError, TypeError) as e:
+        raise ValueError(f"E_HIP_AUTOTUNE_JSON: failed to load autotune recommendations: {e}")
+
+
+def _match_key(r: HipAutotuneRecommendation, lanes: int, steps: int):
+    return (abs(r.lanes - lanes), abs(r.steps - steps),
+            r.autotune_metric_ns if r.autotune_metric_ns is not None else float("inf"), r.lanes, r.steps)
+
+
+def find_recommendation(items: Sequence[HipAutotuneRecommendation], case: str, lanes: int, steps: int,
+                        mode: str = "exact") -> Optional[HipAutotuneRecommendation]:
+    if mode == "exact":
+        for r in items:
+            if r.case == case and r.lanes == lanes and r.steps == steps:
+                return r
+        return None
+    cands = [r for r in items if r.case == case]
+    return min(cands, key=lambda r: _match_key(r, lanes, steps)) if cands else None
+
+
+def options_from_recommendations(items, case: str, lanes: int, steps: int, mode: str = "exact") -> dict:
+    r = find_recommendation(items, case, lanes, steps, mode)
+    if r is None:
+        raise KeyError(f"E_HIP_AUTOTUNE_RECOMMENDATION: no recommendation for case `{case}` lanes={lanes} steps={steps}")
+    return r.options()

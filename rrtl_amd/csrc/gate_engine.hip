@@ -27,7 +27,7 @@
 #include <unordered_map>
 
 namespace rrtl {
-namespace {
+namespace {  // anon
 
 // 4-bit gate op: bit0 invert a, bit1 invert b, bit2 invert result, bit3 xor.
 // r = (x ? a ^ b : (a ^ ia) & (b ^ ib)) ^ io covers every 2-input gate kind.
@@ -37,6 +37,8 @@ constexpr uint32_t G_NOT = GI_A;  // b = ones
 constexpr uint32_t G_COPY = 0;    // b = ones
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t CLAMP_COMMIT = 0xFFFFFFFEu;
+constexpr uint32_t TILE_THREADS = 256;
+constexpr uint32_t MAX_LEVEL_GATES = TILE_THREADS;  // one gate per thread per level
 
 struct CGate { uint32_t op, a, b, dst; };
 struct CCommit { uint32_t reg, val, rsig, pol, rbit, has_reset; };
@@ -338,20 +340,29 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
     T.commit.insert(T.commit.end(), {T.persist.at(c.reg), vs, rs == NONE ? 0u : rs, c.pol | (c.rbit << 1) | (c.has_reset << 2)});
   }
   if (T.n_slots > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
-  uint32_t n_cycle = L + (staging.empty() ? 0 : 1);
+  // Kernel levels: empty levels are dropped, levels wider than one gate per
+  // thread are split into independent sub-levels (one barrier each).
   T.level_start.push_back(0);
+  uint32_t nlev = 0;
   auto flush = [&](const std::vector<uint32_t>& d) {
-    T.desc.insert(T.desc.end(), d.begin(), d.end());
-    T.level_start.push_back(uint32_t(T.desc.size() / 2));
+    const size_t n = d.size() / 2;
+    for (size_t at = 0; at < n; at += MAX_LEVEL_GATES) {
+      size_t e = std::min(n, at + MAX_LEVEL_GATES);
+      T.desc.insert(T.desc.end(), d.begin() + 2 * at, d.begin() + 2 * e);
+      T.level_start.push_back(uint32_t(T.desc.size() / 2));
+      nlev++;
+    }
+    return nlev;  // clamps of these gates run after the level's last sub-level
   };
-  for (uint32_t l = 1; l <= L; l++) flush(lv_desc[l]);
-  if (!staging.empty()) flush(staging);
-  for (uint32_t l = 1; l <= L2; l++) flush(lv2_desc[l]);
-  T.n_cycle_levels = n_cycle; T.n_post_levels = L2;
-  // clamp sites in kernel numbering: cycle level l -> l, post level l -> n_cycle + l,
-  // commit -> CLAMP_COMMIT
-  for (auto& fc : forced_cycle) T.clamp_sites.push_back(fc);
-  for (auto& fp : forced_post) T.clamp_sites.push_back({fp.first, {n_cycle + fp.second.first, fp.second.second}});
+  std::vector<uint32_t> kpos_cycle(L + 1, 0), kpos_post(L2 + 1, 0);
+  for (uint32_t l = 1; l <= L; l++) kpos_cycle[l] = flush(lv_desc[l]);
+  flush(staging);
+  const uint32_t n_cycle = nlev;
+  for (uint32_t l = 1; l <= L2; l++) kpos_post[l] = flush(lv2_desc[l]);
+  T.n_cycle_levels = n_cycle; T.n_post_levels = nlev - n_cycle;
+  // clamp sites in kernel numbering (index of the last sub-level + 1); commit -> CLAMP_COMMIT
+  for (auto& fc : forced_cycle) T.clamp_sites.push_back({fc.first, {kpos_cycle[fc.second.first], fc.second.second}});
+  for (auto& fp : forced_post) T.clamp_sites.push_back({fp.first, {kpos_post[fp.second.first], fp.second.second}});
   // persistent load/store lists
   for (auto& kv : T.persist) {
     T.load_slot.push_back(kv.second); T.load_node.push_back(kv.first);
@@ -387,17 +398,47 @@ struct TileArgs {
   uint32_t n_slots, c0, c1, observe_post;
 };
 
-constexpr uint32_t TILE_THREADS = 256;
+// W consecutive u32 lane-words of one slot, moved with one LDS access
+template <int W> struct Words { uint32_t v[W]; };
 
 template <int W>
-__device__ __forceinline__ void run_level(const TileArgs& A, uint32_t* lds, uint32_t L) {
-  const uint32_t gb = A.level_start[L], ge = A.level_start[L + 1];
-  for (uint32_t p = gb * W + threadIdx.x; p < ge * W; p += TILE_THREADS) {
-    const uint32_t g = p / W, w = p % W;
-    const uint2 d = A.desc[g];
-    const uint32_t a = lds[(d.x & 0xFFFFu) * W + w], b = lds[(d.x >> 16) * W + w];
-    lds[(d.y & 0xFFFFu) * W + w] = gate_eval(d.y >> 16, a, b);
+__device__ __forceinline__ Words<W> ld_words(const uint32_t* lds, uint32_t slot) {
+  Words<W> r;
+  const uint32_t* p = lds + size_t(slot) * W;
+  if constexpr (W == 1) { r.v[0] = p[0]; }
+  else if constexpr (W == 2) { uint2 x = *reinterpret_cast<const uint2*>(p); r.v[0] = x.x; r.v[1] = x.y; }
+  else {
+#pragma unroll
+    for (int k = 0; k < W; k += 4) {
+      uint4 x = *reinterpret_cast<const uint4*>(p + k);
+      r.v[k] = x.x; r.v[k + 1] = x.y; r.v[k + 2] = x.z; r.v[k + 3] = x.w;
+    }
   }
+  return r;
+}
+
+template <int W>
+__device__ __forceinline__ void st_words(uint32_t* lds, uint32_t slot, const Words<W>& r) {
+  uint32_t* p = lds + size_t(slot) * W;
+  if constexpr (W == 1) { p[0] = r.v[0]; }
+  else if constexpr (W == 2) { *reinterpret_cast<uint2*>(p) = make_uint2(r.v[0], r.v[1]); }
+  else {
+#pragma unroll
+    for (int k = 0; k < W; k += 4) *reinterpret_cast<uint4*>(p + k) = make_uint4(r.v[k], r.v[k + 1], r.v[k + 2], r.v[k + 3]);
+  }
+}
+
+// One gate for every lane-word of the tile.
+template <int W>
+__device__ __forceinline__ void eval_gate(uint32_t* lds, uint2 d) {
+  const uint32_t op = d.y >> 16;
+  const Words<W> a = ld_words<W>(lds, d.x & 0xFFFFu), b = ld_words<W>(lds, d.x >> 16);
+  const uint32_t ia = 0u - (op & 1u), ib = 0u - ((op >> 1) & 1u), io = 0u - ((op >> 2) & 1u);
+  const bool x = (op & GI_X) != 0;
+  Words<W> r;
+#pragma unroll
+  for (int k = 0; k < W; k++) r.v[k] = (x ? (a.v[k] ^ b.v[k]) : ((a.v[k] ^ ia) & (b.v[k] ^ ib))) ^ io;
+  st_words<W>(lds, d.y & 0xFFFFu, r);
 }
 
 template <int W>
@@ -423,97 +464,122 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32
       if (gw < A.nwords) A.trace[(c * A.n_obs + j) * A.nwords + gw] = lds[A.obs_slot[j] * W + w];
     }
   if (A.digest) {
-    constexpr uint32_t PER = (64 * W + TILE_THREADS - 1) / TILE_THREADS;
-    for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
-      uint32_t p = i / W, w = i % W, x = sig[i];
-      for (uint32_t j = (p - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) x ^= lds[A.obs_slot[j] * W + w];
-      sig[i] = x;
-    }
-    __syncthreads();
-    uint32_t nv[PER];
+    Words<W> s;
+    if (tid < 64) {
+      s = ld_words<W>(sig, tid);
+      for (uint32_t j = (tid - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) {
+        const Words<W> v = ld_words<W>(lds, A.obs_slot[j]);
 #pragma unroll
-    for (uint32_t k = 0; k < PER; k++) {
-      uint32_t i = tid + k * TILE_THREADS;
-      if (i < 64 * W) {
-        uint32_t p = i / W, w = i % W;
-        nv[k] = sig[i] ^ (sig[((p + 17) & 63) * W + w] & ~sig[((p + 41) & 63) * W + w]);
+        for (int k = 0; k < W; k++) s.v[k] ^= v.v[k];
       }
+      st_words<W>(sig, tid, s);
     }
     __syncthreads();
+    if (tid < 64) {
+      const Words<W> a = ld_words<W>(sig, (tid + 17) & 63), b = ld_words<W>(sig, (tid + 41) & 63);
 #pragma unroll
-    for (uint32_t k = 0; k < PER; k++) {
-      uint32_t i = tid + k * TILE_THREADS;
-      if (i < 64 * W) sig[i] = nv[k];
+      for (int k = 0; k < W; k++) s.v[k] ^= a.v[k] & ~b.v[k];
     }
+    __syncthreads();
+    if (tid < 64) st_words<W>(sig, tid, s);
   }
   __syncthreads();
 }
 
 template <int W>
 __global__ void __launch_bounds__(TILE_THREADS) tile_kernel(TileArgs A) {
-  extern __shared__ uint32_t lds[];
-  uint32_t* sig = lds + size_t(A.n_slots) * W;  // 64 x W signature words
+  extern __shared__ __align__(16) uint32_t lds[];
+  uint32_t* sig = lds + size_t(A.n_slots) * W;          // 64 x W signature words
+  uint32_t* lstart = sig + 64 * W;                      // level_start copy
   const uint32_t tid = threadIdx.x;
   const uint64_t w0 = uint64_t(blockIdx.x) * W;
+  const uint32_t nlev = A.n_cycle_levels + A.n_post_levels;
   const uint32_t fb = A.force_start ? A.force_start[blockIdx.x] : 0, fe = A.force_start ? A.force_start[blockIdx.x + 1] : 0;
+  for (uint32_t i = tid; i <= nlev; i += TILE_THREADS) lstart[i] = A.level_start[i];
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
     uint32_t s = i / W, w = i % W;
     uint64_t gw = w0 + w;
     lds[A.load_slot[s] * W + w] = gw < A.nwords ? A.state[A.load_node[s] * A.nwords + gw] : 0u;
   }
   for (uint32_t w = tid; w < W; w += TILE_THREADS) { lds[A.c0 * W + w] = 0u; lds[A.c1 * W + w] = ~0u; }
-  if (A.digest)
-    for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
-      uint32_t p = i / W, w = i % W;
-      uint64_t gw = w0 + w;
-      sig[i] = gw < A.nwords ? A.digest[p * A.nwords + gw] : 0u;
-    }
+  for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
+    uint32_t p = i / W, w = i % W;
+    uint64_t gw = w0 + w;
+    sig[i] = (A.digest && gw < A.nwords) ? A.digest[p * A.nwords + gw] : 0u;
+  }
   __syncthreads();
   for (uint64_t c = 0; c < A.cycles; c++) {
     const uint64_t cyc = A.cycle_base + c;
-    for (uint32_t i = tid; i < A.n_stim * W; i += TILE_THREADS) {
-      uint32_t s = i / W, w = i % W;
-      uint64_t gw = w0 + w, arg = A.stim_arg[s];
-      uint32_t v;
+    // primary inputs for this cycle (thread per input, all words of the tile)
+    for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS) {
+      const uint64_t arg = A.stim_arg[s];
+      Words<W> v;
       if (A.stim_mode[s] == IN_RANDOM) {
-        uint64_t word = stim_from_key(stim_key(A.seed, A.stim_stream[s], cyc), gw >> 1);
-        v = uint32_t(word >> (32 * (gw & 1)));
-      } else if (A.stim_mode[s] == IN_CONST) {
-        v = (arg & 1) ? ~0u : 0u;
-      } else {  // IN_PULSE
-        v = ((cyc < (arg & 0xFFFFFFFFull)) ? (arg >> 32) & 1 : (arg >> 33) & 1) ? ~0u : 0u;
+        const uint64_t key = stim_key(A.seed, A.stim_stream[s], cyc);
+#pragma unroll
+        for (int k = 0; k < W; k += 2) {
+          const uint64_t gw = w0 + k;
+          const uint64_t word = stim_from_key(key, gw >> 1);
+          v.v[k] = uint32_t(word >> (32 * (gw & 1)));
+          if (k + 1 < W) v.v[k + 1] = uint32_t(word >> 32);
+        }
+      } else {
+        const uint32_t bit = A.stim_mode[s] == IN_CONST ? uint32_t(arg & 1)
+                                                        : uint32_t((cyc < (arg & 0xFFFFFFFFull)) ? (arg >> 32) & 1 : (arg >> 33) & 1);
+#pragma unroll
+        for (int k = 0; k < W; k++) v.v[k] = bit ? ~0u : 0u;
       }
-      lds[A.stim_slot[s] * W + w] = v;
+      st_words<W>(lds, A.stim_slot[s], v);
     }
     __syncthreads();
-    for (uint32_t L = 0; L < A.n_cycle_levels; L++) {
-      run_level<W>(A, lds, L);
+    // levels: each thread evaluates at most one gate per level; the next
+    // level's descriptor is fetched before this level's gate runs
+    uint2 dnext = make_uint2(0, 0);
+    bool hnext = false;
+    if (nlev) { const uint32_t g = lstart[0] + tid; hnext = g < lstart[1]; if (hnext) dnext = A.desc[g]; }
+    for (uint32_t L = 0; L < nlev; L++) {
+      const uint2 dcur = dnext;
+      const bool hcur = hnext;
+      hnext = false;
+      if (L + 1 < nlev) { const uint32_t g = lstart[L + 1] + tid; hnext = g < lstart[L + 2]; if (hnext) dnext = A.desc[g]; }
+      if (L == A.n_cycle_levels) {
+        // cycle program done: observe before the edge, commit, then the post program
+        if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+        for (uint32_t k = tid; k < A.n_commit; k += TILE_THREADS) {
+          const uint4 cm = A.commit[k];
+          Words<W> v = ld_words<W>(lds, cm.y);
+          if (cm.w & 4) {
+            const Words<W> a = ld_words<W>(lds, cm.z);
+            const uint32_t pol = 0u - (cm.w & 1u), rv = 0u - ((cm.w >> 1) & 1u);
+#pragma unroll
+            for (int q = 0; q < W; q++) { const uint32_t as = a.v[q] ^ pol; v.v[q] = (as & rv) | (~as & v.v[q]); }
+          }
+          st_words<W>(lds, cm.x, v);
+        }
+        __syncthreads();
+        if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
+      }
+      if (hcur) eval_gate<W>(lds, dcur);
       __syncthreads();
       if (fe > fb) { apply_clamps<W>(A, lds, L + 1, fb, fe); __syncthreads(); }
     }
-    if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
-    for (uint32_t i = tid; i < A.n_commit * W; i += TILE_THREADS) {
-      uint32_t k = i / W, w = i % W;
-      const uint4 cm = A.commit[k];
-      uint32_t v = lds[cm.y * W + w];
-      if (cm.w & 4) {
-        uint32_t a = lds[cm.z * W + w];
-        if (cm.w & 1) a = ~a;
-        uint32_t rv = (cm.w & 2) ? ~0u : 0u;
-        v = (a & rv) | (~a & v);
+    if (A.n_post_levels == 0) {
+      if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+      for (uint32_t k = tid; k < A.n_commit; k += TILE_THREADS) {
+        const uint4 cm = A.commit[k];
+        Words<W> v = ld_words<W>(lds, cm.y);
+        if (cm.w & 4) {
+          const Words<W> a = ld_words<W>(lds, cm.z);
+          const uint32_t pol = 0u - (cm.w & 1u), rv = 0u - ((cm.w >> 1) & 1u);
+#pragma unroll
+          for (int q = 0; q < W; q++) { const uint32_t as = a.v[q] ^ pol; v.v[q] = (as & rv) | (~as & v.v[q]); }
+        }
+        st_words<W>(lds, cm.x, v);
       }
-      lds[cm.x * W + w] = v;
+      __syncthreads();
+      if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
     }
-    __syncthreads();
-    if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
-    if (A.observe_post) {
-      for (uint32_t L = A.n_cycle_levels; L < A.n_cycle_levels + A.n_post_levels; L++) {
-        run_level<W>(A, lds, L);
-        __syncthreads();
-        if (fe > fb) { apply_clamps<W>(A, lds, L + 1, fb, fe); __syncthreads(); }
-      }
-      if (A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
-    }
+    if (A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
   }
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
     uint32_t s = i / W, w = i % W;
@@ -788,9 +854,10 @@ class GateEngine final : public Engine {
     TileProgram T = compile_tile(N, P, spec, forced);
     uint32_t W = choose_tile_words(T.n_slots);
     if (W != 1 && W != 2 && W != 4 && W != 8) throw Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
-    size_t lds = size_t(T.n_slots + 64) * W * 4;
+    const size_t lev_words = T.level_start.size() + 3;
+    size_t lds = (size_t(T.n_slots + 64) * W + lev_words) * 4;
     if (lds > lds_max_) {
-      W = 1; lds = size_t(T.n_slots + 64) * 4;
+      W = 1; lds = (size_t(T.n_slots + 64) + lev_words) * 4;
       if (lds > lds_max_) throw Error("E_HIP_TOO_LARGE: live gate state does not fit in LDS for one 32-lane tile");
     }
     W_ = W; last_slots_ = T.n_slots; last_levels_ = T.n_cycle_levels + T.n_post_levels; last_gates_ = uint32_t(T.desc.size() / 2);

@@ -86,3 +86,23 @@ def test_create_rejects_bad_blob_without_gpu_work():
     assert lib.rrtl_hip_create_error().decode().startswith("E_PROGRAM_MAGIC")
     st = lib.rrtl_hip_create(b"x", 1, 0, None, ctypes.byref(h))
     assert lib.rrtl_hip_create_error().decode().startswith("E_SIM_IR_LANES")
+
+
+def test_autotune_recommendation_lookup(tmp_path):
+    """Exact and nearest matching like find_gpu_autotune_recommendation_with_mode
+    (crates/rrtl-gpu-sim/src/lib.rs:108-172)."""
+    from rrtl_amd.autotune import (HipAutotuneRecommendation, find_recommendation, load_recommendations,
+                                   options_from_recommendations, save_recommendations)
+    items = [HipAutotuneRecommendation("net", 4096, 1000, "gate", tile_words=2, autotune_metric_ns=1.0),
+             HipAutotuneRecommendation("net", 65536, 10000, "gate", tile_words=4, autotune_metric_ns=0.5),
+             HipAutotuneRecommendation("ctr", 4096, 1000, "word", block_threads=128, autotune_metric_ns=2.0)]
+    p = tmp_path / "rec.json"
+    save_recommendations(str(p), items)
+    back = load_recommendations(str(p))
+    assert back == items
+    assert find_recommendation(back, "net", 65536, 10000).tile_words == 4
+    assert find_recommendation(back, "net", 60000, 9000) is None
+    assert find_recommendation(back, "net", 60000, 9000, mode="nearest").tile_words == 4
+    assert options_from_recommendations(back, "ctr", 4096, 1000)["block_threads"] == 128
+    with pytest.raises(KeyError):
+        options_from_recommendations(back, "nope", 1, 1)
