@@ -49,7 +49,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device with torch (scale reference for HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         return 2
@@ -81,20 +85,30 @@ def main() -> int:
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
     achieved = lane_cycles * B_ALG / avg_kernel_s / 1e9
 
-    # bit-exactness: sampled lanes of the last timed run against the host engine
+    # bit-exactness: sampled lane words replay the same history (warmup + timed
+    # runs, each continuing the last) on the host engine; the last run's output
+    # signatures and the final flop state must match
     check = None
     if not args.no_check:
         import oracle
         from oracle import stim
+        from rrtl_amd.program import KIND_REG
         dg = stim.u32_rows_to_u64(sim.digest())
+        regs = [i for i, s in enumerate(p.signals) if s.kind == KIND_REG]
         words64 = args.lanes // 64
-        sample = sorted({0, words64 // 3, words64 - 8})
+        sample = sorted({0, words64 // 2, words64 - 8})
         ok = True
         for base in sample:
             ref = oracle.BitParallelFast(p, 512, threads=1)
-            d, _, _ = ref.run(seed=args.seed, cycles=args.cycles, inputs=ins, observe=obs, word_base=base, digest=True)
+            for _ in range(args.warmup + args.steps):
+                d, _, _ = ref.run(seed=args.seed, cycles=args.cycles, inputs=ins, observe=obs, word_base=base,
+                                  digest=True)
             ok &= bool(np.array_equal(d, dg[:, base:base + 8]))
-        check = {"bit_exact": ok, "sample": f"{len(sample)} x 512 lanes x {args.cycles} cycles, output signatures"}
+            for r in regs[::97]:
+                got = stim.u32_rows_to_u64(sim.get_signal_bits(r))[base:base + 8]
+                ok &= bool(np.array_equal(got, ref.get_words(r)))
+        check = {"bit_exact": ok, "sample": f"{len(sample)} x 512 lanes x {args.warmup + args.steps} x {args.cycles} "
+                                            "cycles: output signatures of the last run and sampled flops"}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -106,8 +120,9 @@ def main() -> int:
         cyc = max(probe_cycles, min(args.cycles, int(probe_cycles * args.cpu_seconds / max(secs, 1e-6))))
         base = oracle.BitParallelFast(p, args.lanes, threads=threads)
         d_cpu, _, secs = base.run(seed=args.seed, cycles=cyc, inputs=ins, observe=obs, digest=True)
-        sim.run(cyc, seed=args.seed, inputs=ins, observe=obs)
-        same = bool(np.array_equal(stim_u64(sim.digest()), d_cpu))
+        fresh = HipSimulator(p, args.lanes, engine="gate", tile_words=args.tile_words)
+        fresh.run(cyc, seed=args.seed, inputs=ins, observe=obs)
+        same = bool(np.array_equal(stim_u64(fresh.digest()), d_cpu))
         cpu = {"value": args.lanes * cyc / secs, "unit": "lane-cycles/s", "cores": threads, "kind": "port",
                "sample": f"{args.lanes} lanes x {cyc} cycles of the same workload ({secs:.1f} s)",
                "bit_exact_vs_gpu": same}
@@ -128,6 +143,25 @@ def main() -> int:
         "check": check,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def calibrate_copy(nbytes: int) -> int:
+    """A plain device-to-device copy of known size (the runtime's copy kernel):
+    rocprofv3's FETCH_SIZE / WRITE_SIZE for it give the counters' scale against
+    algorithmic bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 3}), flush=True)
     return 0
 
 

@@ -138,6 +138,24 @@ rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out);
 /* Gate engine: [tile_words, live_slots, levels, gates] of the last compiled run. */
 rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4);
 
+/* ---- multi-design batches: several engines (different designs, lane counts,
+ * cycle counts; all on one device) run their rrtl_hip_run specs in ONE
+ * persistent, HIP-graph-captured launch whose workgroups pull (design, tile)
+ * items longest first from an atomic queue.  No counterpart in the reference,
+ * which runs each design on its own engine (crates/rrtl-runtime/src/lib.rs:8063).
+ * The engines must outlive the batch; trace_inputs buffers of the specs must
+ * stay valid while the batch exists.  Results land in each engine (digest,
+ * trace, halt cycles) exactly as after rrtl_hip_run. */
+typedef struct rrtl_hip_batch rrtl_hip_batch;
+enum { RRTL_BATCH_PERSISTENT = 0, RRTL_BATCH_PER_DESIGN = 1 };
+rrtl_status rrtl_hip_batch_create(rrtl_hip_engine* const* engines, const rrtl_hip_run_spec* specs, uint32_t n,
+                                  rrtl_hip_batch** out);
+/* mode RRTL_BATCH_PERSISTENT: one graph launch; RRTL_BATCH_PER_DESIGN: one launch per design (the comparison) */
+rrtl_status rrtl_hip_batch_run(rrtl_hip_batch* b, uint32_t mode);
+rrtl_status rrtl_hip_batch_stats(const rrtl_hip_batch* b, rrtl_hip_run_stats* out);
+const char* rrtl_hip_batch_last_error(const rrtl_hip_batch* b);
+void rrtl_hip_batch_destroy(rrtl_hip_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,8 +39,10 @@ EXPORTS = [
     "rrtl_hip_snapshot", "rrtl_hip_restore", "rrtl_hip_set_force", "rrtl_hip_clear_forces",
     "rrtl_hip_synchronize", "rrtl_hip_run", "rrtl_hip_last_run_stats", "rrtl_hip_digest_bytes",
     "rrtl_hip_get_digest", "rrtl_hip_trace_bytes", "rrtl_hip_get_trace", "rrtl_hip_get_halt_cycles",
-    "rrtl_hip_gate_tile_info",
+    "rrtl_hip_gate_tile_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
+    "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy",
 ]
+BATCH_PERSISTENT, BATCH_PER_DESIGN = 0, 1
 
 
 class HipError(RuntimeError):
@@ -108,6 +110,12 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_get_trace.argtypes = [vp, vp]
     L.rrtl_hip_get_halt_cycles.argtypes = [vp, vp]
     L.rrtl_hip_gate_tile_info.argtypes = [vp, vp]
+    L.rrtl_hip_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(RunSpec), u32, ctypes.POINTER(vp)]
+    L.rrtl_hip_batch_run.argtypes = [vp, u32]
+    L.rrtl_hip_batch_stats.argtypes = [vp, ctypes.POINTER(RunStats)]
+    L.rrtl_hip_batch_last_error.restype = ctypes.c_char_p
+    L.rrtl_hip_batch_last_error.argtypes = [vp]
+    L.rrtl_hip_batch_destroy.argtypes = [vp]
     for fn in EXPORTS:
         if fn != "rrtl_hip_destroy" and getattr(L, fn).restype is ctypes.c_int:
             getattr(L, fn).restype = ctypes.c_int
@@ -160,10 +168,16 @@ class HipSimulator:
             raise HipError(st, self.L.rrtl_hip_last_error(self.h).decode())
 
     def sig(self, s: SignalRef) -> int:
-        return self.program.signal_index(s) if isinstance(s, str) else int(s)
+        i = self.program.signal_index(s) if isinstance(s, str) else int(s)
+        if not 0 <= i < len(self.program.signals):
+            raise HipError(1, f"E_SIM_IR_SIGNAL: signal index {i} is out of range")
+        return i
 
     def mem(self, m: Union[int, str]) -> int:
-        return self.program.memory_index(m) if isinstance(m, str) else int(m)
+        i = self.program.memory_index(m) if isinstance(m, str) else int(m)
+        if not 0 <= i < len(self.program.memories):
+            raise HipError(1, f"E_SIM_IR_MEMORY: memory index {i} is out of range")
+        return i
 
     @property
     def kind(self) -> str:
@@ -261,6 +275,13 @@ class HipSimulator:
             trace_inputs: Optional[np.ndarray] = None) -> None:
         """``inputs``: (signal, mode, arg) triples with modes IN_*; ``observe``: signals
         folded into the digest (and the trace when ``trace``)."""
+        spec = self.run_spec(cycles, seed, inputs, observe, observe_post, trace, halt, trace_inputs)
+        self._check(self.L.rrtl_hip_run(self.h, ctypes.byref(spec)))
+
+    def run_spec(self, cycles: int, seed: int = 0, inputs: Sequence[Tuple] = (), observe: Sequence[SignalRef] = (),
+                 observe_post: bool = False, trace: bool = False, halt: Optional[SignalRef] = None,
+                 trace_inputs: Optional[np.ndarray] = None) -> RunSpec:
+        """Builds the C run spec; its arrays stay alive on this object until the next call."""
         isig = np.array([self.sig(i[0]) for i in inputs] or [0], dtype=np.uint32)
         imode = np.array([i[1] for i in inputs] or [0], dtype=np.uint32)
         iarg = np.array([int(i[2]) if len(i) > 2 else 0 for i in inputs] or [0], dtype=np.uint64)
@@ -271,7 +292,7 @@ class HipSimulator:
                        iarg.ctypes.data, tin.ctypes.data if tin is not None else None, len(observe),
                        osig.ctypes.data, OBSERVE_POST if observe_post else OBSERVE_PRE, int(trace),
                        NO_HALT if halt is None else self.sig(halt))
-        self._check(self.L.rrtl_hip_run(self.h, ctypes.byref(spec)))
+        return spec
 
     def stats(self) -> dict:
         s = RunStats()
@@ -303,3 +324,44 @@ class HipSimulator:
         out = np.zeros(4, dtype=np.uint32)
         self._check(self.L.rrtl_hip_gate_tile_info(self.h, _ptr(out)))
         return {"tile_words": int(out[0]), "live_slots": int(out[1]), "levels": int(out[2]), "gates": int(out[3])}
+
+
+class Batch:
+    """Several engines (designs) run in one persistent, graph-captured launch
+    (``rrtl_hip_batch_*``).  ``runs`` pairs each simulator with its ``run`` keyword
+    arguments; results land in each simulator as after ``HipSimulator.run``."""
+
+    def __init__(self, runs: Sequence[Tuple["HipSimulator", dict]]):
+        self.L = load_library()
+        self.sims = [r[0] for r in runs]
+        # every spec needs its own arrays alive for the batch's lifetime
+        self._specs = []
+        for sim, kw in runs:
+            holder = HipSimulator.__new__(HipSimulator)
+            holder.program, holder.lanes = sim.program, sim.lanes
+            spec = HipSimulator.run_spec(holder, **kw)
+            self._specs.append((holder, spec))
+        arr_h = (ctypes.c_void_p * len(runs))(*[s.h for s in self.sims])
+        arr_s = (RunSpec * len(runs))(*[sp for _, sp in self._specs])
+        self._arrays = (arr_h, arr_s)
+        h = ctypes.c_void_p()
+        st = self.L.rrtl_hip_batch_create(arr_h, arr_s, len(runs), ctypes.byref(h))
+        if st != 0:
+            raise HipError(st, self.L.rrtl_hip_create_error().decode())
+        self.h = h
+
+    def run(self, persistent: bool = True) -> dict:
+        st = self.L.rrtl_hip_batch_run(self.h, BATCH_PERSISTENT if persistent else BATCH_PER_DESIGN)
+        if st != 0:
+            raise HipError(st, self.L.rrtl_hip_batch_last_error(self.h).decode())
+        s = RunStats()
+        self.L.rrtl_hip_batch_stats(self.h, ctypes.byref(s))
+        return {"kernel_ms": s.kernel_ms, "launches": s.launches, "lane_cycles": s.lane_cycles}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.rrtl_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

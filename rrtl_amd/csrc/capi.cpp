@@ -9,7 +9,18 @@
 namespace rrtl {
 void gate_engine_get_words(Engine* e, uint32_t sig, uint32_t* out);
 void gate_engine_tile_info(Engine* e, uint32_t* info);
+class Batch;
+Batch* make_batch(std::vector<Engine*> engines, std::vector<RunSpec> specs);
+void batch_run(Batch* b, bool persistent);
+const RunStats& batch_stats(const Batch* b);
+void batch_destroy(Batch* b);
 }
+
+struct rrtl_hip_batch {
+  rrtl::Batch* impl = nullptr;
+  std::string err;
+  ~rrtl_hip_batch() { if (impl) rrtl::batch_destroy(impl); }
+};
 
 struct rrtl_hip_engine {
   std::unique_ptr<rrtl::Engine> impl;
@@ -177,5 +188,49 @@ rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4) {
     rrtl::gate_engine_tile_info(&x, out4);
   });
 }
+
+rrtl_status rrtl_hip_batch_create(rrtl_hip_engine* const* engines, const rrtl_hip_run_spec* specs, uint32_t n,
+                                  rrtl_hip_batch** out) {
+  if (!out) return RRTL_E_INVALID;
+  *out = nullptr;
+  try {
+    if (!engines || !specs || n == 0) throw rrtl::Error("E_HIP_BATCH: empty batch");
+    std::vector<rrtl::Engine*> es;
+    std::vector<rrtl::RunSpec> ss;
+    for (uint32_t i = 0; i < n; i++) {
+      if (!engines[i] || !engines[i]->impl) throw rrtl::Error("E_HIP_NULL: null engine in batch");
+      es.push_back(engines[i]->impl.get());
+      ss.push_back(to_spec(&specs[i]));
+    }
+    auto* b = new rrtl_hip_batch;
+    try { b->impl = rrtl::make_batch(std::move(es), std::move(ss)); } catch (...) { delete b; throw; }
+    *out = b;
+    g_create_err.clear();
+    return RRTL_OK;
+  } catch (const std::exception& ex) {
+    g_create_err = ex.what();
+    return classify(g_create_err);
+  }
+}
+rrtl_status rrtl_hip_batch_run(rrtl_hip_batch* b, uint32_t mode) {
+  if (!b || !b->impl) return RRTL_E_INVALID;
+  try {
+    if (mode > RRTL_BATCH_PER_DESIGN) throw rrtl::Error("E_HIP_BATCH: unknown batch mode");
+    rrtl::batch_run(b->impl, mode == RRTL_BATCH_PERSISTENT);
+    b->err.clear();
+    return RRTL_OK;
+  } catch (const std::exception& ex) {
+    b->err = ex.what();
+    return classify(b->err);
+  }
+}
+rrtl_status rrtl_hip_batch_stats(const rrtl_hip_batch* b, rrtl_hip_run_stats* out) {
+  if (!b || !b->impl || !out) return RRTL_E_INVALID;
+  const rrtl::RunStats& s = rrtl::batch_stats(b->impl);
+  out->kernel_ms = s.kernel_ms; out->launches = s.launches; out->lane_cycles = s.lane_cycles;
+  return RRTL_OK;
+}
+const char* rrtl_hip_batch_last_error(const rrtl_hip_batch* b) { return b ? b->err.c_str() : "E_HIP_NULL: null batch"; }
+void rrtl_hip_batch_destroy(rrtl_hip_batch* b) { delete b; }
 
 }  // extern "C"

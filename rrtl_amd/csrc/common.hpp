@@ -63,9 +63,12 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+  // Keeps the allocation when it is already large enough, so device pointers
+  // handed to earlier launches (or captured graphs) stay valid across runs.
   void alloc(size_t count) {
-    release();
     if (count == 0) count = 1;
+    if (p && count <= n) return;
+    release();
     RRTL_HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
     n = count;
   }

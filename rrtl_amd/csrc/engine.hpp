@@ -19,6 +19,18 @@ struct EngineOptions {
   uint32_t ticks_per_launch = 0;   // 0: whole run in one launch
 };
 
+// One design's share of a multi-design batch launch (batch.hip): the kernel
+// arguments of its run, how many work items (gate tiles or word lane blocks)
+// it has, their LDS need and an estimate of each item's cost for ordering.
+struct BatchJob {
+  uint32_t kind = 0;          // ENGINE_GATE or ENGINE_WORD
+  uint32_t variant = 0;       // gate: tile words W; word: 1 if scratch lives in LDS
+  uint32_t n_items = 0;
+  size_t lds_bytes = 0;
+  double item_cost = 0;
+  std::vector<uint8_t> args;  // gate::TileArgs or word::WordArgs
+};
+
 class Engine {
  public:
   Engine(Program prog, uint32_t lanes, const EngineOptions& opt) : P(std::move(prog)), lanes_(lanes), opt_(opt) {}
@@ -52,6 +64,11 @@ class Engine {
   virtual void get_trace(void* out) = 0;
   virtual void get_halt_cycles(uint64_t* out) { (void)out; throw Error("E_HIP_UNSUPPORTED: engine has no halting"); }
   virtual void synchronize() { RRTL_HIP_CHECK(hipStreamSynchronize(stream_)); }
+  // Multi-design batches: compile and upload a run, return its kernel
+  // arguments; reset its outputs before every launch; book-keep after it.
+  virtual BatchJob batch_prepare(const RunSpec& spec) = 0;
+  virtual void batch_reset(hipStream_t s) = 0;
+  virtual void batch_finish(const RunSpec& spec) = 0;
   const RunStats& last_stats() const { return stats_; }
   hipStream_t stream() const { return stream_; }
 

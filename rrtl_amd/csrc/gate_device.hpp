@@ -1,0 +1,248 @@
+// Device side of the bit-sliced gate engine: gate encoding, the tile program
+// arguments and the body of the LDS-resident tile kernel.  Shared by the gate
+// engine's own launches and by the multi-design batch kernel.
+#pragma once
+#include "common.hpp"
+
+namespace rrtl {
+namespace gate {
+
+// 4-bit gate op: bit0 invert a, bit1 invert b, bit2 invert result, bit3 xor.
+// r = (x ? a ^ b : (a ^ ia) & (b ^ ib)) ^ io covers every 2-input gate kind.
+enum : uint32_t { GI_A = 1, GI_B = 2, GI_O = 4, GI_X = 8 };
+constexpr uint32_t G_AND = 0, G_OR = GI_A | GI_B | GI_O, G_XOR = GI_X, G_ANDN = GI_B, G_ORN = GI_A | GI_O;
+constexpr uint32_t G_NOT = GI_A;  // b = ones
+constexpr uint32_t G_COPY = 0;    // b = ones
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t CLAMP_COMMIT = 0xFFFFFFFEu;
+constexpr uint32_t TILE_THREADS = 256;
+constexpr uint32_t MAX_LEVEL_GATES = TILE_THREADS;  // one gate per thread per level
+
+struct CGate { uint32_t op, a, b, dst; };
+struct CCommit { uint32_t reg, val, rsig, pol, rbit, has_reset; };
+
+__host__ __device__ inline uint32_t gate_eval(uint32_t op, uint32_t a, uint32_t b) {
+  uint32_t ia = 0u - (op & 1u), ib = 0u - ((op >> 1) & 1u), io = 0u - ((op >> 2) & 1u);
+  uint32_t r = (op & GI_X) ? (a ^ b) : ((a ^ ia) & (b ^ ib));
+  return r ^ io;
+}
+
+// ------------------------------------------------------------------ kernels
+struct TileArgs {
+  uint32_t* state; uint64_t nwords;             // HBM rows [node][nwords]
+  const uint2* desc; const uint32_t* level_start;
+  uint32_t n_cycle_levels, n_post_levels;
+  const uint32_t* load_slot; const uint32_t* load_node; uint32_t n_load;
+  const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
+  const uint4* commit; uint32_t n_commit;
+  const uint32_t* obs_slot; uint32_t n_obs;
+  const uint32_t* force; const uint32_t* force_start;  // per tile: 5-word entries (pos, slot, word, and, or)
+  uint32_t* digest;                                    // [64][nwords] or null
+  uint32_t* trace;                                     // [cycles][n_obs][nwords] or null
+  uint64_t seed, cycle_base, cycles;
+  uint32_t n_slots, c0, c1, observe_post;
+};
+
+// W consecutive u32 lane-words of one slot, moved with one LDS access
+template <int W> struct Words { uint32_t v[W]; };
+
+template <int W>
+__device__ __forceinline__ Words<W> ld_words(const uint32_t* lds, uint32_t slot) {
+  Words<W> r;
+  const uint32_t* p = lds + size_t(slot) * W;
+  if constexpr (W == 1) { r.v[0] = p[0]; }
+  else if constexpr (W == 2) { uint2 x = *reinterpret_cast<const uint2*>(p); r.v[0] = x.x; r.v[1] = x.y; }
+  else {
+#pragma unroll
+    for (int k = 0; k < W; k += 4) {
+      uint4 x = *reinterpret_cast<const uint4*>(p + k);
+      r.v[k] = x.x; r.v[k + 1] = x.y; r.v[k + 2] = x.z; r.v[k + 3] = x.w;
+    }
+  }
+  return r;
+}
+
+template <int W>
+__device__ __forceinline__ void st_words(uint32_t* lds, uint32_t slot, const Words<W>& r) {
+  uint32_t* p = lds + size_t(slot) * W;
+  if constexpr (W == 1) { p[0] = r.v[0]; }
+  else if constexpr (W == 2) { *reinterpret_cast<uint2*>(p) = make_uint2(r.v[0], r.v[1]); }
+  else {
+#pragma unroll
+    for (int k = 0; k < W; k += 4) *reinterpret_cast<uint4*>(p + k) = make_uint4(r.v[k], r.v[k + 1], r.v[k + 2], r.v[k + 3]);
+  }
+}
+
+// One gate for every lane-word of the tile.
+template <int W>
+__device__ __forceinline__ void eval_gate(uint32_t* lds, uint2 d) {
+  const uint32_t op = d.y >> 16;
+  const Words<W> a = ld_words<W>(lds, d.x & 0xFFFFu), b = ld_words<W>(lds, d.x >> 16);
+  const uint32_t ia = 0u - (op & 1u), ib = 0u - ((op >> 1) & 1u), io = 0u - ((op >> 2) & 1u);
+  const bool x = (op & GI_X) != 0;
+  Words<W> r;
+#pragma unroll
+  for (int k = 0; k < W; k++) r.v[k] = (x ? (a.v[k] ^ b.v[k]) : ((a.v[k] ^ ia) & (b.v[k] ^ ib))) ^ io;
+  st_words<W>(lds, d.y & 0xFFFFu, r);
+}
+
+template <int W>
+__device__ __forceinline__ void apply_clamps(const TileArgs& A, uint32_t* lds, uint32_t pos, uint32_t fb, uint32_t fe) {
+  for (uint32_t i = fb + threadIdx.x; i < fe; i += TILE_THREADS) {
+    const uint32_t* f = &A.force[size_t(i) * 5];
+    if (f[0] != pos) continue;
+    uint32_t* v = &lds[f[1] * W + f[2]];
+    *v = (*v & f[3]) | f[4];
+  }
+}
+
+// Signature of the observed outputs: per cycle c, S[(j + c) & 63] ^= v_j for every
+// observed word v_j, then one nonlinear round S'[p] = S[p] ^ (S[p+17] & ~S[p+41]).
+// Bitwise, so every lane keeps its own 64-bit signature.
+template <int W>
+__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32_t* sig, uint64_t c, uint64_t cyc, uint64_t w0) {
+  const uint32_t tid = threadIdx.x;
+  if (A.trace)
+    for (uint32_t i = tid; i < A.n_obs * W; i += TILE_THREADS) {
+      uint32_t j = i / W, w = i % W;
+      uint64_t gw = w0 + w;
+      if (gw < A.nwords) A.trace[(c * A.n_obs + j) * A.nwords + gw] = lds[A.obs_slot[j] * W + w];
+    }
+  if (A.digest) {
+    Words<W> s;
+    if (tid < 64) {
+      s = ld_words<W>(sig, tid);
+      for (uint32_t j = (tid - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) {
+        const Words<W> v = ld_words<W>(lds, A.obs_slot[j]);
+#pragma unroll
+        for (int k = 0; k < W; k++) s.v[k] ^= v.v[k];
+      }
+      st_words<W>(sig, tid, s);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      const Words<W> a = ld_words<W>(sig, (tid + 17) & 63), b = ld_words<W>(sig, (tid + 41) & 63);
+#pragma unroll
+      for (int k = 0; k < W; k++) s.v[k] ^= a.v[k] & ~b.v[k];
+    }
+    __syncthreads();
+    if (tid < 64) st_words<W>(sig, tid, s);
+  }
+  __syncthreads();
+}
+
+template <int W>
+__device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
+  uint32_t* sig = lds + size_t(A.n_slots) * W;          // 64 x W signature words
+  uint32_t* lstart = sig + 64 * W;                      // level_start copy
+  const uint32_t tid = threadIdx.x;
+  const uint64_t w0 = uint64_t(tile) * W;
+  const uint32_t nlev = A.n_cycle_levels + A.n_post_levels;
+  const uint32_t fb = A.force_start ? A.force_start[tile] : 0, fe = A.force_start ? A.force_start[tile + 1] : 0;
+  for (uint32_t i = tid; i <= nlev; i += TILE_THREADS) lstart[i] = A.level_start[i];
+  for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
+    uint32_t s = i / W, w = i % W;
+    uint64_t gw = w0 + w;
+    lds[A.load_slot[s] * W + w] = gw < A.nwords ? A.state[A.load_node[s] * A.nwords + gw] : 0u;
+  }
+  for (uint32_t w = tid; w < W; w += TILE_THREADS) { lds[A.c0 * W + w] = 0u; lds[A.c1 * W + w] = ~0u; }
+  for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
+    uint32_t p = i / W, w = i % W;
+    uint64_t gw = w0 + w;
+    sig[i] = (A.digest && gw < A.nwords) ? A.digest[p * A.nwords + gw] : 0u;
+  }
+  __syncthreads();
+  for (uint64_t c = 0; c < A.cycles; c++) {
+    const uint64_t cyc = A.cycle_base + c;
+    // primary inputs for this cycle (thread per input, all words of the tile)
+    for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS) {
+      const uint64_t arg = A.stim_arg[s];
+      Words<W> v;
+      if (A.stim_mode[s] == IN_RANDOM) {
+        const uint64_t key = stim_key(A.seed, A.stim_stream[s], cyc);
+#pragma unroll
+        for (int k = 0; k < W; k += 2) {
+          const uint64_t gw = w0 + k;
+          const uint64_t word = stim_from_key(key, gw >> 1);
+          v.v[k] = uint32_t(word >> (32 * (gw & 1)));
+          if (k + 1 < W) v.v[k + 1] = uint32_t(word >> 32);
+        }
+      } else {
+        const uint32_t bit = A.stim_mode[s] == IN_CONST ? uint32_t(arg & 1)
+                                                        : uint32_t((cyc < (arg & 0xFFFFFFFFull)) ? (arg >> 32) & 1 : (arg >> 33) & 1);
+#pragma unroll
+        for (int k = 0; k < W; k++) v.v[k] = bit ? ~0u : 0u;
+      }
+      st_words<W>(lds, A.stim_slot[s], v);
+    }
+    __syncthreads();
+    // levels: each thread evaluates at most one gate per level; the next
+    // level's descriptor is fetched before this level's gate runs
+    uint2 dnext = make_uint2(0, 0);
+    bool hnext = false;
+    if (nlev) { const uint32_t g = lstart[0] + tid; hnext = g < lstart[1]; if (hnext) dnext = A.desc[g]; }
+    for (uint32_t L = 0; L < nlev; L++) {
+      const uint2 dcur = dnext;
+      const bool hcur = hnext;
+      hnext = false;
+      if (L + 1 < nlev) { const uint32_t g = lstart[L + 1] + tid; hnext = g < lstart[L + 2]; if (hnext) dnext = A.desc[g]; }
+      if (L == A.n_cycle_levels) {
+        // cycle program done: observe before the edge, commit, then the post program
+        if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+        for (uint32_t k = tid; k < A.n_commit; k += TILE_THREADS) {
+          const uint4 cm = A.commit[k];
+          Words<W> v = ld_words<W>(lds, cm.y);
+          if (cm.w & 4) {
+            const Words<W> a = ld_words<W>(lds, cm.z);
+            const uint32_t pol = 0u - (cm.w & 1u), rv = 0u - ((cm.w >> 1) & 1u);
+#pragma unroll
+            for (int q = 0; q < W; q++) { const uint32_t as = a.v[q] ^ pol; v.v[q] = (as & rv) | (~as & v.v[q]); }
+          }
+          st_words<W>(lds, cm.x, v);
+        }
+        __syncthreads();
+        if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
+      }
+      if (hcur) eval_gate<W>(lds, dcur);
+      __syncthreads();
+      if (fe > fb) { apply_clamps<W>(A, lds, L + 1, fb, fe); __syncthreads(); }
+    }
+    if (A.n_post_levels == 0) {
+      if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+      for (uint32_t k = tid; k < A.n_commit; k += TILE_THREADS) {
+        const uint4 cm = A.commit[k];
+        Words<W> v = ld_words<W>(lds, cm.y);
+        if (cm.w & 4) {
+          const Words<W> a = ld_words<W>(lds, cm.z);
+          const uint32_t pol = 0u - (cm.w & 1u), rv = 0u - ((cm.w >> 1) & 1u);
+#pragma unroll
+          for (int q = 0; q < W; q++) { const uint32_t as = a.v[q] ^ pol; v.v[q] = (as & rv) | (~as & v.v[q]); }
+        }
+        st_words<W>(lds, cm.x, v);
+      }
+      __syncthreads();
+      if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
+    }
+    if (A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+  }
+  for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
+    uint32_t s = i / W, w = i % W;
+    uint64_t gw = w0 + w;
+    if (gw < A.nwords) A.state[A.load_node[s] * A.nwords + gw] = lds[A.load_slot[s] * W + w];
+  }
+  if (A.digest)
+    for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
+      uint32_t p = i / W, w = i % W;
+      uint64_t gw = w0 + w;
+      if (gw < A.nwords) A.digest[p * A.nwords + gw] = sig[i];
+    }
+}
+
+template <int W>
+__global__ void __launch_bounds__(TILE_THREADS) tile_kernel(TileArgs A) {
+  extern __shared__ __align__(16) uint32_t lds[];
+  tile_body<W>(A, blockIdx.x, lds);
+}
+
+}  // namespace gate
+}  // namespace rrtl

@@ -1,0 +1,71 @@
+"""Multi-design batches: one persistent graph-captured launch must give every
+design exactly what its own run gives (digests, traces, final state)."""
+import numpy as np
+import pytest
+
+import oracle
+from rrtl_amd import designs, lower
+from rrtl_amd.backend import IN_CONST, IN_PULSE, IN_RANDOM, Batch, HipSimulator
+from rrtl_amd.program import KIND_INPUT, KIND_OUTPUT
+from conftest import flush_print
+from designgen import random_design
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_jobs():
+    jobs = []
+    p = lower.lower_module(designs.counter())
+    jobs.append((p, 2000, "word", dict(cycles=3000, seed=1, observe=[p.signal_index("out")],
+                                       inputs=[(p.signal_index("clk"), IN_CONST, 1),
+                                               (p.signal_index("rst"), IN_PULSE, oracle.pulse_arg(1, 1, 0)),
+                                               (p.signal_index("en"), IN_RANDOM, 0)])))
+    g = lower.lower_module(designs.synthetic_netlist(gates=4000, levels=32, dffs=400, inputs=64, outputs=32, seed=7))
+    jobs.append((g, 4096, "gate", dict(cycles=500, seed=2,
+                                       inputs=[(i, IN_RANDOM, 0) for i, s in enumerate(g.signals) if s.kind == KIND_INPUT],
+                                       observe=[i for i, s in enumerate(g.signals) if s.kind == KIND_OUTPUT])))
+    f = lower.lower_module(designs.ready_valid_fifo())
+    jobs.append((f, 1000, "word", dict(cycles=700, seed=3,
+                                       inputs=[(f.signal_index("clk"), IN_CONST, 1),
+                                               (f.signal_index("rst"), IN_PULSE, oracle.pulse_arg(1, 1, 0)),
+                                               (f.signal_index("in_valid"), IN_RANDOM, 0),
+                                               (f.signal_index("out_ready"), IN_RANDOM, 0),
+                                               (f.signal_index("in_data"), IN_RANDOM, 0)],
+                                       observe=[f.signal_index(n) for n in ("out_valid", "out_data", "count")])))
+    r = lower.lower_module(random_design(77, async_resets=False))
+    jobs.append((r, 300, "word", dict(cycles=150, seed=4,
+                                      inputs=[(i, IN_RANDOM, 0) for i, s in enumerate(r.signals) if s.kind == KIND_INPUT],
+                                      observe=[i for i, s in enumerate(r.signals) if s.kind != KIND_INPUT])))
+    return jobs
+
+
+def test_batch_matches_individual_runs():
+    flush_print("case: mixed batch persistent vs individual")
+    jobs = mixed_jobs()
+    solo = [HipSimulator(p, lanes, engine=eng) for p, lanes, eng, _ in jobs]
+    both = [HipSimulator(p, lanes, engine=eng) for p, lanes, eng, _ in jobs]
+    batch = Batch([(s, kw) for s, (_, _, _, kw) in zip(both, jobs)])
+    for rep in range(2):  # the second launch replays the captured graph
+        for s, (_, _, _, kw) in zip(solo, jobs):
+            s.run(**kw)
+        st = batch.run(persistent=True)
+        assert st["launches"] == 1
+        assert st["lane_cycles"] == sum(lanes * kw["cycles"] for _, lanes, _, kw in jobs)
+        for a, b in zip(solo, both):
+            assert np.array_equal(a.digest(), b.digest()), (rep, a.kind)
+            va, ma = a.snapshot()
+            vb, mb = b.snapshot()
+            assert np.array_equal(va, vb) and np.array_equal(ma, mb), (rep, a.kind)
+
+
+def test_batch_per_design_mode_matches():
+    jobs = mixed_jobs()[:2]
+    sims = [HipSimulator(p, lanes, engine=eng) for p, lanes, eng, _ in jobs]
+    ref = [HipSimulator(p, lanes, engine=eng) for p, lanes, eng, _ in jobs]
+    batch = Batch([(s, kw) for s, (_, _, _, kw) in zip(sims, jobs)])
+    st = batch.run(persistent=False)
+    assert st["launches"] == 2
+    for s, (_, _, _, kw) in zip(ref, jobs):
+        s.run(**kw)
+    for a, b in zip(sims, ref):
+        assert np.array_equal(a.digest(), b.digest())

@@ -235,6 +235,10 @@ def test_errors_are_reported():
     with pytest.raises(HipError) as e:
         sim.get_signal_limbs(999)
     assert e.value.code == "E_SIM_IR_SIGNAL"
+    # the library's own check, below the binding
+    buf = np.zeros(64, np.uint32)
+    assert sim.L.rrtl_hip_get_signal_limbs(sim.h, 999, buf.ctypes.data) == 1
+    assert sim.L.rrtl_hip_last_error(sim.h).decode().startswith("E_SIM_IR_SIGNAL")
     with pytest.raises(HipError) as e:
         sim.set_force(0, 0, 1)
     assert e.value.code == "E_HIP_UNSUPPORTED"
