@@ -59,10 +59,14 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs B) {
     const JobDev& J = B.jobs[wi.x];
     if (J.kind == ENGINE_GATE) {
       switch (J.variant) {
-        case 1: gate::tile_body<1>(J.u.g, wi.y, lds); break;
-        case 2: gate::tile_body<2>(J.u.g, wi.y, lds); break;
-        case 4: gate::tile_body<4>(J.u.g, wi.y, lds); break;
-        default: gate::tile_body<8>(J.u.g, wi.y, lds); break;
+        case 1: gate::tile_body<1, false>(J.u.g, wi.y, lds); break;
+        case 2: gate::tile_body<2, false>(J.u.g, wi.y, lds); break;
+        case 4: gate::tile_body<4, false>(J.u.g, wi.y, lds); break;
+        case 8: gate::tile_body<8, false>(J.u.g, wi.y, lds); break;
+        case 1 | gate::GATE_VARIANT_BYTE_ADDR: gate::tile_body<1, true>(J.u.g, wi.y, lds); break;
+        case 2 | gate::GATE_VARIANT_BYTE_ADDR: gate::tile_body<2, true>(J.u.g, wi.y, lds); break;
+        case 4 | gate::GATE_VARIANT_BYTE_ADDR: gate::tile_body<4, true>(J.u.g, wi.y, lds); break;
+        default: gate::tile_body<8, true>(J.u.g, wi.y, lds); break;
       }
     } else if (J.variant) {
       word::word_body<true>(J.u.w, wi.y, lds);

@@ -73,17 +73,59 @@ __device__ __forceinline__ void st_words(uint32_t* lds, uint32_t slot, const Wor
   }
 }
 
-// One gate for every lane-word of the tile.
+// Gate descriptor, one per thread and level (8 bytes): x = a | b << 16,
+// y = dst | op << 16.  a, b and dst are LDS byte offsets when the tile fits in
+// 64 KiB (kBA) and slot indices otherwise.  XOR-family ops carry no input
+// inversions (they cancel into the output inversion).  A thread with no gate in
+// a level reads the constant-0 slot and writes one of 64 sink slots (thread %
+// 64), so every thread runs the same straight-line code.
+template <int W, bool kBA>
+__device__ __forceinline__ uint32_t* lds_at(uint32_t* lds, uint32_t field) {
+  return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + (kBA ? field : field * uint32_t(W * 4)));
+}
+
 template <int W>
+__device__ __forceinline__ Words<W> ld_words_p(const uint32_t* p) {
+  Words<W> r;
+  if constexpr (W == 1) { r.v[0] = p[0]; }
+  else if constexpr (W == 2) { uint2 x = *reinterpret_cast<const uint2*>(p); r.v[0] = x.x; r.v[1] = x.y; }
+  else {
+#pragma unroll
+    for (int k = 0; k < W; k += 4) {
+      uint4 x = *reinterpret_cast<const uint4*>(p + k);
+      r.v[k] = x.x; r.v[k + 1] = x.y; r.v[k + 2] = x.z; r.v[k + 3] = x.w;
+    }
+  }
+  return r;
+}
+
+// x ? A ^ B : A & B in one v_bitop3_b32.  The truth table indexes
+// (S0 << 2 | S1 << 1 | S2) = (A, B, x): evaluated on A = 0xF0, B = 0xCC, x = 0xAA
+// it is (0xAA & 0x3C) | (0x55 & 0xC0) = 0x68.  Written as the builtin because
+// the compiler turns the bitwise form into per-word selects.
+__device__ __forceinline__ uint32_t op3_xor_or_and(uint32_t A, uint32_t B, uint32_t x) {
+  return __builtin_amdgcn_bitop3_b32(A, B, x, 0x68);
+}
+
+// One gate for every lane-word of the tile:
+//   r = (x ? A ^ B : A & B) ^ io,  A = a ^ ia, B = b ^ ib  (per-thread masks).
+template <int W, bool kBA>
 __device__ __forceinline__ void eval_gate(uint32_t* lds, uint2 d) {
-  const uint32_t op = d.y >> 16;
-  const Words<W> a = ld_words<W>(lds, d.x & 0xFFFFu), b = ld_words<W>(lds, d.x >> 16);
-  const uint32_t ia = 0u - (op & 1u), ib = 0u - ((op >> 1) & 1u), io = 0u - ((op >> 2) & 1u);
-  const bool x = (op & GI_X) != 0;
+  const Words<W> a = ld_words_p<W>(lds_at<W, kBA>(lds, d.x & 0xFFFFu));
+  const Words<W> b = ld_words_p<W>(lds_at<W, kBA>(lds, d.x >> 16));
+  const uint32_t y = d.y;
+  const uint32_t ia = 0u - ((y >> 16) & 1u), ib = 0u - ((y >> 17) & 1u), io = 0u - ((y >> 18) & 1u),
+                 x = 0u - ((y >> 19) & 1u);
   Words<W> r;
 #pragma unroll
-  for (int k = 0; k < W; k++) r.v[k] = (x ? (a.v[k] ^ b.v[k]) : ((a.v[k] ^ ia) & (b.v[k] ^ ib))) ^ io;
-  st_words<W>(lds, d.y & 0xFFFFu, r);
+  for (int k = 0; k < W; k++) r.v[k] = op3_xor_or_and(a.v[k] ^ ia, b.v[k] ^ ib, x) ^ io;
+  uint32_t* o = lds_at<W, kBA>(lds, y & 0xFFFFu);
+  if constexpr (W == 1) { o[0] = r.v[0]; }
+  else if constexpr (W == 2) { *reinterpret_cast<uint2*>(o) = make_uint2(r.v[0], r.v[1]); }
+  else {
+#pragma unroll
+    for (int k = 0; k < W; k += 4) *reinterpret_cast<uint4*>(o + k) = make_uint4(r.v[k], r.v[k + 1], r.v[k + 2], r.v[k + 3]);
+  }
 }
 
 template <int W>
@@ -131,15 +173,73 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32
   __syncthreads();
 }
 
+constexpr uint32_t PREFETCH = 4;       // levels of descriptor look-ahead
+constexpr uint32_t SINK_SLOTS = 64;     // per-tile slots written by threads with no gate
+constexpr uint32_t GATE_VARIANT_BYTE_ADDR = 0x100;  // batch job variant flag: descriptors hold byte offsets
+
 template <int W>
+__device__ __forceinline__ void commit_regs(const TileArgs& A, uint32_t* lds, uint32_t fb, uint32_t fe) {
+  for (uint32_t k = threadIdx.x; k < A.n_commit; k += TILE_THREADS) {
+    const uint4 cm = A.commit[k];
+    Words<W> v = ld_words<W>(lds, cm.y);
+    if (cm.w & 4) {
+      const Words<W> a = ld_words<W>(lds, cm.z);
+      const uint32_t pol = 0u - (cm.w & 1u), rv = 0u - ((cm.w >> 1) & 1u);
+#pragma unroll
+      for (int q = 0; q < W; q++) { const uint32_t as = a.v[q] ^ pol; v.v[q] = (as & rv) | (~as & v.v[q]); }
+    }
+    st_words<W>(lds, cm.x, v);
+  }
+  __syncthreads();
+  if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
+}
+
+// One level: this thread's gate (descriptor fetched PREFETCH levels earlier),
+// then the fetch for PREFETCH levels later into the same registers (the
+// descriptor table carries PREFETCH padding levels, so the fetch needs no
+// bound check), the level barrier and, when forces exist, their clamps.  No
+// other global access happens in this loop, so the wait before a gate leaves
+// the PREFETCH-1 younger fetches in flight.
+template <int W, bool kBA, bool kClamp>
+__device__ __forceinline__ void level_stage(const TileArgs& A, uint32_t* lds, uint32_t L, const uint2* ahead, uint2& d,
+                                            uint32_t fb, uint32_t fe) {
+  eval_gate<W, kBA>(lds, d);
+  d = ahead[threadIdx.x];
+  __syncthreads();
+  if (kClamp) { apply_clamps<W>(A, lds, L + 1, fb, fe); __syncthreads(); }
+}
+
+// Levels [L0, L1): unrolled by PREFETCH so each of the four descriptor
+// registers is loaded and consumed in a fixed stage (no register rotation).
+// `row` is the uniform address of a level's descriptor row, so a fetch is one
+// scalar-base load with the thread's fixed offset.
+template <int W, bool kBA, bool kClamp>
+__device__ __forceinline__ void run_levels(const TileArgs& A, uint32_t* lds, uint32_t L0, uint32_t L1, uint32_t fb,
+                                           uint32_t fe) {
+  static_assert(PREFETCH == 4, "run_levels is unrolled for four descriptor registers");
+  const uint32_t tid = threadIdx.x;
+  const uint2* row = A.desc + size_t(L0) * TILE_THREADS;
+  uint2 d0 = row[tid], d1 = row[TILE_THREADS + tid], d2 = row[2 * TILE_THREADS + tid], d3 = row[3 * TILE_THREADS + tid];
+  row += PREFETCH * TILE_THREADS;
+  uint32_t L = L0;
+  for (; L + 4 <= L1; L += 4, row += 4 * TILE_THREADS) {
+    level_stage<W, kBA, kClamp>(A, lds, L + 0, row, d0, fb, fe);
+    level_stage<W, kBA, kClamp>(A, lds, L + 1, row + TILE_THREADS, d1, fb, fe);
+    level_stage<W, kBA, kClamp>(A, lds, L + 2, row + 2 * TILE_THREADS, d2, fb, fe);
+    level_stage<W, kBA, kClamp>(A, lds, L + 3, row + 3 * TILE_THREADS, d3, fb, fe);
+  }
+  if (L < L1) level_stage<W, kBA, kClamp>(A, lds, L, row, d0, fb, fe);
+  if (L + 1 < L1) level_stage<W, kBA, kClamp>(A, lds, L + 1, row + TILE_THREADS, d1, fb, fe);
+  if (L + 2 < L1) level_stage<W, kBA, kClamp>(A, lds, L + 2, row + 2 * TILE_THREADS, d2, fb, fe);
+}
+
+template <int W, bool kBA>
 __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
   uint32_t* sig = lds + size_t(A.n_slots) * W;          // 64 x W signature words
-  uint32_t* lstart = sig + 64 * W;                      // level_start copy
   const uint32_t tid = threadIdx.x;
   const uint64_t w0 = uint64_t(tile) * W;
   const uint32_t nlev = A.n_cycle_levels + A.n_post_levels;
   const uint32_t fb = A.force_start ? A.force_start[tile] : 0, fe = A.force_start ? A.force_start[tile + 1] : 0;
-  for (uint32_t i = tid; i <= nlev; i += TILE_THREADS) lstart[i] = A.level_start[i];
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
     uint32_t s = i / W, w = i % W;
     uint64_t gw = w0 + w;
@@ -176,52 +276,14 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
       st_words<W>(lds, A.stim_slot[s], v);
     }
     __syncthreads();
-    // levels: each thread evaluates at most one gate per level; the next
-    // level's descriptor is fetched before this level's gate runs
-    uint2 dnext = make_uint2(0, 0);
-    bool hnext = false;
-    if (nlev) { const uint32_t g = lstart[0] + tid; hnext = g < lstart[1]; if (hnext) dnext = A.desc[g]; }
-    for (uint32_t L = 0; L < nlev; L++) {
-      const uint2 dcur = dnext;
-      const bool hcur = hnext;
-      hnext = false;
-      if (L + 1 < nlev) { const uint32_t g = lstart[L + 1] + tid; hnext = g < lstart[L + 2]; if (hnext) dnext = A.desc[g]; }
-      if (L == A.n_cycle_levels) {
-        // cycle program done: observe before the edge, commit, then the post program
-        if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
-        for (uint32_t k = tid; k < A.n_commit; k += TILE_THREADS) {
-          const uint4 cm = A.commit[k];
-          Words<W> v = ld_words<W>(lds, cm.y);
-          if (cm.w & 4) {
-            const Words<W> a = ld_words<W>(lds, cm.z);
-            const uint32_t pol = 0u - (cm.w & 1u), rv = 0u - ((cm.w >> 1) & 1u);
-#pragma unroll
-            for (int q = 0; q < W; q++) { const uint32_t as = a.v[q] ^ pol; v.v[q] = (as & rv) | (~as & v.v[q]); }
-          }
-          st_words<W>(lds, cm.x, v);
-        }
-        __syncthreads();
-        if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
-      }
-      if (hcur) eval_gate<W>(lds, dcur);
-      __syncthreads();
-      if (fe > fb) { apply_clamps<W>(A, lds, L + 1, fb, fe); __syncthreads(); }
-    }
-    if (A.n_post_levels == 0) {
-      if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
-      for (uint32_t k = tid; k < A.n_commit; k += TILE_THREADS) {
-        const uint4 cm = A.commit[k];
-        Words<W> v = ld_words<W>(lds, cm.y);
-        if (cm.w & 4) {
-          const Words<W> a = ld_words<W>(lds, cm.z);
-          const uint32_t pol = 0u - (cm.w & 1u), rv = 0u - ((cm.w >> 1) & 1u);
-#pragma unroll
-          for (int q = 0; q < W; q++) { const uint32_t as = a.v[q] ^ pol; v.v[q] = (as & rv) | (~as & v.v[q]); }
-        }
-        st_words<W>(lds, cm.x, v);
-      }
-      __syncthreads();
-      if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
+    // cycle program, observe before the edge, commit, post program
+    if (fe > fb) run_levels<W, kBA, true>(A, lds, 0, A.n_cycle_levels, fb, fe);
+    else run_levels<W, kBA, false>(A, lds, 0, A.n_cycle_levels, fb, fe);
+    if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+    commit_regs<W>(A, lds, fb, fe);
+    if (A.n_post_levels) {
+      if (fe > fb) run_levels<W, kBA, true>(A, lds, A.n_cycle_levels, nlev, fb, fe);
+      else run_levels<W, kBA, false>(A, lds, A.n_cycle_levels, nlev, fb, fe);
     }
     if (A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
   }
@@ -238,10 +300,10 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
     }
 }
 
-template <int W>
+template <int W, bool kBA>
 __global__ void __launch_bounds__(TILE_THREADS) tile_kernel(TileArgs A) {
   extern __shared__ __align__(16) uint32_t lds[];
-  tile_body<W>(A, blockIdx.x, lds);
+  tile_body<W, kBA>(A, blockIdx.x, lds);
 }
 
 }  // namespace gate

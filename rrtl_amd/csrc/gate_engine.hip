@@ -607,17 +607,20 @@ class GateEngine final : public Engine {
     return reinterpret_cast<T*>(b.p);
   }
 
+  // LDS of one tile: live slots, the sink slots of gate-less threads, the
+  // 64-word signature.
+  static size_t tile_lds_bytes(uint32_t n_slots, uint32_t w) { return size_t(n_slots + SINK_SLOTS + 64) * w * 4; }
+
   uint32_t choose_tile_words(uint32_t n_slots) const {
     if (opt_.tile_words) return opt_.tile_words;
     // largest tile that still leaves room for two workgroups per CU
-    for (uint32_t w : {8u, 4u, 2u, 1u}) {
-      size_t bytes = size_t(n_slots + 64) * w * 4;
-      if (bytes * 2 <= lds_max_ || w == 1) return w;
-    }
+    for (uint32_t w : {8u, 4u, 2u, 1u})
+      if (tile_lds_bytes(n_slots, w) * 2 <= lds_max_) return w;
     return 1;
   }
 
-  struct Prepared { TileArgs A; uint32_t W = 0, tiles = 0; size_t lds = 0; };
+  // W lane-words per tile; byte_addr: descriptors hold LDS byte offsets.
+  struct Prepared { TileArgs A; uint32_t W = 0, tiles = 0; size_t lds = 0; bool byte_addr = false; };
 
   // The compiled tile program depends on which inputs are driven, what is
   // observed and where forces sit; repeated runs with the same shape reuse it.
@@ -636,7 +639,9 @@ class GateEngine final : public Engine {
     for (uint32_t s : spec.obs_sig) check_sig(s);
     for (uint32_t s : spec.in_sig) check_sig(s);
     if (spec.halt_sig != NONE) throw Error("E_HIP_UNSUPPORTED: the gate engine does not halt lanes");
-    refresh_if_stale();
+    // No settle of stale wires here: a tile run loads only nodes the settle
+    // never writes (inputs, registers, host-driven wires) and recomputes every
+    // wire it reads, so stale wire rows stay stale until an API read.
     const std::string key = plan_key(spec);
     Prepared R;
     if (key == plan_key_ && have_plan_) {
@@ -667,12 +672,14 @@ class GateEngine final : public Engine {
     TileProgram T = compile_tile(N, P, spec, forced);
     uint32_t W = choose_tile_words(T.n_slots);
     if (W != 1 && W != 2 && W != 4 && W != 8) throw Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
-    const size_t lev_words = T.level_start.size() + 3;
-    size_t lds = (size_t(T.n_slots + 64) * W + lev_words) * 4;
+    size_t lds = tile_lds_bytes(T.n_slots, W);
     if (lds > lds_max_) {
-      W = 1; lds = (size_t(T.n_slots + 64) + lev_words) * 4;
+      W = 1; lds = tile_lds_bytes(T.n_slots, 1);
       if (lds > lds_max_) throw Error("E_HIP_TOO_LARGE: live gate state does not fit in LDS for one 32-lane tile");
     }
+    const uint32_t sink = T.n_slots, n_slots = T.n_slots + SINK_SLOTS;
+    if (n_slots > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
+    const bool byte_addr = lds <= 65536;
     W_ = W; last_slots_ = T.n_slots; last_levels_ = T.n_cycle_levels + T.n_post_levels; last_gates_ = uint32_t(T.desc.size() / 2);
     const uint32_t tiles = uint32_t((nwords_ + W - 1) / W);
     // per-tile force clamp entries (pos, slot, word, and, or), merged per LDS word
@@ -701,8 +708,22 @@ class GateEngine final : public Engine {
     }
     TileArgs A{};
     A.state = state_.p; A.nwords = nwords_;
-    std::vector<uint2> desc(T.desc.size() / 2);
-    for (size_t g = 0; g < desc.size(); g++) desc[g] = make_uint2(T.desc[2 * g], T.desc[2 * g + 1]);
+    // descriptors at a fixed stride per level (level * TILE_THREADS + thread),
+    // PREFETCH look-ahead levels of padding at the end; a thread without a
+    // gate reads constant 0 and writes its sink slot
+    const uint32_t nlev = T.n_cycle_levels + T.n_post_levels;
+    auto enc = [&](uint32_t slot) { return byte_addr ? slot * W * 4 : slot; };
+    std::vector<uint2> desc(size_t(nlev + PREFETCH) * TILE_THREADS);
+    for (size_t i = 0; i < desc.size(); i++)
+      desc[i] = make_uint2(enc(T.c0) | (enc(T.c0) << 16), enc(sink + uint32_t(i % SINK_SLOTS)));
+    for (uint32_t L = 0; L < nlev; L++)
+      for (uint32_t g = T.level_start[L]; g < T.level_start[L + 1]; g++) {
+        const uint32_t x = T.desc[2 * g], y = T.desc[2 * g + 1];
+        uint32_t op = y >> 16;
+        if (op & GI_X) op &= ~(GI_A | GI_B);  // xor ignores input inversions
+        desc[size_t(L) * TILE_THREADS + (g - T.level_start[L])] =
+            make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16));
+      }
     A.desc = upload_prog(0, desc);
     A.level_start = upload_prog(1, T.level_start);
     A.n_cycle_levels = T.n_cycle_levels; A.n_post_levels = T.n_post_levels;
@@ -715,9 +736,9 @@ class GateEngine final : public Engine {
     A.obs_slot = upload_prog(9, T.obs_slot); A.n_obs = uint32_t(T.obs_slot.size());
     A.force = fent.empty() ? nullptr : upload_prog(10, fent);
     A.force_start = fent.empty() ? nullptr : upload_prog(11, fstart);
-    A.n_slots = T.n_slots; A.c0 = T.c0; A.c1 = T.c1;
+    A.n_slots = n_slots; A.c0 = T.c0; A.c1 = T.c1;
     Prepared R;
-    R.A = A; R.W = W; R.tiles = tiles; R.lds = lds;
+    R.A = A; R.W = W; R.tiles = tiles; R.lds = lds; R.byte_addr = byte_addr;
     return R;
   }
 
@@ -734,7 +755,7 @@ class GateEngine final : public Engine {
       A.cycle_base = done;
       A.cycles = std::min<uint64_t>(chunk, spec.cycles - done);
       if (A.trace) A.trace = trace_.p + done * A.n_obs * nwords_;
-      launch_tile(W, tiles, lds, A);
+      launch_tile(W, R.byte_addr, tiles, lds, A);
       stats_.launches++;
     }
     RRTL_HIP_CHECK(hipEventRecord(ev1_, stream_));
@@ -743,14 +764,14 @@ class GateEngine final : public Engine {
     RRTL_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
     stats_.kernel_ms = ms;
     stats_.lane_cycles = uint64_t(lanes_) * spec.cycles;
-    wires_stale_ = spec.cycles > 0;
+    wires_stale_ = wires_stale_ || spec.cycles > 0;
   }
 
   BatchJob batch_prepare(const RunSpec& spec) override {
     Prepared R = prepare(spec, true);
     batch_digest_ = R.A.digest != nullptr;
     BatchJob J;
-    J.kind = ENGINE_GATE; J.variant = R.W; J.n_items = R.tiles; J.lds_bytes = R.lds;
+    J.kind = ENGINE_GATE; J.variant = R.W | (R.byte_addr ? GATE_VARIANT_BYTE_ADDR : 0u); J.n_items = R.tiles; J.lds_bytes = R.lds;
     // per tile and cycle: one barrier-separated pass per level plus its gates
     J.item_cost = double(spec.cycles) * (double(R.A.n_cycle_levels + R.A.n_post_levels) * 64.0 +
                                          double(last_gates_) * R.W / 16.0);
@@ -765,22 +786,26 @@ class GateEngine final : public Engine {
   void batch_finish(const RunSpec& spec) override {
     stats_ = RunStats{};
     stats_.lane_cycles = uint64_t(lanes_) * spec.cycles;
-    wires_stale_ = spec.cycles > 0;
+    wires_stale_ = wires_stale_ || spec.cycles > 0;
   }
 
-  template <int W>
+  template <int W, bool kBA>
   void launch_w(uint32_t tiles, size_t lds, const TileArgs& A) {
     if (lds > 65536)
-      RRTL_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_kernel<W>),
+      RRTL_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_kernel<W, kBA>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL(tile_kernel<W>, dim3(tiles), dim3(TILE_THREADS), lds, stream_, A);
+    hipLaunchKernelGGL((tile_kernel<W, kBA>), dim3(tiles), dim3(TILE_THREADS), lds, stream_, A);
   }
-  void launch_tile(uint32_t W, uint32_t tiles, size_t lds, const TileArgs& A) {
-    switch (W) {
-      case 1: launch_w<1>(tiles, lds, A); break;
-      case 2: launch_w<2>(tiles, lds, A); break;
-      case 4: launch_w<4>(tiles, lds, A); break;
-      default: launch_w<8>(tiles, lds, A); break;
+  void launch_tile(uint32_t W, bool byte_addr, uint32_t tiles, size_t lds, const TileArgs& A) {
+    switch (W * 2 + (byte_addr ? 1 : 0)) {
+      case 2: launch_w<1, false>(tiles, lds, A); break;
+      case 3: launch_w<1, true>(tiles, lds, A); break;
+      case 4: launch_w<2, false>(tiles, lds, A); break;
+      case 5: launch_w<2, true>(tiles, lds, A); break;
+      case 8: launch_w<4, false>(tiles, lds, A); break;
+      case 9: launch_w<4, true>(tiles, lds, A); break;
+      case 16: launch_w<8, false>(tiles, lds, A); break;
+      default: launch_w<8, true>(tiles, lds, A); break;
     }
     RRTL_HIP_CHECK(hipGetLastError());
   }
