@@ -127,6 +127,7 @@ def main() -> int:
                "sample": f"{args.lanes} lanes x {cyc} cycles of the same workload ({secs:.1f} s)",
                "bit_exact_vs_gpu": same}
 
+    traffic = measured_traffic(args.lanes, args.cycles, info["tile_words"])
     line = {
         "metric": "lane-cycles/sec at 1 GPU (bit-exact) + % HBM roofline vs host batch-SIMD",
         "value": value, "unit": "lane-cycles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -137,13 +138,33 @@ def main() -> int:
                    "lanes": args.lanes, "cycles": args.cycles, "engine": "gate", "tile_words": info["tile_words"],
                    "live_slots": info["live_slots"], "levels": info["levels"], "gates_per_cycle": info["gates"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None, "traffic_detail": traffic,
                      "b_alg_bytes_per_lane_cycle": B_ALG, "kernel_ms_avg": avg_kernel_s * 1e3},
         "cpu_baseline": cpu,
         "check": check,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def measured_traffic(lanes: int, cycles: int, tile_words: int):
+    """HBM bytes of one launch from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE of the timed kernel on this workload, each scaled by what the
+    same counter reads for a device copy of known size, per lane-cycle, times
+    this launch's lane-cycles.  None when no such profile exists."""
+    path = os.path.join(ROOT, "profiles", "gate_tile_kernel_20k_netlist_hbm.json")
+    if not os.path.exists(path) or tile_words != 2:
+        return None
+    with open(path) as f:
+        m = json.load(f)
+    cal = m["copy_calibration"]
+    fetch_scale = cal["bytes"] / 1024.0 / cal["fetch_size_kib"]
+    write_scale = cal["bytes"] / 1024.0 / cal["write_size_kib"]
+    per_lane_cycle = 1024.0 * (m["fetch_size_kib"] * fetch_scale + m["write_size_kib"] * write_scale) / (
+        m["lanes"] * m["cycles"])
+    return {"bytes": per_lane_cycle * lanes * cycles, "bytes_per_lane_cycle": per_lane_cycle,
+            "fetch_scale": fetch_scale, "write_scale": write_scale, "profiled": m["workload"]}
 
 
 def calibrate_copy(nbytes: int) -> int:

@@ -52,6 +52,11 @@ typedef struct rrtl_hip_options {
   uint32_t block_threads;    /* word engine workgroup size, 0 = default (256) */
   uint32_t tile_words;       /* gate engine u32 lane-words per workgroup (1,2,4,8), 0 = auto */
   uint32_t ticks_per_launch; /* cycles per kernel launch in runs, 0 = whole run in one launch */
+  /* Gate engine runs with more tiles than fit on the device at once: each
+   * tile's cycles are cut into this many chunks handed out by a work queue, so
+   * the last wave of tiles does not run alone.  0 = automatic, 1 = off.  Read
+   * only when struct_size covers it. */
+  uint32_t tile_chunks;
 } rrtl_hip_options;
 
 /* SimdCpuSimulator::new (lib.rs:5529): lanes == 0 is E_SIM_IR_LANES; the

@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs B) {
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(B.counter, 1u);
     __syncthreads();
-    const uint32_t it = s_item;
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);  // uniform exit branch
     __syncthreads();
     if (it >= B.n_items) break;  // uniform: every wave of the workgroup leaves together
     const uint2 wi = B.items[it];

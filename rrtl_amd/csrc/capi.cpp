@@ -1,6 +1,7 @@
 // C ABI over the engines (include/rrtl_hip.h).  Every entry point catches C++
 // exceptions and turns them into a status plus an "E_..." message.
 #include "../../include/rrtl_hip.h"
+#include <cstddef>
 #include "engine.hpp"
 #include <cstring>
 #include <memory>
@@ -84,9 +85,12 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
     rrtl::EngineOptions opt;
     uint32_t kind = RRTL_HIP_AUTO;
     if (options) {
-      if (options->struct_size < sizeof(rrtl_hip_options)) throw rrtl::Error("E_HIP_OPTIONS: options struct is too small");
+      if (options->struct_size < offsetof(rrtl_hip_options, tile_chunks))
+        throw rrtl::Error("E_HIP_OPTIONS: options struct is too small");
       kind = options->engine; opt.device = options->device; opt.block_threads = options->block_threads;
       opt.tile_words = options->tile_words; opt.ticks_per_launch = options->ticks_per_launch;
+      if (options->struct_size >= offsetof(rrtl_hip_options, tile_chunks) + sizeof(uint32_t))
+        opt.tile_chunks = options->tile_chunks;
     }
     if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 1024))
       throw rrtl::Error("E_HIP_OPTIONS: block_threads must be a multiple of 64 up to 1024");

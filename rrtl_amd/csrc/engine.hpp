@@ -17,6 +17,7 @@ struct EngineOptions {
   uint32_t block_threads = 0;      // 0: engine default
   uint32_t tile_words = 0;         // gate engine: u32 lane-words per workgroup tile (0: autotune)
   uint32_t ticks_per_launch = 0;   // 0: whole run in one launch
+  uint32_t tile_chunks = 0;        // gate engine: work-queue chunks per tile (0: automatic, 1: off)
 };
 
 // One design's share of a multi-design batch launch (batch.hip): the kernel

@@ -306,5 +306,74 @@ __global__ void __launch_bounds__(TILE_THREADS) tile_kernel(TileArgs A) {
   tile_body<W, kBA>(A, blockIdx.x, lds);
 }
 
+// Work-queue schedule for runs with more tiles than the device holds at once.
+// Item i is chunk k = i / n_tiles of tile i % n_tiles: cycles [k * chunk,
+// (k + 1) * chunk) of that tile, continuing from the state and signature that
+// chunk k - 1 left in HBM.  Items are taken in index order, so the chunk an item
+// waits for was taken earlier by a running workgroup and always finishes; the
+// wait is still bounded by a wall-clock limit which, if reached, raises the
+// error flag and lets every remaining item drain without work.
+struct QueueArgs {
+  uint32_t* ctl;          // [0] next item, [1] error flag, [2 + tile] chunks finished
+  uint32_t n_tiles, n_chunks;
+  uint64_t chunk;         // cycles per chunk
+  uint64_t wait_limit;    // wall-clock ticks
+};
+
+// Control flow: thread 0's work of an iteration (publish the item finished
+// last, take the next one; later the dependency wait) sits in single regions,
+// and the values every thread branches on are made wave-uniform with
+// readfirstlane, so the loop exit and the skip are uniform branches.  (With a
+// thread-0 region on each side of the back edge and an LDS-read exit test the
+// compiler may build an inner exec-mask loop that runs the other lanes again on
+// the old item while thread 0 waits at its exit.)
+template <int W, bool kBA>
+__global__ void __launch_bounds__(TILE_THREADS) tile_queue_kernel(TileArgs A, QueueArgs Q) {
+  extern __shared__ __align__(16) uint32_t lds[];
+  __shared__ uint32_t s_item, s_err;
+  const uint32_t total = Q.n_tiles * Q.n_chunks;
+  uint32_t last_tile = NONE, last_k = 0;  // item this workgroup finished last
+  for (;;) {
+    if (threadIdx.x == 0) {
+      // every thread fenced its stores of that item before the loop-end barrier
+      if (last_tile != NONE)
+        __hip_atomic_store(&Q.ctl[2 + last_tile], last_k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      s_item = atomicAdd(&Q.ctl[0], 1u);
+    }
+    __syncthreads();
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (it >= total) break;
+    const uint32_t k = it / Q.n_tiles, tile = it - k * Q.n_tiles;
+    if (threadIdx.x == 0) {
+      uint32_t err = __hip_atomic_load(&Q.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (k > 0 && !err) {
+        const uint64_t t0 = wall_clock64();
+        while (__hip_atomic_load(&Q.ctl[2 + tile], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < k) {
+          if (wall_clock64() - t0 > Q.wait_limit) {
+            __hip_atomic_store(&Q.ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            err = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(8);
+        }
+      }
+      s_err = err;
+    }
+    __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(s_err) == 0) {
+      __threadfence();  // acquire: chunk k - 1's state and signature stores
+      TileArgs B = A;
+      B.cycle_base = A.cycle_base + k * Q.chunk;
+      B.cycles = min(Q.chunk, A.cycles - k * Q.chunk);
+      if (A.trace) B.trace = A.trace + k * Q.chunk * A.n_obs * A.nwords;
+      tile_body<W, kBA>(B, tile, lds);
+      __threadfence();  // release: this thread's stores before the chunk count moves
+    }
+    last_tile = tile; last_k = k;
+    __syncthreads();  // LDS and s_err are reused by the next item
+  }
+}
+
 }  // namespace gate
 }  // namespace rrtl

@@ -23,6 +23,7 @@
 #include "engine.hpp"
 #include "gate_device.hpp"
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <unordered_set>
@@ -32,6 +33,11 @@ namespace rrtl {
 namespace {  // anon
 
 using namespace gate;
+
+// Clears the work-queue control words (a kernel, so a captured graph could replay it).
+__global__ void queue_reset_kernel(uint32_t* ctl, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0u;
+}
 
 // ------------------------------------------------------------------ compile
 struct GateNet {
@@ -434,6 +440,10 @@ class GateEngine final : public Engine {
     hipDeviceProp_t prop;
     RRTL_HIP_CHECK(hipGetDeviceProperties(&prop, opt.device));
     lds_max_ = prop.sharedMemPerBlock;
+    cus_ = uint32_t(prop.multiProcessorCount);
+    int wall_khz = 0;
+    RRTL_HIP_CHECK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, opt.device));
+    wall_khz_ = wall_khz > 0 ? uint64_t(wall_khz) : 100000u;
     RRTL_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     stage_row_ = N.n_nodes;
     rows_ = N.n_nodes + uint32_t(N.commits.size());
@@ -742,24 +752,59 @@ class GateEngine final : public Engine {
     return R;
   }
 
+  // Chunks per tile for the work-queue schedule: the K in 2..16 whose rounds
+  // ceil(tiles * K / resident) / K come closest to the ideal tiles / resident
+  // (1 when every tile is resident at once).
+  static uint32_t choose_chunks(uint32_t tiles, uint32_t resident) {
+    if (resident == 0 || tiles <= resident) return 1;
+    uint32_t best = 1;
+    double best_cost = std::ceil(double(tiles) / resident);
+    for (uint32_t k = 2; k <= 16; k++) {
+      const double cost = std::ceil(double(tiles) * k / resident) / k;
+      if (cost < best_cost - 1e-9) { best = k; best_cost = cost; }
+    }
+    return best;
+  }
+
   void run_impl(const RunSpec& spec, bool with_outputs) {
     Prepared R = prepare(spec, with_outputs);
     TileArgs A = R.A;
     const uint32_t W = R.W, tiles = R.tiles;
     const size_t lds = R.lds;
     const uint64_t chunk = opt_.ticks_per_launch ? opt_.ticks_per_launch : std::max<uint64_t>(spec.cycles, 1);
+    const uint32_t resident = resident_tiles(W, R.byte_addr, lds);
+    const uint32_t chunks = opt_.tile_chunks ? opt_.tile_chunks : choose_chunks(tiles, resident);
+    if (chunks > 1) { qctl_.alloc(size_t(tiles) + 2); }
     stats_ = RunStats{};
     RRTL_HIP_CHECK(hipStreamSynchronize(stream_));
     RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
+    last_chunks_ = 1;
     for (uint64_t done = 0; done < spec.cycles; done += chunk) {
       A.cycle_base = done;
       A.cycles = std::min<uint64_t>(chunk, spec.cycles - done);
       if (A.trace) A.trace = trace_.p + done * A.n_obs * nwords_;
-      launch_tile(W, R.byte_addr, tiles, lds, A);
+      if (chunks > 1 && A.cycles > 1) {
+        QueueArgs Q{};
+        Q.ctl = qctl_.p; Q.n_tiles = tiles;
+        Q.chunk = (A.cycles + chunks - 1) / chunks;
+        Q.n_chunks = uint32_t((A.cycles + Q.chunk - 1) / Q.chunk);
+        Q.wait_limit = wall_khz_ * 1000u * 120u;  // two minutes
+        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(256), 0, stream_, qctl_.p, tiles + 2);
+        const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(resident, 1), tiles * Q.n_chunks);
+        launch_queue(W, R.byte_addr, grid, lds, A, Q);
+        last_chunks_ = Q.n_chunks;
+      } else {
+        launch_tile(W, R.byte_addr, tiles, lds, A);
+      }
       stats_.launches++;
     }
     RRTL_HIP_CHECK(hipEventRecord(ev1_, stream_));
     RRTL_HIP_CHECK(hipEventSynchronize(ev1_));
+    if (chunks > 1) {
+      uint32_t err = 0;
+      RRTL_HIP_CHECK(hipMemcpy(&err, qctl_.p + 1, sizeof(err), hipMemcpyDeviceToHost));
+      if (err) throw Error("E_HIP_INTERNAL: tile work queue: a chunk waited past its limit for the previous chunk");
+    }
     float ms = 0;
     RRTL_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
     stats_.kernel_ms = ms;
@@ -796,6 +841,46 @@ class GateEngine final : public Engine {
                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL((tile_kernel<W, kBA>), dim3(tiles), dim3(TILE_THREADS), lds, stream_, A);
   }
+  template <int W, bool kBA>
+  const void* queue_fn() const { return reinterpret_cast<const void*>(&tile_queue_kernel<W, kBA>); }
+  const void* queue_kernel(uint32_t W, bool byte_addr) const {
+    switch (W * 2 + (byte_addr ? 1 : 0)) {
+      case 2: return queue_fn<1, false>();
+      case 3: return queue_fn<1, true>();
+      case 4: return queue_fn<2, false>();
+      case 5: return queue_fn<2, true>();
+      case 8: return queue_fn<4, false>();
+      case 9: return queue_fn<4, true>();
+      case 16: return queue_fn<8, false>();
+      default: return queue_fn<8, true>();
+    }
+  }
+  // Workgroups of the queue kernel the device holds at once.
+  uint32_t resident_tiles(uint32_t W, bool byte_addr, size_t lds) {
+    const void* fn = queue_kernel(W, byte_addr);
+    if (lds > 65536) RRTL_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    int per_cu = 0;
+    RRTL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE_THREADS, lds));
+    return uint32_t(std::max(per_cu, 0)) * cus_;
+  }
+  template <int W, bool kBA>
+  void launch_q(uint32_t grid, size_t lds, const TileArgs& A, const QueueArgs& Q) {
+    hipLaunchKernelGGL((tile_queue_kernel<W, kBA>), dim3(grid), dim3(TILE_THREADS), lds, stream_, A, Q);
+  }
+  void launch_queue(uint32_t W, bool byte_addr, uint32_t grid, size_t lds, const TileArgs& A, const QueueArgs& Q) {
+    switch (W * 2 + (byte_addr ? 1 : 0)) {
+      case 2: launch_q<1, false>(grid, lds, A, Q); break;
+      case 3: launch_q<1, true>(grid, lds, A, Q); break;
+      case 4: launch_q<2, false>(grid, lds, A, Q); break;
+      case 5: launch_q<2, true>(grid, lds, A, Q); break;
+      case 8: launch_q<4, false>(grid, lds, A, Q); break;
+      case 9: launch_q<4, true>(grid, lds, A, Q); break;
+      case 16: launch_q<8, false>(grid, lds, A, Q); break;
+      default: launch_q<8, true>(grid, lds, A, Q); break;
+    }
+    RRTL_HIP_CHECK(hipGetLastError());
+  }
+
   void launch_tile(uint32_t W, bool byte_addr, uint32_t tiles, size_t lds, const TileArgs& A) {
     switch (W * 2 + (byte_addr ? 1 : 0)) {
       case 2: launch_w<1, false>(tiles, lds, A); break;
@@ -821,7 +906,9 @@ class GateEngine final : public Engine {
   std::vector<Force> forces_;
   bool forces_dirty_ = true, wires_stale_ = false, has_digest_ = false;
   size_t trace_bytes_ = 0;
-  uint32_t W_ = 0, last_slots_ = 0, last_levels_ = 0, last_gates_ = 0;
+  uint32_t W_ = 0, last_slots_ = 0, last_levels_ = 0, last_gates_ = 0, last_chunks_ = 1, cus_ = 1;
+  uint64_t wall_khz_ = 100000;
+  DevBuf<uint32_t> qctl_;  // work-queue counter, error flag, per-tile chunk counts
   Prepared plan_;
   std::string plan_key_;
   bool have_plan_ = false, batch_digest_ = false;

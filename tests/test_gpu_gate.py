@@ -182,7 +182,7 @@ def test_gate_engine_api_matches_oracle():
 def test_full_size_netlist_sampled_words():
     """The 20k-gate workload at full shape (65,536 lanes) for 300 cycles; four lane
     words checked against the oracle, every lane against a second GPU run with a
-    different tile shape."""
+    different tile shape and one pass per tile (no work queue)."""
     flush_print("case: 20k-gate netlist 65536 lanes x 300 cycles")
     p = small_netlist_program()
     lanes, cycles, seed = 65536, 300, 1
@@ -190,10 +190,39 @@ def test_full_size_netlist_sampled_words():
     sim = HipSimulator(p, lanes, engine="gate")
     sim.run(cycles, seed=seed, inputs=ins, observe=obs)
     d1 = stim.u32_rows_to_u64(sim.digest())
-    sim2 = HipSimulator(p, lanes, engine="gate", tile_words=1)
+    sim2 = HipSimulator(p, lanes, engine="gate", tile_words=1, tile_chunks=1)
     sim2.run(cycles, seed=seed, inputs=ins, observe=obs)
     assert np.array_equal(d1, stim.u32_rows_to_u64(sim2.digest()))
     for w64 in (0, 313, 777, 1023):
         ref = oracle.BitParallelRef(p, 64)
         dg, _ = ref.run(seed=seed, cycles=cycles, inputs=ins, observe=obs, word_base=w64, digest=True)
         assert np.array_equal(d1[:, w64], dg[:, 0]), w64
+
+
+@pytest.mark.parametrize("chunks,ticks", [(5, 0), (3, 16), (40, 0)])
+def test_work_queue_chunks_match_single_pass(chunks, ticks):
+    """The work-queue schedule (each tile's cycles cut into chunks that continue
+    from HBM) gives the same trace, signature and state as one pass per tile and
+    as the oracle, with a stuck-at force clamped on the way."""
+    flush_print(f"case: work queue chunks={chunks} ticks_per_launch={ticks}")
+    p = small_netlist_program(gates=3000, levels=24, dffs=300, inputs=48, outputs=32, seed=5)
+    lanes, cycles, seed = 1000, 40, 23
+    ins, obs = netlist_inputs(p), netlist_outputs(p)
+    forced = [i for i, s in enumerate(p.signals) if s.kind == KIND_REG][3]
+    runs = []
+    for ch in (1, chunks):
+        sim = HipSimulator(p, lanes, engine="gate", tile_chunks=ch, ticks_per_launch=ticks)
+        sim.set_force(forced, 77, 1)
+        sim.run(cycles, seed=seed, inputs=ins, observe=obs, trace=True)
+        runs.append(sim)
+    a, b = runs
+    assert np.array_equal(a.trace(), b.trace())
+    assert np.array_equal(a.digest(), b.digest())
+    va, _ = a.snapshot()
+    vb, _ = b.snapshot()
+    assert np.array_equal(va, vb)
+    ref = oracle.BitParallelRef(p, lanes)
+    ref.set_force(forced, 77, 1)
+    dg, tr = ref.run(seed=seed, cycles=cycles, inputs=ins, observe=obs, digest=True, trace=True)
+    assert np.array_equal(stim.u32_rows_to_u64(b.trace().reshape(cycles, len(obs), -1)), tr)
+    assert np.array_equal(stim.u32_rows_to_u64(b.digest()), dg)
