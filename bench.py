@@ -45,6 +45,7 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=65536)
     ap.add_argument("--cycles", type=int, default=10000)
     ap.add_argument("--tile-words", type=int, default=0)
+    ap.add_argument("--stage-gates", type=int, default=0, help="gate positions per thread per stage (0: automatic)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -64,7 +65,7 @@ def main() -> int:
     p = build_program()
     ins = [(i, IN_RANDOM, 0) for i, s in enumerate(p.signals) if s.kind == KIND_INPUT]
     obs = [i for i, s in enumerate(p.signals) if s.kind == KIND_OUTPUT]
-    sim = HipSimulator(p, args.lanes, engine="gate", tile_words=args.tile_words)
+    sim = HipSimulator(p, args.lanes, engine="gate", tile_words=args.tile_words, stage_gates=args.stage_gates)
 
     def step():
         sim.run(args.cycles, seed=args.seed, inputs=ins, observe=obs)
@@ -120,7 +121,7 @@ def main() -> int:
         cyc = max(probe_cycles, min(args.cycles, int(probe_cycles * args.cpu_seconds / max(secs, 1e-6))))
         base = oracle.BitParallelFast(p, args.lanes, threads=threads)
         d_cpu, _, secs = base.run(seed=args.seed, cycles=cyc, inputs=ins, observe=obs, digest=True)
-        fresh = HipSimulator(p, args.lanes, engine="gate", tile_words=args.tile_words)
+        fresh = HipSimulator(p, args.lanes, engine="gate", tile_words=args.tile_words, stage_gates=args.stage_gates)
         fresh.run(cyc, seed=args.seed, inputs=ins, observe=obs)
         same = bool(np.array_equal(stim_u64(fresh.digest()), d_cpu))
         cpu = {"value": args.lanes * cyc / secs, "unit": "lane-cycles/s", "cores": threads, "kind": "port",

@@ -57,6 +57,10 @@ typedef struct rrtl_hip_options {
    * the last wave of tiles does not run alone.  0 = automatic, 1 = off.  Read
    * only when struct_size covers it. */
   uint32_t tile_chunks;
+  /* Gate engine: gate positions per thread between two barriers (2 or 3); a
+   * position may take the previous position's result through registers.
+   * 0 = automatic.  Read only when struct_size covers it. */
+  uint32_t stage_gates;
 } rrtl_hip_options;
 
 /* SimdCpuSimulator::new (lib.rs:5529): lanes == 0 is E_SIM_IR_LANES; the

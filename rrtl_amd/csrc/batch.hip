@@ -58,16 +58,11 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs B) {
     const uint2 wi = B.items[it];
     const JobDev& J = B.jobs[wi.x];
     if (J.kind == ENGINE_GATE) {
-      switch (J.variant) {
-        case 1: gate::tile_body<1, false>(J.u.g, wi.y, lds); break;
-        case 2: gate::tile_body<2, false>(J.u.g, wi.y, lds); break;
-        case 4: gate::tile_body<4, false>(J.u.g, wi.y, lds); break;
-        case 8: gate::tile_body<8, false>(J.u.g, wi.y, lds); break;
-        case 1 | gate::GATE_VARIANT_BYTE_ADDR: gate::tile_body<1, true>(J.u.g, wi.y, lds); break;
-        case 2 | gate::GATE_VARIANT_BYTE_ADDR: gate::tile_body<2, true>(J.u.g, wi.y, lds); break;
-        case 4 | gate::GATE_VARIANT_BYTE_ADDR: gate::tile_body<4, true>(J.u.g, wi.y, lds); break;
-        default: gate::tile_body<8, true>(J.u.g, wi.y, lds); break;
-      }
+      const uint32_t v = J.variant;
+      gate::with_tile_variant(v & 0xFFu, (v & gate::GATE_VARIANT_BYTE_ADDR) != 0, v >> gate::GATE_VARIANT_M_SHIFT,
+                              [&](auto w, auto ba, auto m) {
+                                gate::tile_body<decltype(w)::value, decltype(ba)::value, decltype(m)::value>(J.u.g, wi.y, lds);
+                              });
     } else if (J.variant) {
       word::word_body<true>(J.u.w, wi.y, lds);
     } else {

@@ -91,11 +91,15 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
       opt.tile_words = options->tile_words; opt.ticks_per_launch = options->ticks_per_launch;
       if (options->struct_size >= offsetof(rrtl_hip_options, tile_chunks) + sizeof(uint32_t))
         opt.tile_chunks = options->tile_chunks;
+      if (options->struct_size >= offsetof(rrtl_hip_options, stage_gates) + sizeof(uint32_t))
+        opt.stage_gates = options->stage_gates;
     }
     if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 1024))
       throw rrtl::Error("E_HIP_OPTIONS: block_threads must be a multiple of 64 up to 1024");
     if (opt.tile_words && opt.tile_words != 1 && opt.tile_words != 2 && opt.tile_words != 4 && opt.tile_words != 8)
       throw rrtl::Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
+    if (opt.stage_gates && opt.stage_gates != 2 && opt.stage_gates != 3)
+      throw rrtl::Error("E_HIP_OPTIONS: stage_gates must be 2 or 3");
     rrtl::Program prog = rrtl::Program::parse(blob, bytes);
     if (kind == RRTL_HIP_AUTO) kind = prog.is_gate_level() ? RRTL_HIP_GATE : RRTL_HIP_WORD;
     auto* h = new rrtl_hip_engine;

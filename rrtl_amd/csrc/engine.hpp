@@ -18,6 +18,7 @@ struct EngineOptions {
   uint32_t tile_words = 0;         // gate engine: u32 lane-words per workgroup tile (0: autotune)
   uint32_t ticks_per_launch = 0;   // 0: whole run in one launch
   uint32_t tile_chunks = 0;        // gate engine: work-queue chunks per tile (0: automatic, 1: off)
+  uint32_t stage_gates = 0;        // gate engine: gate positions per thread per stage (0: automatic, 2 or 3)
 };
 
 // One design's share of a multi-design batch launch (batch.hip): the kernel
@@ -25,7 +26,7 @@ struct EngineOptions {
 // it has, their LDS need and an estimate of each item's cost for ordering.
 struct BatchJob {
   uint32_t kind = 0;          // ENGINE_GATE or ENGINE_WORD
-  uint32_t variant = 0;       // gate: tile words W; word: 1 if scratch lives in LDS
+  uint32_t variant = 0;       // gate: W | byte-address flag | M << 12; word: 1 if scratch lives in LDS
   uint32_t n_items = 0;
   size_t lds_bytes = 0;
   double item_cost = 0;

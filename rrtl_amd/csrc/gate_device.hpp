@@ -3,6 +3,7 @@
 // engine's own launches and by the multi-design batch kernel.
 #pragma once
 #include "common.hpp"
+#include <type_traits>
 
 namespace rrtl {
 namespace gate {
@@ -16,7 +17,6 @@ constexpr uint32_t G_COPY = 0;    // b = ones
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t CLAMP_COMMIT = 0xFFFFFFFEu;
 constexpr uint32_t TILE_THREADS = 256;
-constexpr uint32_t MAX_LEVEL_GATES = TILE_THREADS;  // one gate per thread per level
 
 struct CGate { uint32_t op, a, b, dst; };
 struct CCommit { uint32_t reg, val, rsig, pol, rbit, has_reset; };
@@ -30,8 +30,8 @@ __host__ __device__ inline uint32_t gate_eval(uint32_t op, uint32_t a, uint32_t 
 // ------------------------------------------------------------------ kernels
 struct TileArgs {
   uint32_t* state; uint64_t nwords;             // HBM rows [node][nwords]
-  const uint2* desc; const uint32_t* level_start;
-  uint32_t n_cycle_levels, n_post_levels;
+  const uint2* desc;                            // [stage][position][thread], STAGE_AHEAD padding stages
+  uint32_t n_cycle_stages, n_post_stages;
   const uint32_t* load_slot; const uint32_t* load_node; uint32_t n_load;
   const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
   const uint4* commit; uint32_t n_commit;
@@ -73,12 +73,13 @@ __device__ __forceinline__ void st_words(uint32_t* lds, uint32_t slot, const Wor
   }
 }
 
-// Gate descriptor, one per thread and level (8 bytes): x = a | b << 16,
-// y = dst | op << 16.  a, b and dst are LDS byte offsets when the tile fits in
-// 64 KiB (kBA) and slot indices otherwise.  XOR-family ops carry no input
-// inversions (they cancel into the output inversion).  A thread with no gate in
-// a level reads the constant-0 slot and writes one of 64 sink slots (thread %
-// 64), so every thread runs the same straight-line code.
+// Gate descriptor, one per thread, stage and position (8 bytes): x = a | b << 16,
+// y = dst | op << 16 | chained << 20.  a, b and dst are LDS byte offsets when
+// the tile fits in 64 KiB (kBA) and slot indices otherwise.  XOR-family ops
+// carry no input inversions (they cancel into the output inversion).  A
+// position without a gate reads an operand some other lane of its half-wave
+// reads (or constant 0) and writes one of 64 sink slots, so every thread runs
+// the same straight-line code.
 template <int W, bool kBA>
 __device__ __forceinline__ uint32_t* lds_at(uint32_t* lds, uint32_t field) {
   return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + (kBA ? field : field * uint32_t(W * 4)));
@@ -107,24 +108,59 @@ __device__ __forceinline__ uint32_t op3_xor_or_and(uint32_t A, uint32_t B, uint3
   return __builtin_amdgcn_bitop3_b32(A, B, x, 0x68);
 }
 
-// One gate for every lane-word of the tile:
-//   r = (x ? A ^ B : A & B) ^ io,  A = a ^ ia, B = b ^ ib  (per-thread masks).
-template <int W, bool kBA>
-__device__ __forceinline__ void eval_gate(uint32_t* lds, uint2 d) {
-  const Words<W> a = ld_words_p<W>(lds_at<W, kBA>(lds, d.x & 0xFFFFu));
-  const Words<W> b = ld_words_p<W>(lds_at<W, kBA>(lds, d.x >> 16));
-  const uint32_t y = d.y;
-  const uint32_t ia = 0u - ((y >> 16) & 1u), ib = 0u - ((y >> 17) & 1u), io = 0u - ((y >> 18) & 1u),
-                 x = 0u - ((y >> 19) & 1u);
-  Words<W> r;
+// a chained operand: ch ? prev : v per bit, in one v_bitop3_b32 (S0, S1, S2) =
+// (v, prev, ch): on v = 0xF0, prev = 0xCC, ch = 0xAA it is (0xAA & 0xCC) |
+// (0x55 & 0xF0) = 0xD8.
+__device__ __forceinline__ uint32_t mux_chain(uint32_t v, uint32_t prev, uint32_t ch) {
+  return __builtin_amdgcn_bitop3_b32(v, prev, ch, 0xD8);
+}
+
+constexpr uint32_t DESC_CHAINED = 1u << 20;  // descriptor flag: operand a is the previous position's result
+
+template <int M> struct StageDesc { uint2 d[M]; };
+
+template <int M>
+__device__ __forceinline__ void load_stage(StageDesc<M>& D, const uint2* row) {
 #pragma unroll
-  for (int k = 0; k < W; k++) r.v[k] = op3_xor_or_and(a.v[k] ^ ia, b.v[k] ^ ib, x) ^ io;
-  uint32_t* o = lds_at<W, kBA>(lds, y & 0xFFFFu);
-  if constexpr (W == 1) { o[0] = r.v[0]; }
-  else if constexpr (W == 2) { *reinterpret_cast<uint2*>(o) = make_uint2(r.v[0], r.v[1]); }
-  else {
+  for (int k = 0; k < M; k++) D.d[k] = row[k * TILE_THREADS + threadIdx.x];
+}
+
+// One stage of this thread: M gate positions over every lane-word of the tile,
+//   r = (x ? A ^ B : A & B) ^ io,  A = a ^ ia, B = b ^ ib  (per-thread masks),
+// where a is the previous position's r when the descriptor says chained.  All
+// LDS reads of the stage are issued first (their operands come from earlier
+// stages), then the positions run in order, each storing its result.
+template <int W, bool kBA, int M>
+__device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D) {
+  Words<W> va[M], vb[M];
 #pragma unroll
-    for (int k = 0; k < W; k += 4) *reinterpret_cast<uint4*>(o + k) = make_uint4(r.v[k], r.v[k + 1], r.v[k + 2], r.v[k + 3]);
+  for (int k = 0; k < M; k++) {
+    va[k] = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[k].x & 0xFFFFu));
+    vb[k] = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[k].x >> 16));
+  }
+  Words<W> prev;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const uint32_t y = D.d[k].y;
+    const uint32_t ia = 0u - ((y >> 16) & 1u), ib = 0u - ((y >> 17) & 1u), io = 0u - ((y >> 18) & 1u),
+                   x = 0u - ((y >> 19) & 1u);
+    Words<W> r;
+    if (k == 0) {
+#pragma unroll
+      for (int w = 0; w < W; w++) r.v[w] = op3_xor_or_and(va[k].v[w] ^ ia, vb[k].v[w] ^ ib, x) ^ io;
+    } else {
+      const uint32_t ch = 0u - ((y >> 20) & 1u);
+#pragma unroll
+      for (int w = 0; w < W; w++) r.v[w] = op3_xor_or_and(mux_chain(va[k].v[w], prev.v[w], ch) ^ ia, vb[k].v[w] ^ ib, x) ^ io;
+    }
+    uint32_t* o = lds_at<W, kBA>(lds, y & 0xFFFFu);
+    if constexpr (W == 1) { o[0] = r.v[0]; }
+    else if constexpr (W == 2) { *reinterpret_cast<uint2*>(o) = make_uint2(r.v[0], r.v[1]); }
+    else {
+#pragma unroll
+      for (int q = 0; q < W; q += 4) *reinterpret_cast<uint4*>(o + q) = make_uint4(r.v[q], r.v[q + 1], r.v[q + 2], r.v[q + 3]);
+    }
+    prev = r;
   }
 }
 
@@ -173,9 +209,11 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32
   __syncthreads();
 }
 
-constexpr uint32_t PREFETCH = 4;       // levels of descriptor look-ahead
-constexpr uint32_t SINK_SLOTS = 64;     // per-tile slots written by threads with no gate
-constexpr uint32_t GATE_VARIANT_BYTE_ADDR = 0x100;  // batch job variant flag: descriptors hold byte offsets
+constexpr uint32_t STAGE_AHEAD = 2;     // stages of descriptor look-ahead
+constexpr uint32_t SINK_SLOTS = 64;     // per-tile slots written by lanes with no output to keep
+// batch job variant: tile words W | descriptors hold byte offsets | positions per stage M << 12
+constexpr uint32_t GATE_VARIANT_BYTE_ADDR = 0x100;
+constexpr uint32_t GATE_VARIANT_M_SHIFT = 12;
 
 template <int W>
 __device__ __forceinline__ void commit_regs(const TileArgs& A, uint32_t* lds, uint32_t fb, uint32_t fe) {
@@ -194,51 +232,47 @@ __device__ __forceinline__ void commit_regs(const TileArgs& A, uint32_t* lds, ui
   if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
 }
 
-// One level: this thread's gate (descriptor fetched PREFETCH levels earlier),
-// then the fetch for PREFETCH levels later into the same registers (the
-// descriptor table carries PREFETCH padding levels, so the fetch needs no
-// bound check), the level barrier and, when forces exist, their clamps.  No
-// other global access happens in this loop, so the wait before a gate leaves
-// the PREFETCH-1 younger fetches in flight.
-template <int W, bool kBA, bool kClamp>
-__device__ __forceinline__ void level_stage(const TileArgs& A, uint32_t* lds, uint32_t L, const uint2* ahead, uint2& d,
-                                            uint32_t fb, uint32_t fe) {
-  eval_gate<W, kBA>(lds, d);
-  d = ahead[threadIdx.x];
+// One stage: this thread's positions (descriptors fetched STAGE_AHEAD stages
+// earlier), the fetch for STAGE_AHEAD stages later into the same registers
+// (the table carries STAGE_AHEAD padding stages, so the fetch needs no bound
+// check), the stage barrier and, when forces exist, their clamps.  No other
+// global access happens in this loop, so the fetches stay in flight.
+template <int W, bool kBA, int M, bool kClamp>
+__device__ __forceinline__ void stage_step(const TileArgs& A, uint32_t* lds, uint32_t S, const uint2* ahead,
+                                           StageDesc<M>& D, uint32_t fb, uint32_t fe) {
+  eval_stage<W, kBA, M>(lds, D);
+  load_stage<M>(D, ahead);
   __syncthreads();
-  if (kClamp) { apply_clamps<W>(A, lds, L + 1, fb, fe); __syncthreads(); }
+  if (kClamp) { apply_clamps<W>(A, lds, S + 1, fb, fe); __syncthreads(); }
 }
 
-// Levels [L0, L1): unrolled by PREFETCH so each of the four descriptor
-// registers is loaded and consumed in a fixed stage (no register rotation).
-// `row` is the uniform address of a level's descriptor row, so a fetch is one
-// scalar-base load with the thread's fixed offset.
-template <int W, bool kBA, bool kClamp>
-__device__ __forceinline__ void run_levels(const TileArgs& A, uint32_t* lds, uint32_t L0, uint32_t L1, uint32_t fb,
+// Stages [S0, S1): unrolled by STAGE_AHEAD so each descriptor register set is
+// loaded and consumed in a fixed step (no register rotation).  `row` is the
+// uniform address of a stage's descriptor rows.
+template <int W, bool kBA, int M, bool kClamp>
+__device__ __forceinline__ void run_stages(const TileArgs& A, uint32_t* lds, uint32_t S0, uint32_t S1, uint32_t fb,
                                            uint32_t fe) {
-  static_assert(PREFETCH == 4, "run_levels is unrolled for four descriptor registers");
-  const uint32_t tid = threadIdx.x;
-  const uint2* row = A.desc + size_t(L0) * TILE_THREADS;
-  uint2 d0 = row[tid], d1 = row[TILE_THREADS + tid], d2 = row[2 * TILE_THREADS + tid], d3 = row[3 * TILE_THREADS + tid];
-  row += PREFETCH * TILE_THREADS;
-  uint32_t L = L0;
-  for (; L + 4 <= L1; L += 4, row += 4 * TILE_THREADS) {
-    level_stage<W, kBA, kClamp>(A, lds, L + 0, row, d0, fb, fe);
-    level_stage<W, kBA, kClamp>(A, lds, L + 1, row + TILE_THREADS, d1, fb, fe);
-    level_stage<W, kBA, kClamp>(A, lds, L + 2, row + 2 * TILE_THREADS, d2, fb, fe);
-    level_stage<W, kBA, kClamp>(A, lds, L + 3, row + 3 * TILE_THREADS, d3, fb, fe);
+  static_assert(STAGE_AHEAD == 2, "run_stages is unrolled for two descriptor register sets");
+  constexpr uint32_t stride = uint32_t(M) * TILE_THREADS;
+  const uint2* row = A.desc + size_t(S0) * stride;
+  StageDesc<M> D0, D1;
+  load_stage<M>(D0, row);
+  load_stage<M>(D1, row + stride);
+  row += 2 * stride;
+  uint32_t S = S0;
+  for (; S + 2 <= S1; S += 2, row += 2 * stride) {
+    stage_step<W, kBA, M, kClamp>(A, lds, S, row, D0, fb, fe);
+    stage_step<W, kBA, M, kClamp>(A, lds, S + 1, row + stride, D1, fb, fe);
   }
-  if (L < L1) level_stage<W, kBA, kClamp>(A, lds, L, row, d0, fb, fe);
-  if (L + 1 < L1) level_stage<W, kBA, kClamp>(A, lds, L + 1, row + TILE_THREADS, d1, fb, fe);
-  if (L + 2 < L1) level_stage<W, kBA, kClamp>(A, lds, L + 2, row + 2 * TILE_THREADS, d2, fb, fe);
+  if (S < S1) stage_step<W, kBA, M, kClamp>(A, lds, S, row, D0, fb, fe);
 }
 
-template <int W, bool kBA>
+template <int W, bool kBA, int M>
 __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
   uint32_t* sig = lds + size_t(A.n_slots) * W;          // 64 x W signature words
   const uint32_t tid = threadIdx.x;
   const uint64_t w0 = uint64_t(tile) * W;
-  const uint32_t nlev = A.n_cycle_levels + A.n_post_levels;
+  const uint32_t nst = A.n_cycle_stages + A.n_post_stages;
   const uint32_t fb = A.force_start ? A.force_start[tile] : 0, fe = A.force_start ? A.force_start[tile + 1] : 0;
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
     uint32_t s = i / W, w = i % W;
@@ -277,13 +311,13 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
     }
     __syncthreads();
     // cycle program, observe before the edge, commit, post program
-    if (fe > fb) run_levels<W, kBA, true>(A, lds, 0, A.n_cycle_levels, fb, fe);
-    else run_levels<W, kBA, false>(A, lds, 0, A.n_cycle_levels, fb, fe);
+    if (fe > fb) run_stages<W, kBA, M, true>(A, lds, 0, A.n_cycle_stages, fb, fe);
+    else run_stages<W, kBA, M, false>(A, lds, 0, A.n_cycle_stages, fb, fe);
     if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
     commit_regs<W>(A, lds, fb, fe);
-    if (A.n_post_levels) {
-      if (fe > fb) run_levels<W, kBA, true>(A, lds, A.n_cycle_levels, nlev, fb, fe);
-      else run_levels<W, kBA, false>(A, lds, A.n_cycle_levels, nlev, fb, fe);
+    if (A.n_post_stages) {
+      if (fe > fb) run_stages<W, kBA, M, true>(A, lds, A.n_cycle_stages, nst, fb, fe);
+      else run_stages<W, kBA, M, false>(A, lds, A.n_cycle_stages, nst, fb, fe);
     }
     if (A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
   }
@@ -300,10 +334,30 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
     }
 }
 
-template <int W, bool kBA>
+template <int W, bool kBA, int M>
 __global__ void __launch_bounds__(TILE_THREADS) tile_kernel(TileArgs A) {
   extern __shared__ __align__(16) uint32_t lds[];
-  tile_body<W, kBA>(A, blockIdx.x, lds);
+  tile_body<W, kBA, M>(A, blockIdx.x, lds);
+}
+
+// Calls f(W, byte_addr, M) with each as a std::integral_constant, for the
+// compiled variants: W in {1, 2, 4, 8}, M in {2, 3}.
+template <class F>
+__host__ __device__ inline void with_tile_variant(uint32_t W, bool byte_addr, uint32_t M, F&& f) {
+  auto m_ = [&](auto w, auto ba) {
+    if (M == 3) f(w, ba, std::integral_constant<int, 3>{});
+    else f(w, ba, std::integral_constant<int, 2>{});
+  };
+  auto b_ = [&](auto w) {
+    if (byte_addr) m_(w, std::true_type{});
+    else m_(w, std::false_type{});
+  };
+  switch (W) {
+    case 1: b_(std::integral_constant<int, 1>{}); break;
+    case 2: b_(std::integral_constant<int, 2>{}); break;
+    case 4: b_(std::integral_constant<int, 4>{}); break;
+    default: b_(std::integral_constant<int, 8>{}); break;
+  }
 }
 
 // Work-queue schedule for runs with more tiles than the device holds at once.
@@ -327,7 +381,7 @@ struct QueueArgs {
 // thread-0 region on each side of the back edge and an LDS-read exit test the
 // compiler may build an inner exec-mask loop that runs the other lanes again on
 // the old item while thread 0 waits at its exit.)
-template <int W, bool kBA>
+template <int W, bool kBA, int M>
 __global__ void __launch_bounds__(TILE_THREADS) tile_queue_kernel(TileArgs A, QueueArgs Q) {
   extern __shared__ __align__(16) uint32_t lds[];
   __shared__ uint32_t s_item, s_err;
@@ -367,7 +421,7 @@ __global__ void __launch_bounds__(TILE_THREADS) tile_queue_kernel(TileArgs A, Qu
       B.cycle_base = A.cycle_base + k * Q.chunk;
       B.cycles = min(Q.chunk, A.cycles - k * Q.chunk);
       if (A.trace) B.trace = A.trace + k * Q.chunk * A.n_obs * A.nwords;
-      tile_body<W, kBA>(B, tile, lds);
+      tile_body<W, kBA, M>(B, tile, lds);
       __threadfence();  // release: this thread's stores before the chunk count moves
     }
     last_tile = tile; last_k = k;

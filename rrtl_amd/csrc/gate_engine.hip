@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <queue>
 #include <unordered_set>
 #include <unordered_map>
 
@@ -152,107 +153,218 @@ GateNet build_net(const Program& P) {
   return N;
 }
 
-// One compiled tile program: levels of gates over LDS slots.
+// One compiled tile program: stages of up to M gates per thread over LDS
+// slots, one barrier per stage.
 struct TileProgram {
-  std::vector<uint32_t> desc;          // 2 words per gate: a | b<<16, dst | op<<16
-  std::vector<uint32_t> level_start;   // gate ranges per level (cycle levels, then post levels)
-  uint32_t n_cycle_levels = 0, n_post_levels = 0;
+  uint32_t M = 1;                        // gate positions per thread per stage
+  uint32_t n_cycle_stages = 0, n_post_stages = 0;
+  // [stage][depth][thread] x 2 words: a | b << 16, dst | op << 16 | chained << 20
+  // (slot indices; sink slots are sink + k)
+  std::vector<uint32_t> desc;
+  uint32_t n_gates = 0;                  // gates evaluated per cycle (cycle and post programs)
   std::vector<uint32_t> load_slot, load_node;     // persistent slots loaded at start, stored at end
   std::vector<uint32_t> stim_slot, stim_mode, stim_stream;
   std::vector<uint64_t> stim_arg;
-  std::vector<uint32_t> commit;        // 4 words: reg_slot, val_slot, rsig_slot|NONE, flags(pol | rbit<<1 | has<<2)
+  std::vector<uint32_t> commit;          // 4 words: reg_slot, val_slot, rsig_slot|NONE, flags(pol | rbit<<1 | has<<2)
   std::vector<uint32_t> obs_slot;
-  uint32_t n_slots = 0, c0 = 0, c1 = 0;
+  uint32_t n_slots = 0, sink = 0, c0 = 0, c1 = 0;  // live slots [0, n_slots), sink slots [sink, sink + 64)
   // per node: persistent slot, or NONE
   std::unordered_map<uint32_t, uint32_t> persist;
-  // forced-node clamp positions: node -> (timeline level, slot) for each instance
+  // forced-node clamp positions: node -> (stage position, slot) for each instance
   std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> clamp_sites;
 };
 
-struct Timeline {
-  // positions: cycle gate levels 1..L, PRE observe L+1, commit L+2, post levels L+3.., POST observe after
-  uint32_t L = 0, L2 = 0;
+constexpr uint32_t DESC_CHAINED = 1u << 20;  // operand a is this thread's previous gate result
+
+// Stage schedule of a gate subset.  A stage gives every thread up to M gate
+// positions, evaluated in order between two barriers.  Position 0 and any
+// position a thread cannot chain take a ready gate (every input from an
+// earlier stage), most urgent first (smallest as-late-as-possible level);
+// position k > 0 first tries to chain: a gate whose only same-stage input is
+// the thread's gate at position k - 1, read through exactly one operand and
+// passed in registers.  deps[i] are ordering edges (read-after-write through
+// prod_a / prod_b, plus write-after-read/write on rewritten nodes).
+struct StageSchedule {
+  uint32_t stages = 0;
+  std::vector<uint32_t> stage, depth, thread;  // per gate of the subset
+  std::vector<uint8_t> chained, chain_b;       // chained through operand a (or b)
 };
 
-TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec, const std::vector<uint8_t>& forced) {
+StageSchedule stage_schedule(size_t n, const std::vector<std::vector<uint32_t>>& deps, const std::vector<uint32_t>& prod_a,
+                             const std::vector<uint32_t>& prod_b, const std::vector<uint8_t>& in_set,
+                             const std::vector<uint32_t>& asap, const std::vector<uint8_t>& no_chain_from, uint32_t M) {
+  StageSchedule S;
+  S.stage.assign(n, 0); S.depth.assign(n, 0); S.thread.assign(n, 0); S.chained.assign(n, 0); S.chain_b.assign(n, 0);
+  std::vector<uint32_t> alap(n, 0), pending(n, 0);
+  std::vector<std::vector<uint32_t>> users(n);
+  uint32_t span = 0;
+  for (size_t i = 0; i < n; i++) if (in_set[i]) span = std::max(span, asap[i]);
+  for (size_t i = 0; i < n; i++) {
+    if (!in_set[i]) continue;
+    std::vector<uint32_t> ds(deps[i]);
+    std::sort(ds.begin(), ds.end());
+    ds.erase(std::unique(ds.begin(), ds.end()), ds.end());
+    for (uint32_t j : ds) if (in_set[j]) { users[j].push_back(uint32_t(i)); pending[i]++; }
+  }
+  for (size_t k = n; k-- > 0;) {
+    if (!in_set[k]) continue;
+    uint32_t a = span;
+    for (uint32_t u : users[k]) a = std::min(a, alap[u] - 1);
+    alap[k] = a;
+  }
+  using Key = std::pair<uint32_t, uint32_t>;  // (alap, index), smallest first
+  std::priority_queue<Key, std::vector<Key>, std::greater<Key>> ready;
+  for (size_t i = 0; i < n; i++) if (in_set[i] && pending[i] == 0) ready.push({alap[i], uint32_t(i)});
+  std::vector<uint8_t> done(n, 0);
+  std::vector<uint32_t> grid(size_t(TILE_THREADS) * M), placed;
+  for (uint32_t s = 1; !ready.empty(); s++) {
+    std::fill(grid.begin(), grid.end(), NONE);
+    placed.clear();
+    auto put = [&](uint32_t g, uint32_t t, uint32_t k) {
+      grid[size_t(t) * M + k] = g; done[g] = 1; placed.push_back(g);
+      S.stage[g] = s; S.depth[g] = k; S.thread[g] = t;
+    };
+    for (uint32_t t = 0; t < TILE_THREADS && !ready.empty(); t++) { put(ready.top().second, t, 0); ready.pop(); }
+    for (uint32_t k = 1; k < M; k++) {
+      for (uint32_t t = 0; t < TILE_THREADS; t++) {
+        const uint32_t p = grid[size_t(t) * M + k - 1];
+        if (p != NONE && !no_chain_from[p]) {
+          uint32_t best = NONE;
+          for (uint32_t u : users[p]) {
+            if (done[u]) continue;
+            const bool ra = prod_a[u] == p, rb = prod_b[u] == p;
+            if (ra == rb) continue;  // p reaches u through both operands or through neither
+            bool ok = true;
+            for (uint32_t j : deps[u])
+              if (j != p && in_set[j] && (!done[j] || S.stage[j] >= s)) { ok = false; break; }
+            if (ok && (best == NONE || alap[u] < alap[best])) best = u;
+          }
+          if (best != NONE) {
+            put(best, t, k);
+            S.chained[best] = 1; S.chain_b[best] = prod_b[best] == p;
+            continue;
+          }
+        }
+        if (!ready.empty()) { put(ready.top().second, t, k); ready.pop(); }
+      }
+    }
+    for (uint32_t g : placed)
+      for (uint32_t u : users[g])
+        if (--pending[u] == 0 && !done[u]) ready.push({alap[u], u});
+    S.stages = s;
+  }
+  return S;
+}
+
+TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec, const std::vector<uint8_t>& forced,
+                         uint32_t M) {
   TileProgram T;
+  T.M = M;
   const uint32_t nsig = N.nsig;
-  // writer gate per node (within settle + next), levels
   std::vector<const CGate*> gl;
   for (auto& g : N.settle) gl.push_back(&g);
   for (auto& g : N.next) gl.push_back(&g);
+  const size_t ng = gl.size();
+  // Ordering edges in list order: a read depends on the node's writer at that
+  // point (prod_a / prod_b); a write (an async-reset rewrite of a register, or
+  // any node written twice) also orders after the earlier readers and the
+  // earlier writer of that node.
   std::vector<int64_t> writer(N.n_nodes, -1);
-  std::vector<uint32_t> lvl(gl.size(), 0);
-  for (size_t i = 0; i < gl.size(); i++) {
+  std::vector<uint32_t> lvl(ng, 0), prod_a(ng, NONE), prod_b(ng, NONE);
+  std::vector<std::vector<uint32_t>> deps(ng);
+  std::vector<std::vector<uint32_t>> readers(N.n_nodes);
+  for (size_t i = 0; i < ng; i++) {
+    if (writer[gl[i]->a] >= 0) prod_a[i] = uint32_t(writer[gl[i]->a]);
+    if (writer[gl[i]->b] >= 0) prod_b[i] = uint32_t(writer[gl[i]->b]);
+    for (uint32_t s : {gl[i]->a, gl[i]->b}) {
+      if (writer[s] >= 0) deps[i].push_back(uint32_t(writer[s]));
+      readers[s].push_back(uint32_t(i));
+    }
+    const uint32_t d = gl[i]->dst;
+    for (uint32_t r : readers[d]) if (r != i) deps[i].push_back(r);
+    readers[d].clear();
+    if (writer[d] >= 0) deps[i].push_back(uint32_t(writer[d]));
     uint32_t l = 0;
-    for (uint32_t s : {gl[i]->a, gl[i]->b}) if (writer[s] >= 0) l = std::max(l, lvl[writer[s]]);
-    // a gate that rewrites a register (async reset) also orders after earlier readers of it
-    lvl[i] = l + 1;
-    writer[gl[i]->dst] = int64_t(i);
+    for (uint32_t j : deps[i]) l = std::max(l, lvl[j]);
+    lvl[i] = l + 1;  // as soon as possible
+    writer[d] = int64_t(i);
   }
-  // async-reset rewrites: every comb reader of that register must see the new
-  // value; readers after the rewrite already depend on it through `writer`.
-  std::vector<uint8_t> is_obs(N.n_nodes, 0);
-  for (uint32_t s : spec.obs_sig) is_obs[s] = 1;
-  // liveness roots
-  std::vector<uint8_t> need(gl.size(), 0);
+  // cycle program: gates reaching a commit, a PRE observation or a forced register
+  std::vector<uint8_t> need(ng, 0);
   std::vector<uint32_t> stack;
-  auto mark_node = [&](uint32_t n) { if (writer[n] >= 0 && !need[writer[n]]) { need[writer[n]] = 1; stack.push_back(uint32_t(writer[n])); } };
-  for (auto& c : N.commits) { mark_node(c.val); if (c.rsig != NONE) mark_node(c.rsig); }
-  if (!spec.observe_post) for (uint32_t s : spec.obs_sig) mark_node(s);
-  for (uint32_t s = 0; s < nsig; s++) if (forced[s] && P.signals[s].kind == K_REG) mark_node(s);
-  while (!stack.empty()) {
-    uint32_t i = stack.back(); stack.pop_back();
-    mark_node(gl[i]->a); mark_node(gl[i]->b);
-    // an async-reset rewrite of a register reads the register's prior writer too
-  }
-  // compact levels among needed gates
-  uint32_t L = 0;
-  for (size_t i = 0; i < gl.size(); i++) if (need[i]) L = std::max(L, lvl[i]);
-  // post program: cone of observed signals, re-evaluated after the commit
-  std::vector<uint8_t> need2(gl.size(), 0);
-  uint32_t L2 = 0;
+  auto mark_gate = [&](std::vector<uint8_t>& set, uint32_t g) { if (g != NONE && !set[g]) { set[g] = 1; stack.push_back(g); } };
+  auto mark_node = [&](std::vector<uint8_t>& set, uint32_t n) { if (writer[n] >= 0) mark_gate(set, uint32_t(writer[n])); };
+  for (auto& c : N.commits) { mark_node(need, c.val); if (c.rsig != NONE) mark_node(need, c.rsig); }
+  if (!spec.observe_post) for (uint32_t s : spec.obs_sig) mark_node(need, s);
+  for (uint32_t s = 0; s < nsig; s++) if (forced[s] && P.signals[s].kind == K_REG) mark_node(need, s);
+  while (!stack.empty()) { uint32_t i = stack.back(); stack.pop_back(); mark_gate(need, prod_a[i]); mark_gate(need, prod_b[i]); }
+  // post program: cone of the observed signals within the settle, re-evaluated after the commit
+  std::vector<uint8_t> need2(ng, 0);
   if (spec.observe_post) {
-    std::vector<uint32_t> st2;
-    auto mark2 = [&](uint32_t n) { if (writer[n] >= 0 && size_t(writer[n]) < N.settle.size() && !need2[writer[n]]) { need2[writer[n]] = 1; st2.push_back(uint32_t(writer[n])); } };
-    for (uint32_t s : spec.obs_sig) mark2(s);
-    while (!st2.empty()) { uint32_t i = st2.back(); st2.pop_back(); mark2(gl[i]->a); mark2(gl[i]->b); }
-    for (size_t i = 0; i < N.settle.size(); i++) if (need2[i]) L2 = std::max(L2, lvl[i]);
+    for (uint32_t s : spec.obs_sig) if (writer[s] >= 0 && size_t(writer[s]) < N.settle.size()) mark_gate(need2, uint32_t(writer[s]));
+    while (!stack.empty()) {
+      uint32_t i = stack.back(); stack.pop_back();
+      for (uint32_t p : {prod_a[i], prod_b[i]}) if (p != NONE && p < N.settle.size()) mark_gate(need2, p);
+    }
   }
   // persistent nodes: signals not written by the settle (inputs, registers,
   // host-driven wires) that are read, committed, observed or stimulated
   std::vector<uint8_t> pers(N.n_nodes, 0);
   auto want_persistent = [&](uint32_t n) { if (n < nsig && (!N.written[n] || P.signals[n].kind == K_REG)) pers[n] = 1; };
-  for (size_t i = 0; i < gl.size(); i++) if (need[i] || need2[i]) { want_persistent(gl[i]->a); want_persistent(gl[i]->b); want_persistent(gl[i]->dst); }
+  for (size_t i = 0; i < ng; i++) if (need[i] || need2[i]) { want_persistent(gl[i]->a); want_persistent(gl[i]->b); want_persistent(gl[i]->dst); }
   for (auto& c : N.commits) { pers[c.reg] = 1; want_persistent(c.val); if (c.rsig != NONE) want_persistent(c.rsig); }
   for (uint32_t s : spec.obs_sig) want_persistent(s);
   for (uint32_t s : spec.in_sig) pers[s] = 1;
-  // slot 0 / 1: constants
+  // no chaining out of a gate whose value must sit in LDS before its readers
+  // run (a clamped force) or that writes a persistent node
+  std::vector<uint8_t> no_chain(ng, 0);
+  for (size_t i = 0; i < ng; i++) {
+    const uint32_t d = gl[i]->dst;
+    if (pers[d] || (d < nsig && forced[d])) no_chain[i] = 1;
+  }
+  const StageSchedule CS = stage_schedule(ng, deps, prod_a, prod_b, need, lvl, no_chain, M);
+  const StageSchedule PS = stage_schedule(ng, deps, prod_a, prod_b, need2, lvl, no_chain, M);
+  const uint32_t L = CS.stages, L2 = spec.observe_post ? PS.stages : 0;
+  // slot 0 / 1: constants; then persistent nodes
   T.c0 = 0; T.c1 = 1;
   uint32_t next_slot = 2;
   for (uint32_t n = 0; n < nsig; n++) if (pers[n]) T.persist[n] = next_slot++;
   const uint32_t first_free = next_slot;
-  // linear-scan allocation of gate outputs over the timeline
+  // timeline positions: cycle stages 1..L, PRE observe L+1, commit L+2, post stages L+3..
   const uint32_t POS_OBS_PRE = L + 1, POS_COMMIT = L + 2, POS_POST0 = L + 2;
-  struct Inst { uint32_t gate; uint32_t pos; uint32_t slot; };
+  struct Inst { uint32_t gate, pos, slot, depth, thread; bool post, chained, chain_b; };
   std::vector<Inst> insts;
-  std::vector<uint32_t> inst_of_cycle(gl.size(), NONE), inst_of_post(gl.size(), NONE);
-  for (size_t i = 0; i < gl.size(); i++) if (need[i]) { inst_of_cycle[i] = uint32_t(insts.size()); insts.push_back({uint32_t(i), lvl[i], NONE}); }
-  for (size_t i = 0; i < gl.size(); i++) if (need2[i]) { inst_of_post[i] = uint32_t(insts.size()); insts.push_back({uint32_t(i), POS_POST0 + lvl[i], NONE}); }
-  // last use of each instance
+  std::vector<uint32_t> inst_of_cycle(ng, NONE), inst_of_post(ng, NONE);
+  for (size_t i = 0; i < ng; i++)
+    if (need[i]) {
+      inst_of_cycle[i] = uint32_t(insts.size());
+      insts.push_back({uint32_t(i), CS.stage[i], NONE, CS.depth[i], CS.thread[i], false, CS.chained[i] != 0, CS.chain_b[i] != 0});
+    }
+  for (size_t i = 0; i < ng; i++)
+    if (need2[i]) {
+      inst_of_post[i] = uint32_t(insts.size());
+      insts.push_back({uint32_t(i), POS_POST0 + PS.stage[i], NONE, PS.depth[i], PS.thread[i], true, PS.chained[i] != 0,
+                       PS.chain_b[i] != 0});
+    }
+  T.n_gates = uint32_t(insts.size());
+  auto inst_of = [&](uint32_t gate, bool post) { return post ? inst_of_post[gate] : inst_of_cycle[gate]; };
+  // last LDS read of each instance (chained reads come from registers)
   std::vector<uint32_t> last(insts.size(), 0);
-  auto src_inst = [&](uint32_t node, bool post) -> uint32_t {
-    if (writer[node] < 0) return NONE;
-    return post ? inst_of_post[writer[node]] : inst_of_cycle[writer[node]];
-  };
   for (auto& in : insts) {
-    bool post = in.pos > POS_POST0;
-    for (uint32_t s : {gl[in.gate]->a, gl[in.gate]->b}) {
-      if (pers.size() > s && pers[s]) continue;
-      uint32_t si = src_inst(s, post);
+    for (int op = 0; op < 2; op++) {
+      const uint32_t p = op == 0 ? prod_a[in.gate] : prod_b[in.gate];
+      if (p == NONE) continue;
+      const uint32_t node = op == 0 ? gl[in.gate]->a : gl[in.gate]->b;
+      if (pers[node]) continue;
+      if (in.chained && (op == 1) == in.chain_b) continue;
+      const uint32_t si = inst_of(p, in.post);
       if (si != NONE) last[si] = std::max(last[si], in.pos);
     }
   }
+  auto src_inst = [&](uint32_t node, bool post) -> uint32_t {
+    if (writer[node] < 0) return NONE;
+    return inst_of(uint32_t(writer[node]), post);
+  };
   for (auto& c : N.commits) {
     for (uint32_t s : {c.val, c.rsig}) {
       if (s == NONE || pers[s]) continue;
@@ -265,94 +377,221 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
     uint32_t si = src_inst(s, spec.observe_post);
     if (si != NONE) last[si] = std::max(last[si], spec.observe_post ? POS_POST0 + L2 + 1 : POS_OBS_PRE);
   }
-  // allocate: process instances by position; free slots whose last use < position
-  std::vector<uint32_t> order(insts.size());
-  for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return insts[x].pos < insts[y].pos; });
-  std::multimap<uint32_t, uint32_t> releases;  // last-use pos -> slot
-  std::vector<uint32_t> free_slots;
-  uint32_t max_slot = first_free;
-  for (uint32_t ii : order) {
-    Inst& in = insts[ii];
-    while (!releases.empty() && releases.begin()->first < in.pos) { free_slots.push_back(releases.begin()->second); releases.erase(releases.begin()); }
-    uint32_t d = gl[in.gate]->dst;
-    if (d < nsig && pers[d]) { in.slot = T.persist.at(d); continue; }
-    if (!free_slots.empty()) { in.slot = free_slots.back(); free_slots.pop_back(); }
-    else in.slot = max_slot++;
-    releases.emplace(std::max(last[ii], in.pos), in.slot);
-  }
-  T.n_slots = max_slot;
-  if (T.n_slots > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
+  // operand slots of an instance (producer instance at that point of the list)
+  auto operand_slot = [&](const Inst& in, int op) -> uint32_t {
+    const uint32_t node = op == 0 ? gl[in.gate]->a : gl[in.gate]->b;
+    if (node == N.c0) return T.c0;
+    if (node == N.c1) return T.c1;
+    if (pers[node]) return T.persist.at(node);
+    const uint32_t p = op == 0 ? prod_a[in.gate] : prod_b[in.gate];
+    const uint32_t si = p == NONE ? NONE : inst_of(p, in.post);
+    if (si == NONE) throw Error("E_HIP_INTERNAL: node has no producer in the tile program");
+    if (insts[si].slot == NONE) throw Error("E_HIP_INTERNAL: operand read before its producer's stage");
+    return insts[si].slot;
+  };
   auto slot_of = [&](uint32_t node, bool post) -> uint32_t {
     if (node == N.c0) return T.c0;
     if (node == N.c1) return T.c1;
     if (node < nsig && pers[node]) return T.persist.at(node);
-    uint32_t si = src_inst(node, post);
-    if (si == NONE) throw Error("E_HIP_INTERNAL: node has no producer in the tile program");
+    const uint32_t si = src_inst(node, post);
+    if (si == NONE || insts[si].slot == NONE) throw Error("E_HIP_INTERNAL: node has no producer in the tile program");
     return insts[si].slot;
   };
-  // emit levels: cycle levels 1..L (plus one staging level when needed), then post levels
-  std::vector<std::vector<uint32_t>> lv_desc(L + 1), lv2_desc(L2 + 1);
-  std::vector<std::vector<uint32_t>> by_level(L + 1), by_level2(L2 + 1);
-  for (uint32_t ii = 0; ii < insts.size(); ii++) {
-    if (insts[ii].pos <= L) by_level[insts[ii].pos].push_back(ii);
-    else by_level2[insts[ii].pos - POS_POST0].push_back(ii);
-  }
-  std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> forced_cycle, forced_post;  // (signal, (level, slot))
-  auto emit = [&](const std::vector<uint32_t>& v, bool post, std::vector<uint32_t>& out, uint32_t level) {
-    for (uint32_t ii : v) {
-      const CGate& g = *gl[insts[ii].gate];
-      uint32_t a = slot_of(g.a, post), b = slot_of(g.b, post), d = insts[ii].slot;
-      out.push_back(a | (b << 16));
-      out.push_back(d | (g.op << 16));
-      if (g.dst < nsig && forced[g.dst]) (post ? forced_post : forced_cycle).push_back({g.dst, {level, d}});
+  // Allocation, stage by stage, with LDS banks in mind.  A wavefront's LDS
+  // access is served per half-wave (32 lanes); the lanes of a half conflict
+  // when distinct slots share (slot mod 32) (at W = 2 a slot is an 8-byte pair
+  // of the 64 banks; every W keeps the residue -> bank-group map), and the
+  // access takes as many passes as the most shared residue.  So per stage:
+  //  * each thread's gates move together to one of the 8 half-waves, chosen
+  //    where their a and b residues (per position) are least used so far
+  //    (distinct slots only: equal slots are a broadcast);
+  //  * per position, the outputs of a half take distinct residues -- residues
+  //    with free slots first (fullest first), else a fresh slot of the residue
+  //    whose fresh run is shortest; the lanes without an output write sink
+  //    slots of the residues left over, so every write of a half hits 32 bank
+  //    groups.
+  // A value read only through the register chain gets no slot (it is written
+  // to a sink slot).  Slots are released after their last read (linear scan);
+  // a fresh slot is taken only when no slot of an unused residue is free.
+  constexpr uint32_t RES = 32, HALVES = TILE_THREADS / RES;
+  uint32_t max_pos = 0;
+  for (auto& in : insts) max_pos = std::max(max_pos, in.pos);
+  std::vector<std::vector<uint32_t>> at_pos(size_t(max_pos) + 1);
+  for (uint32_t ii = 0; ii < insts.size(); ii++) at_pos[insts[ii].pos].push_back(ii);
+  std::vector<std::vector<uint32_t>> free_res(RES);
+  std::vector<uint32_t> fresh(RES);
+  for (uint32_t r = 0; r < RES; r++) fresh[r] = first_free + ((r + RES - first_free % RES) % RES);
+  uint32_t slot_end = first_free;
+  std::multimap<uint32_t, uint32_t> releases;  // last-read pos -> slot
+  std::vector<uint32_t> lane_of(insts.size(), NONE);
+  std::vector<uint32_t> write_mask;            // per (position, depth, half): residues written
+  std::vector<uint32_t> mask_base(size_t(max_pos) + 2, 0);
+  for (uint32_t pos = 1; pos <= max_pos + 1; pos++) mask_base[pos] = mask_base[pos - 1] + M * HALVES;
+  write_mask.assign(mask_base[max_pos + 1] + M * HALVES, 0);
+  std::vector<uint32_t> cnt_a(size_t(M) * HALVES * RES), cnt_b(size_t(M) * HALVES * RES);
+  std::vector<std::vector<uint32_t>> half_slots(size_t(M) * HALVES);
+  std::vector<std::vector<uint32_t>> by_thread(TILE_THREADS);
+  std::vector<uint32_t> half_fill(HALVES);
+  for (uint32_t pos = 1; pos <= max_pos; pos++) {
+    while (!releases.empty() && releases.begin()->first < pos) {
+      free_res[releases.begin()->second % RES].push_back(releases.begin()->second);
+      releases.erase(releases.begin());
     }
-  };
-  for (uint32_t l = 1; l <= L; l++) emit(by_level[l], false, lv_desc[l], l);
-  for (uint32_t l = 1; l <= L2; l++) emit(by_level2[l], true, lv2_desc[l], l);
+    const std::vector<uint32_t>& v = at_pos[pos];
+    if (v.empty()) continue;
+    for (auto& b : by_thread) b.clear();
+    for (uint32_t ii : v) by_thread[insts[ii].thread].push_back(ii);
+    uint32_t nthreads = 0;
+    for (auto& b : by_thread) if (!b.empty()) nthreads++;
+    const uint32_t cap = (nthreads + HALVES - 1) / HALVES;
+    std::fill(cnt_a.begin(), cnt_a.end(), 0u); std::fill(cnt_b.begin(), cnt_b.end(), 0u);
+    for (auto& hs : half_slots) hs.clear();
+    std::fill(half_fill.begin(), half_fill.end(), 0u);
+    for (uint32_t t = 0; t < TILE_THREADS; t++) {
+      if (by_thread[t].empty()) continue;
+      // reads of this thread per position
+      uint32_t best_h = 0, best_key = ~0u;
+      for (uint32_t h = 0; h < HALVES; h++) {
+        if (half_fill[h] >= cap) continue;
+        uint32_t key = half_fill[h];
+        for (uint32_t ii : by_thread[t]) {
+          const Inst& in = insts[ii];
+          const auto& hs = half_slots[in.depth * HALVES + h];
+          for (int op = 0; op < 2; op++) {
+            if (in.chained && (op == 1) == in.chain_b) continue;
+            const uint32_t s = operand_slot(in, op);
+            if (std::find(hs.begin(), hs.end(), s) != hs.end()) continue;
+            const uint32_t c = (op == 0 ? cnt_a : cnt_b)[(in.depth * HALVES + h) * RES + s % RES];
+            key += c * 1024;
+          }
+        }
+        if (key < best_key) { best_key = key; best_h = h; }
+      }
+      for (uint32_t ii : by_thread[t]) {
+        const Inst& in = insts[ii];
+        auto& hs = half_slots[in.depth * HALVES + best_h];
+        for (int op = 0; op < 2; op++) {
+          if (in.chained && (op == 1) == in.chain_b) continue;
+          const uint32_t s = operand_slot(in, op);
+          if (std::find(hs.begin(), hs.end(), s) != hs.end()) continue;
+          (op == 0 ? cnt_a : cnt_b)[(in.depth * HALVES + best_h) * RES + s % RES]++;
+          hs.push_back(s);
+        }
+        lane_of[ii] = best_h * RES + half_fill[best_h];
+      }
+      half_fill[best_h]++;
+    }
+    // outputs, position by position within the stage
+    for (uint32_t k = 0; k < M; k++)
+      for (uint32_t ii : v) {
+        Inst& in = insts[ii];
+        if (in.depth != k) continue;
+        uint32_t& taken = write_mask[mask_base[pos] + k * HALVES + lane_of[ii] / RES];
+        const uint32_t d = gl[in.gate]->dst;
+        if (d < nsig && pers[d]) { in.slot = T.persist.at(d); taken |= 1u << (in.slot % RES); continue; }
+        if (last[ii] == 0) { in.slot = NONE - 1; continue; }  // read only through the register chain
+        uint32_t r = RES;
+        for (uint32_t c = 0; c < RES; c++) {
+          if (taken >> c & 1u) continue;
+          if (r == RES) { r = c; continue; }
+          const size_t fc = free_res[c].size(), fr = free_res[r].size();
+          if (fc > fr || (fc == 0 && fr == 0 && fresh[c] < fresh[r])) r = c;
+        }
+        if (r == RES) r = 0;  // every residue taken: accept a conflict
+        taken |= 1u << r;
+        if (!free_res[r].empty()) { in.slot = free_res[r].back(); free_res[r].pop_back(); }
+        else { in.slot = fresh[r]; fresh[r] += RES; }
+        slot_end = std::max(slot_end, in.slot + 1);
+        releases.emplace(std::max(last[ii], in.pos), in.slot);
+      }
+  }
+  T.n_slots = slot_end;
   // commits: a value or reset signal that is itself a committed register would
-  // be overwritten mid-commit; stage it through a copy in an extra level that
-  // runs after every cycle level.
+  // be overwritten mid-commit; stage it through a copy in an extra stage that
+  // runs after every cycle stage.
   std::unordered_set<uint32_t> committed;
   for (auto& c : N.commits) committed.insert(c.reg);
-  std::vector<uint32_t> staging;
+  std::vector<uint32_t> staging;  // (src slot, temp slot) pairs
   for (auto& c : N.commits) {
     uint32_t vs = slot_of(c.val, false);
     uint32_t rs = c.rsig == NONE ? NONE : slot_of(c.rsig, false);
     auto stage = [&](uint32_t slot) {
       uint32_t t = T.n_slots++;
-      staging.push_back(slot | (T.c1 << 16));
-      staging.push_back(t | (G_COPY << 16));
+      staging.push_back(slot); staging.push_back(t);
       return t;
     };
     if (committed.count(c.val)) vs = stage(vs);
     if (c.rsig != NONE && committed.count(c.rsig)) rs = stage(rs);
     T.commit.insert(T.commit.end(), {T.persist.at(c.reg), vs, rs == NONE ? 0u : rs, c.pol | (c.rbit << 1) | (c.has_reset << 2)});
   }
-  if (T.n_slots > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
-  // Kernel levels: empty levels are dropped, levels wider than one gate per
-  // thread are split into independent sub-levels (one barrier each).
-  T.level_start.push_back(0);
-  uint32_t nlev = 0;
-  auto flush = [&](const std::vector<uint32_t>& d) {
-    const size_t n = d.size() / 2;
-    for (size_t at = 0; at < n; at += MAX_LEVEL_GATES) {
-      size_t e = std::min(n, at + MAX_LEVEL_GATES);
-      T.desc.insert(T.desc.end(), d.begin() + 2 * at, d.begin() + 2 * e);
-      T.level_start.push_back(uint32_t(T.desc.size() / 2));
-      nlev++;
+  T.sink = (T.n_slots + RES - 1) & ~(RES - 1);
+  if (T.sink + SINK_SLOTS > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
+  // Kernel stages: cycle stages, staging stages (copies at position 0), post stages.
+  const uint32_t n_staging = uint32_t((staging.size() / 2 + TILE_THREADS - 1) / TILE_THREADS);
+  T.n_cycle_stages = L + n_staging;
+  T.n_post_stages = L2;
+  const uint32_t KS = T.n_cycle_stages + T.n_post_stages;
+  const size_t row_words = size_t(M) * TILE_THREADS * 2;
+  T.desc.assign(size_t(KS) * row_words, NONE);
+  auto entry = [&](uint32_t kst, uint32_t k, uint32_t lane) -> uint32_t* { return &T.desc[kst * row_words + (size_t(k) * TILE_THREADS + lane) * 2]; };
+  std::vector<uint32_t> kstage_mask(size_t(KS) * M * HALVES, 0);
+  for (uint32_t ii = 0; ii < insts.size(); ii++) {
+    const Inst& in = insts[ii];
+    const CGate& g = *gl[in.gate];
+    const uint32_t kst = in.post ? T.n_cycle_stages + (in.pos - POS_POST0 - 1) : in.pos - 1;
+    uint32_t a = in.chained && !in.chain_b ? NONE : operand_slot(in, 0);
+    uint32_t b = in.chained && in.chain_b ? NONE : operand_slot(in, 1);
+    uint32_t op = g.op;
+    if (in.chained && in.chain_b) {  // the chained operand goes to a
+      std::swap(a, b);
+      op = (op & ~(GI_A | GI_B)) | ((op & GI_A) ? GI_B : 0u) | ((op & GI_B) ? GI_A : 0u);
     }
-    return nlev;  // clamps of these gates run after the level's last sub-level
-  };
-  std::vector<uint32_t> kpos_cycle(L + 1, 0), kpos_post(L2 + 1, 0);
-  for (uint32_t l = 1; l <= L; l++) kpos_cycle[l] = flush(lv_desc[l]);
-  flush(staging);
-  const uint32_t n_cycle = nlev;
-  for (uint32_t l = 1; l <= L2; l++) kpos_post[l] = flush(lv2_desc[l]);
-  T.n_cycle_levels = n_cycle; T.n_post_levels = nlev - n_cycle;
-  // clamp sites in kernel numbering (index of the last sub-level + 1); commit -> CLAMP_COMMIT
-  for (auto& fc : forced_cycle) T.clamp_sites.push_back({fc.first, {kpos_cycle[fc.second.first], fc.second.second}});
-  for (auto& fp : forced_post) T.clamp_sites.push_back({fp.first, {kpos_post[fp.second.first], fp.second.second}});
+    const uint32_t d = in.slot == NONE - 1 ? NONE : in.slot;
+    uint32_t* e = entry(kst, in.depth, lane_of[ii]);
+    e[0] = (a == NONE ? 0xFFFFu : a) | ((b == NONE ? 0xFFFFu : b) << 16);
+    e[1] = (d == NONE ? 0xFFFFu : d) | (op << 16) | (in.chained ? DESC_CHAINED : 0u);
+    kstage_mask[(size_t(kst) * M + in.depth) * HALVES + lane_of[ii] / RES] =
+        write_mask[mask_base[in.pos] + in.depth * HALVES + lane_of[ii] / RES];
+    if (g.dst < nsig && forced[g.dst] && d != NONE) T.clamp_sites.push_back({g.dst, {kst + 1, d}});
+  }
+  for (size_t q = 0; q < staging.size() / 2; q++) {
+    const uint32_t kst = L + uint32_t(q / TILE_THREADS), lane = uint32_t(q % TILE_THREADS);
+    uint32_t* e = entry(kst, 0, lane);
+    e[0] = staging[2 * q] | (T.c1 << 16);
+    e[1] = staging[2 * q + 1] | (G_COPY << 16);
+    kstage_mask[(size_t(kst) * M) * HALVES + lane / RES] |= 1u << (staging[2 * q + 1] % RES);
+  }
+  // Lanes without a gate at a position, and outputs read only through the
+  // register chain, write the sink slot of a residue no output of their half
+  // uses; operand fields not read (a chained operand, an idle lane) repeat the
+  // first real read of the half (a broadcast) or constant 0.
+  for (uint32_t kst = 0; kst < KS; kst++)
+    for (uint32_t k = 0; k < M; k++)
+      for (uint32_t h = 0; h < HALVES; h++) {
+        uint32_t taken = kstage_mask[(size_t(kst) * M + k) * HALVES + h];
+        uint32_t rep_a = T.c0, rep_b = T.c0;
+        bool have_a = false, have_b = false;
+        for (uint32_t l = h * RES; l < h * RES + RES; l++) {
+          const uint32_t* e = entry(kst, k, l);
+          if (e[0] == NONE) continue;
+          if (!have_a && (e[0] & 0xFFFFu) != 0xFFFFu) { rep_a = e[0] & 0xFFFFu; have_a = true; }
+          if (!have_b && (e[0] >> 16) != 0xFFFFu) { rep_b = e[0] >> 16; have_b = true; }
+        }
+        uint32_t next_r = 0;
+        auto sink_slot = [&]() {
+          while (next_r < RES && (taken >> next_r & 1u)) next_r++;
+          const uint32_t r = next_r < RES ? next_r : 0u;
+          if (next_r < RES) taken |= 1u << next_r;
+          return T.sink + r;
+        };
+        for (uint32_t l = h * RES; l < h * RES + RES; l++) {
+          uint32_t* e = entry(kst, k, l);
+          if (e[0] == NONE) { e[0] = rep_a | (rep_b << 16); e[1] = sink_slot() | (G_AND << 16); continue; }
+          if ((e[0] & 0xFFFFu) == 0xFFFFu) e[0] = (e[0] & 0xFFFF0000u) | rep_a;
+          if ((e[0] >> 16) == 0xFFFFu) e[0] = (e[0] & 0xFFFFu) | (rep_b << 16);
+          if ((e[1] & 0xFFFFu) == 0xFFFFu) e[1] = (e[1] & 0xFFFF0000u) | sink_slot();
+        }
+      }
   // persistent load/store lists
   for (auto& kv : T.persist) {
     T.load_slot.push_back(kv.second); T.load_node.push_back(kv.first);
@@ -619,7 +858,10 @@ class GateEngine final : public Engine {
 
   // LDS of one tile: live slots, the sink slots of gate-less threads, the
   // 64-word signature.
-  static size_t tile_lds_bytes(uint32_t n_slots, uint32_t w) { return size_t(n_slots + SINK_SLOTS + 64) * w * 4; }
+  static uint32_t sink_base(uint32_t n_slots) { return (n_slots + 31u) & ~31u; }  // residue of sink + lane = lane
+  static size_t tile_lds_bytes(uint32_t n_slots, uint32_t w) {
+    return size_t(sink_base(n_slots) + SINK_SLOTS + 64) * w * 4;
+  }
 
   uint32_t choose_tile_words(uint32_t n_slots) const {
     if (opt_.tile_words) return opt_.tile_words;
@@ -630,7 +872,7 @@ class GateEngine final : public Engine {
   }
 
   // W lane-words per tile; byte_addr: descriptors hold LDS byte offsets.
-  struct Prepared { TileArgs A; uint32_t W = 0, tiles = 0; size_t lds = 0; bool byte_addr = false; };
+  struct Prepared { TileArgs A; uint32_t W = 0, M = 2, tiles = 0; size_t lds = 0; bool byte_addr = false; };
 
   // The compiled tile program depends on which inputs are driven, what is
   // observed and where forces sit; repeated runs with the same shape reuse it.
@@ -641,7 +883,7 @@ class GateEngine final : public Engine {
     for (size_t i = 0; i < spec.in_sig.size(); i++) { put(spec.in_sig[i]); put(spec.in_mode[i]); put(spec.in_arg.empty() ? 0 : spec.in_arg[i]); }
     put(spec.obs_sig.size());
     for (uint32_t s : spec.obs_sig) put(s);
-    put(spec.observe_post); put(forces_version_); put(opt_.tile_words);
+    put(spec.observe_post); put(forces_version_); put(opt_.tile_words); put(opt_.stage_gates);
     return k;
   }
 
@@ -679,7 +921,21 @@ class GateEngine final : public Engine {
   Prepared compile_and_upload(const RunSpec& spec) {
     std::vector<uint8_t> forced(N.nsig, 0);
     for (auto& f : forces_) forced[f.sig] = 1;
-    TileProgram T = compile_tile(N, P, spec, forced);
+    // Gate positions per thread per stage: the M whose stage count x (fixed
+    // stage cost + per-position cost) is lowest, unless the options fix it.
+    // The kernel is VALU-bound, so a position costs about twice the fixed part
+    // (barrier, LDS round trip) of a stage.
+    TileProgram T;
+    if (opt_.stage_gates) {
+      T = compile_tile(N, P, spec, forced, opt_.stage_gates);
+    } else {
+      double best = 0;
+      for (uint32_t m : {2u, 3u}) {
+        TileProgram C = compile_tile(N, P, spec, forced, m);
+        const double cost = double(C.n_cycle_stages + C.n_post_stages) * (0.5 + m);
+        if (m == 2 || cost < best) { best = cost; T = std::move(C); }
+      }
+    }
     uint32_t W = choose_tile_words(T.n_slots);
     if (W != 1 && W != 2 && W != 4 && W != 8) throw Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
     size_t lds = tile_lds_bytes(T.n_slots, W);
@@ -687,10 +943,11 @@ class GateEngine final : public Engine {
       W = 1; lds = tile_lds_bytes(T.n_slots, 1);
       if (lds > lds_max_) throw Error("E_HIP_TOO_LARGE: live gate state does not fit in LDS for one 32-lane tile");
     }
-    const uint32_t sink = T.n_slots, n_slots = T.n_slots + SINK_SLOTS;
-    if (n_slots > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
+    const uint32_t sink = T.sink, n_slots = sink + SINK_SLOTS;
+    if (sink != sink_base(T.n_slots)) throw Error("E_HIP_INTERNAL: sink slot base");
     const bool byte_addr = lds <= 65536;
-    W_ = W; last_slots_ = T.n_slots; last_levels_ = T.n_cycle_levels + T.n_post_levels; last_gates_ = uint32_t(T.desc.size() / 2);
+    const uint32_t M = T.M;
+    W_ = W; M_ = M; last_slots_ = T.n_slots; last_levels_ = T.n_cycle_stages + T.n_post_stages; last_gates_ = T.n_gates;
     const uint32_t tiles = uint32_t((nwords_ + W - 1) / W);
     // per-tile force clamp entries (pos, slot, word, and, or), merged per LDS word
     std::vector<uint32_t> fent, fstart(tiles + 1, 0);
@@ -718,25 +975,27 @@ class GateEngine final : public Engine {
     }
     TileArgs A{};
     A.state = state_.p; A.nwords = nwords_;
-    // descriptors at a fixed stride per level (level * TILE_THREADS + thread),
-    // PREFETCH look-ahead levels of padding at the end; a thread without a
-    // gate reads constant 0 and writes its sink slot
-    const uint32_t nlev = T.n_cycle_levels + T.n_post_levels;
-    auto enc = [&](uint32_t slot) { return byte_addr ? slot * W * 4 : slot; };
-    std::vector<uint2> desc(size_t(nlev + PREFETCH) * TILE_THREADS);
-    for (size_t i = 0; i < desc.size(); i++)
-      desc[i] = make_uint2(enc(T.c0) | (enc(T.c0) << 16), enc(sink + uint32_t(i % SINK_SLOTS)));
-    for (uint32_t L = 0; L < nlev; L++)
-      for (uint32_t g = T.level_start[L]; g < T.level_start[L + 1]; g++) {
-        const uint32_t x = T.desc[2 * g], y = T.desc[2 * g + 1];
-        uint32_t op = y >> 16;
+    // descriptors [stage][position][thread] as compiled, STAGE_AHEAD padding
+    // stages at the end; every field is checked against the tile's slots
+    const uint32_t KS = T.n_cycle_stages + T.n_post_stages;
+    const size_t stride = size_t(M) * TILE_THREADS;
+    auto enc = [&](uint32_t slot) {
+      if (slot >= n_slots) throw Error("E_HIP_INTERNAL: descriptor slot out of range");
+      return byte_addr ? slot * W * 4 : slot;
+    };
+    std::vector<uint2> desc(size_t(KS + STAGE_AHEAD) * stride);
+    for (size_t i = 0; i < desc.size(); i++) {
+      if (i < size_t(KS) * stride) {
+        const uint32_t x = T.desc[2 * i], y = T.desc[2 * i + 1];
+        uint32_t op = (y >> 16) & 0xFu;
         if (op & GI_X) op &= ~(GI_A | GI_B);  // xor ignores input inversions
-        desc[size_t(L) * TILE_THREADS + (g - T.level_start[L])] =
-            make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16));
+        desc[i] = make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16) | (y & DESC_CHAINED));
+      } else {
+        desc[i] = make_uint2(enc(T.c0) | (enc(T.c0) << 16), enc(sink + uint32_t(i % 32)));
       }
+    }
     A.desc = upload_prog(0, desc);
-    A.level_start = upload_prog(1, T.level_start);
-    A.n_cycle_levels = T.n_cycle_levels; A.n_post_levels = T.n_post_levels;
+    A.n_cycle_stages = T.n_cycle_stages; A.n_post_stages = T.n_post_stages;
     A.load_slot = upload_prog(2, T.load_slot); A.load_node = upload_prog(3, T.load_node); A.n_load = uint32_t(T.load_slot.size());
     A.stim_slot = upload_prog(4, T.stim_slot); A.stim_mode = upload_prog(5, T.stim_mode);
     A.stim_stream = upload_prog(6, T.stim_stream); A.stim_arg = upload_prog(7, T.stim_arg); A.n_stim = uint32_t(T.stim_slot.size());
@@ -748,7 +1007,7 @@ class GateEngine final : public Engine {
     A.force_start = fent.empty() ? nullptr : upload_prog(11, fstart);
     A.n_slots = n_slots; A.c0 = T.c0; A.c1 = T.c1;
     Prepared R;
-    R.A = A; R.W = W; R.tiles = tiles; R.lds = lds; R.byte_addr = byte_addr;
+    R.A = A; R.W = W; R.M = M; R.tiles = tiles; R.lds = lds; R.byte_addr = byte_addr;
     return R;
   }
 
@@ -772,7 +1031,7 @@ class GateEngine final : public Engine {
     const uint32_t W = R.W, tiles = R.tiles;
     const size_t lds = R.lds;
     const uint64_t chunk = opt_.ticks_per_launch ? opt_.ticks_per_launch : std::max<uint64_t>(spec.cycles, 1);
-    const uint32_t resident = resident_tiles(W, R.byte_addr, lds);
+    const uint32_t resident = resident_tiles(W, R.byte_addr, R.M, lds);
     const uint32_t chunks = opt_.tile_chunks ? opt_.tile_chunks : choose_chunks(tiles, resident);
     if (chunks > 1) { qctl_.alloc(size_t(tiles) + 2); }
     stats_ = RunStats{};
@@ -791,10 +1050,10 @@ class GateEngine final : public Engine {
         Q.wait_limit = wall_khz_ * 1000u * 120u;  // two minutes
         hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(256), 0, stream_, qctl_.p, tiles + 2);
         const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(resident, 1), tiles * Q.n_chunks);
-        launch_queue(W, R.byte_addr, grid, lds, A, Q);
+        launch_queue(R, grid, A, Q);
         last_chunks_ = Q.n_chunks;
       } else {
-        launch_tile(W, R.byte_addr, tiles, lds, A);
+        launch_tile(R, A);
       }
       stats_.launches++;
     }
@@ -816,9 +1075,11 @@ class GateEngine final : public Engine {
     Prepared R = prepare(spec, true);
     batch_digest_ = R.A.digest != nullptr;
     BatchJob J;
-    J.kind = ENGINE_GATE; J.variant = R.W | (R.byte_addr ? GATE_VARIANT_BYTE_ADDR : 0u); J.n_items = R.tiles; J.lds_bytes = R.lds;
-    // per tile and cycle: one barrier-separated pass per level plus its gates
-    J.item_cost = double(spec.cycles) * (double(R.A.n_cycle_levels + R.A.n_post_levels) * 64.0 +
+    J.kind = ENGINE_GATE;
+    J.variant = R.W | (R.byte_addr ? GATE_VARIANT_BYTE_ADDR : 0u) | (R.M << GATE_VARIANT_M_SHIFT);
+    J.n_items = R.tiles; J.lds_bytes = R.lds;
+    // per tile and cycle: one barrier-separated pass per stage plus its gates
+    J.item_cost = double(spec.cycles) * (double(R.A.n_cycle_stages + R.A.n_post_stages) * (48.0 + 8.0 * R.M) +
                                          double(last_gates_) * R.W / 16.0);
     J.args.resize(sizeof(TileArgs));
     std::memcpy(J.args.data(), &R.A, sizeof(TileArgs));
@@ -834,64 +1095,43 @@ class GateEngine final : public Engine {
     wires_stale_ = wires_stale_ || spec.cycles > 0;
   }
 
-  template <int W, bool kBA>
-  void launch_w(uint32_t tiles, size_t lds, const TileArgs& A) {
-    if (lds > 65536)
-      RRTL_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_kernel<W, kBA>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL((tile_kernel<W, kBA>), dim3(tiles), dim3(TILE_THREADS), lds, stream_, A);
+  // Kernel entry points of the compiled variant (W, byte addressing, M).
+  static const void* tile_fn(uint32_t W, bool byte_addr, uint32_t M) {
+    const void* fn = nullptr;
+    with_tile_variant(W, byte_addr, M, [&](auto w, auto ba, auto m) {
+      fn = reinterpret_cast<const void*>(&tile_kernel<decltype(w)::value, decltype(ba)::value, decltype(m)::value>);
+    });
+    return fn;
   }
-  template <int W, bool kBA>
-  const void* queue_fn() const { return reinterpret_cast<const void*>(&tile_queue_kernel<W, kBA>); }
-  const void* queue_kernel(uint32_t W, bool byte_addr) const {
-    switch (W * 2 + (byte_addr ? 1 : 0)) {
-      case 2: return queue_fn<1, false>();
-      case 3: return queue_fn<1, true>();
-      case 4: return queue_fn<2, false>();
-      case 5: return queue_fn<2, true>();
-      case 8: return queue_fn<4, false>();
-      case 9: return queue_fn<4, true>();
-      case 16: return queue_fn<8, false>();
-      default: return queue_fn<8, true>();
-    }
+  static const void* queue_fn(uint32_t W, bool byte_addr, uint32_t M) {
+    const void* fn = nullptr;
+    with_tile_variant(W, byte_addr, M, [&](auto w, auto ba, auto m) {
+      fn = reinterpret_cast<const void*>(&tile_queue_kernel<decltype(w)::value, decltype(ba)::value, decltype(m)::value>);
+    });
+    return fn;
   }
   // Workgroups of the queue kernel the device holds at once.
-  uint32_t resident_tiles(uint32_t W, bool byte_addr, size_t lds) {
-    const void* fn = queue_kernel(W, byte_addr);
+  uint32_t resident_tiles(uint32_t W, bool byte_addr, uint32_t M, size_t lds) {
+    const void* fn = queue_fn(W, byte_addr, M);
     if (lds > 65536) RRTL_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     int per_cu = 0;
     RRTL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE_THREADS, lds));
     return uint32_t(std::max(per_cu, 0)) * cus_;
   }
-  template <int W, bool kBA>
-  void launch_q(uint32_t grid, size_t lds, const TileArgs& A, const QueueArgs& Q) {
-    hipLaunchKernelGGL((tile_queue_kernel<W, kBA>), dim3(grid), dim3(TILE_THREADS), lds, stream_, A, Q);
-  }
-  void launch_queue(uint32_t W, bool byte_addr, uint32_t grid, size_t lds, const TileArgs& A, const QueueArgs& Q) {
-    switch (W * 2 + (byte_addr ? 1 : 0)) {
-      case 2: launch_q<1, false>(grid, lds, A, Q); break;
-      case 3: launch_q<1, true>(grid, lds, A, Q); break;
-      case 4: launch_q<2, false>(grid, lds, A, Q); break;
-      case 5: launch_q<2, true>(grid, lds, A, Q); break;
-      case 8: launch_q<4, false>(grid, lds, A, Q); break;
-      case 9: launch_q<4, true>(grid, lds, A, Q); break;
-      case 16: launch_q<8, false>(grid, lds, A, Q); break;
-      default: launch_q<8, true>(grid, lds, A, Q); break;
-    }
+  void launch_queue(const Prepared& R, uint32_t grid, const TileArgs& A, const QueueArgs& Q) {
+    with_tile_variant(R.W, R.byte_addr, R.M, [&](auto w, auto ba, auto m) {
+      hipLaunchKernelGGL((tile_queue_kernel<decltype(w)::value, decltype(ba)::value, decltype(m)::value>), dim3(grid),
+                         dim3(TILE_THREADS), R.lds, stream_, A, Q);
+    });
     RRTL_HIP_CHECK(hipGetLastError());
   }
-
-  void launch_tile(uint32_t W, bool byte_addr, uint32_t tiles, size_t lds, const TileArgs& A) {
-    switch (W * 2 + (byte_addr ? 1 : 0)) {
-      case 2: launch_w<1, false>(tiles, lds, A); break;
-      case 3: launch_w<1, true>(tiles, lds, A); break;
-      case 4: launch_w<2, false>(tiles, lds, A); break;
-      case 5: launch_w<2, true>(tiles, lds, A); break;
-      case 8: launch_w<4, false>(tiles, lds, A); break;
-      case 9: launch_w<4, true>(tiles, lds, A); break;
-      case 16: launch_w<8, false>(tiles, lds, A); break;
-      default: launch_w<8, true>(tiles, lds, A); break;
-    }
+  void launch_tile(const Prepared& R, const TileArgs& A) {
+    const void* fn = tile_fn(R.W, R.byte_addr, R.M);
+    if (R.lds > 65536) RRTL_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(R.lds)));
+    with_tile_variant(R.W, R.byte_addr, R.M, [&](auto w, auto ba, auto m) {
+      hipLaunchKernelGGL((tile_kernel<decltype(w)::value, decltype(ba)::value, decltype(m)::value>), dim3(R.tiles),
+                         dim3(TILE_THREADS), R.lds, stream_, A);
+    });
     RRTL_HIP_CHECK(hipGetLastError());
   }
 
@@ -906,7 +1146,7 @@ class GateEngine final : public Engine {
   std::vector<Force> forces_;
   bool forces_dirty_ = true, wires_stale_ = false, has_digest_ = false;
   size_t trace_bytes_ = 0;
-  uint32_t W_ = 0, last_slots_ = 0, last_levels_ = 0, last_gates_ = 0, last_chunks_ = 1, cus_ = 1;
+  uint32_t W_ = 0, M_ = 2, last_slots_ = 0, last_levels_ = 0, last_gates_ = 0, last_chunks_ = 1, cus_ = 1;
   uint64_t wall_khz_ = 100000;
   DevBuf<uint32_t> qctl_;  // work-queue counter, error flag, per-tile chunk counts
   Prepared plan_;

@@ -129,13 +129,13 @@ def netlist_outputs(p):
     return [i for i, s in enumerate(p.signals) if s.kind == KIND_OUTPUT]
 
 
-@pytest.mark.parametrize("tile_words", [1, 2, 4, 8])
-def test_synthetic_netlist_trace_matches_oracle(tile_words):
-    flush_print(f"case: small synthetic netlist tile_words={tile_words}")
+@pytest.mark.parametrize("tile_words,stage_gates", [(1, 2), (2, 2), (4, 2), (8, 2), (1, 3), (2, 3), (4, 3), (8, 3)])
+def test_synthetic_netlist_trace_matches_oracle(tile_words, stage_gates):
+    flush_print(f"case: small synthetic netlist tile_words={tile_words} stage_gates={stage_gates}")
     p = small_netlist_program(gates=3000, levels=24, dffs=300, inputs=48, outputs=32, seed=4)
     lanes, cycles, seed = 1000, 40, 17
     ins, obs = netlist_inputs(p), netlist_outputs(p)
-    sim = HipSimulator(p, lanes, engine="gate", tile_words=tile_words)
+    sim = HipSimulator(p, lanes, engine="gate", tile_words=tile_words, stage_gates=stage_gates)
     sim.run(cycles, seed=seed, inputs=ins, observe=obs, trace=True)
     ref = oracle.BitParallelRef(p, lanes)
     dg, tr = ref.run(seed=seed, cycles=cycles, inputs=ins, observe=obs, digest=True, trace=True)
