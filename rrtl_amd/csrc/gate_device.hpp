@@ -31,11 +31,12 @@ __host__ __device__ inline uint32_t gate_eval(uint32_t op, uint32_t a, uint32_t 
 struct TileArgs {
   uint32_t* state; uint64_t nwords;             // HBM rows [node][nwords]
   const uint2* desc;                            // [stage][position][thread], STAGE_AHEAD padding stages
-  uint32_t n_cycle_stages, n_post_stages;
+  const uint32_t* stage_fix;                    // per stage: bit k = position k applies output inversions
+  uint32_t n_cycle_stages, n_post_stages, any_fix;
   const uint32_t* load_slot; const uint32_t* load_node; uint32_t n_load;
   const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
   const uint4* commit; uint32_t n_commit;
-  const uint32_t* obs_slot; uint32_t n_obs;
+  const uint32_t* obs_slot; uint32_t n_obs;     // slot | stored value inverted << 31
   const uint32_t* force; const uint32_t* force_start;  // per tile: 5-word entries (pos, slot, word, and, or)
   uint32_t* digest;                                    // [64][nwords] or null
   uint32_t* trace;                                     // [cycles][n_obs][nwords] or null
@@ -108,15 +109,6 @@ __device__ __forceinline__ uint32_t op3_xor_or_and(uint32_t A, uint32_t B, uint3
   return __builtin_amdgcn_bitop3_b32(A, B, x, 0x68);
 }
 
-// a chained operand: ch ? prev : v per bit, in one v_bitop3_b32 (S0, S1, S2) =
-// (v, prev, ch): on v = 0xF0, prev = 0xCC, ch = 0xAA it is (0xAA & 0xCC) |
-// (0x55 & 0xF0) = 0xD8.
-__device__ __forceinline__ uint32_t mux_chain(uint32_t v, uint32_t prev, uint32_t ch) {
-  return __builtin_amdgcn_bitop3_b32(v, prev, ch, 0xD8);
-}
-
-constexpr uint32_t DESC_CHAINED = 1u << 20;  // descriptor flag: operand a is the previous position's result
-
 template <int M> struct StageDesc { uint2 d[M]; };
 
 template <int M>
@@ -126,32 +118,31 @@ __device__ __forceinline__ void load_stage(StageDesc<M>& D, const uint2* row) {
 }
 
 // One stage of this thread: M gate positions over every lane-word of the tile,
-//   r = (x ? A ^ B : A & B) ^ io,  A = a ^ ia, B = b ^ ib  (per-thread masks),
-// where a is the previous position's r when the descriptor says chained.  All
-// LDS reads of the stage are issued first (their operands come from earlier
-// stages), then the positions run in order, each storing its result.
+//   t = x ? A ^ B : A & B,  A = a ^ ia, B = b ^ ib  (per-thread masks),
+// stored as is (its output inversion is carried by the readers) unless the
+// stage's fix bit for the position says some gate there applies it (t ^ io).
+// Position 0 reads a and b; a later position takes a from the previous
+// position's t (registers) and reads b.  All LDS reads of the stage are issued
+// first (their operands come from earlier stages), then the positions run in
+// order, each storing its result.
 template <int W, bool kBA, int M>
-__device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D) {
-  Words<W> va[M], vb[M];
+__device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D, uint32_t fix) {
+  Words<W> va, vb[M];
+  va = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[0].x & 0xFFFFu));
 #pragma unroll
-  for (int k = 0; k < M; k++) {
-    va[k] = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[k].x & 0xFFFFu));
-    vb[k] = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[k].x >> 16));
-  }
+  for (int k = 0; k < M; k++) vb[k] = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[k].x >> 16));
   Words<W> prev;
 #pragma unroll
   for (int k = 0; k < M; k++) {
     const uint32_t y = D.d[k].y;
-    const uint32_t ia = 0u - ((y >> 16) & 1u), ib = 0u - ((y >> 17) & 1u), io = 0u - ((y >> 18) & 1u),
-                   x = 0u - ((y >> 19) & 1u);
+    const uint32_t ia = 0u - ((y >> 16) & 1u), ib = 0u - ((y >> 17) & 1u), x = 0u - ((y >> 19) & 1u);
     Words<W> r;
-    if (k == 0) {
 #pragma unroll
-      for (int w = 0; w < W; w++) r.v[w] = op3_xor_or_and(va[k].v[w] ^ ia, vb[k].v[w] ^ ib, x) ^ io;
-    } else {
-      const uint32_t ch = 0u - ((y >> 20) & 1u);
+    for (int w = 0; w < W; w++) r.v[w] = op3_xor_or_and((k == 0 ? va.v[w] : prev.v[w]) ^ ia, vb[k].v[w] ^ ib, x);
+    if (fix >> k & 1u) {
+      const uint32_t io = 0u - ((y >> 18) & 1u);
 #pragma unroll
-      for (int w = 0; w < W; w++) r.v[w] = op3_xor_or_and(mux_chain(va[k].v[w], prev.v[w], ch) ^ ia, vb[k].v[w] ^ ib, x) ^ io;
+      for (int w = 0; w < W; w++) r.v[w] ^= io;
     }
     uint32_t* o = lds_at<W, kBA>(lds, y & 0xFFFFu);
     if constexpr (W == 1) { o[0] = r.v[0]; }
@@ -184,16 +175,19 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32
     for (uint32_t i = tid; i < A.n_obs * W; i += TILE_THREADS) {
       uint32_t j = i / W, w = i % W;
       uint64_t gw = w0 + w;
-      if (gw < A.nwords) A.trace[(c * A.n_obs + j) * A.nwords + gw] = lds[A.obs_slot[j] * W + w];
+      const uint32_t o = A.obs_slot[j];
+      if (gw < A.nwords) A.trace[(c * A.n_obs + j) * A.nwords + gw] = lds[(o & 0x7FFFFFFFu) * W + w] ^ (0u - (o >> 31));
     }
   if (A.digest) {
     Words<W> s;
     if (tid < 64) {
       s = ld_words<W>(sig, tid);
       for (uint32_t j = (tid - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) {
-        const Words<W> v = ld_words<W>(lds, A.obs_slot[j]);
+        const uint32_t o = A.obs_slot[j];
+        const Words<W> v = ld_words<W>(lds, o & 0x7FFFFFFFu);
+        const uint32_t inv = 0u - (o >> 31);
 #pragma unroll
-        for (int k = 0; k < W; k++) s.v[k] ^= v.v[k];
+        for (int k = 0; k < W; k++) s.v[k] ^= v.v[k] ^ inv;
       }
       st_words<W>(sig, tid, s);
     }
@@ -220,6 +214,9 @@ __device__ __forceinline__ void commit_regs(const TileArgs& A, uint32_t* lds, ui
   for (uint32_t k = threadIdx.x; k < A.n_commit; k += TILE_THREADS) {
     const uint4 cm = A.commit[k];
     Words<W> v = ld_words<W>(lds, cm.y);
+    const uint32_t vinv = 0u - ((cm.w >> 3) & 1u);  // the value's slot holds its complement
+#pragma unroll
+    for (int q = 0; q < W; q++) v.v[q] ^= vinv;
     if (cm.w & 4) {
       const Words<W> a = ld_words<W>(lds, cm.z);
       const uint32_t pol = 0u - (cm.w & 1u), rv = 0u - ((cm.w >> 1) & 1u);
@@ -236,35 +233,45 @@ __device__ __forceinline__ void commit_regs(const TileArgs& A, uint32_t* lds, ui
 // earlier), the fetch for STAGE_AHEAD stages later into the same registers
 // (the table carries STAGE_AHEAD padding stages, so the fetch needs no bound
 // check), the stage barrier and, when forces exist, their clamps.  No other
-// global access happens in this loop, so the fetches stay in flight.
-template <int W, bool kBA, int M, bool kClamp>
-__device__ __forceinline__ void stage_step(const TileArgs& A, uint32_t* lds, uint32_t S, const uint2* ahead,
-                                           StageDesc<M>& D, uint32_t fb, uint32_t fe) {
-  eval_stage<W, kBA, M>(lds, D);
-  load_stage<M>(D, ahead);
+// global access happens in this loop, so the fetches stay in flight.  The
+// fetch address is the uniform row of stage S + STAGE_AHEAD plus the thread's
+// fixed offset.  kFix: some stage has gates that apply their output inversion.
+template <int W, bool kBA, int M, bool kClamp, bool kFix>
+__device__ __forceinline__ void stage_step(const TileArgs& A, uint32_t* lds, uint32_t S, StageDesc<M>& D, uint32_t fb,
+                                           uint32_t fe) {
+  eval_stage<W, kBA, M>(lds, D, kFix ? A.stage_fix[S] : 0u);
+  load_stage<M>(D, A.desc + size_t(S + STAGE_AHEAD) * (uint32_t(M) * TILE_THREADS));
   __syncthreads();
   if (kClamp) { apply_clamps<W>(A, lds, S + 1, fb, fe); __syncthreads(); }
 }
 
 // Stages [S0, S1): unrolled by STAGE_AHEAD so each descriptor register set is
-// loaded and consumed in a fixed step (no register rotation).  `row` is the
-// uniform address of a stage's descriptor rows.
-template <int W, bool kBA, int M, bool kClamp>
+// loaded and consumed in a fixed step (no register rotation).
+template <int W, bool kBA, int M, bool kClamp, bool kFix>
 __device__ __forceinline__ void run_stages(const TileArgs& A, uint32_t* lds, uint32_t S0, uint32_t S1, uint32_t fb,
                                            uint32_t fe) {
   static_assert(STAGE_AHEAD == 2, "run_stages is unrolled for two descriptor register sets");
-  constexpr uint32_t stride = uint32_t(M) * TILE_THREADS;
-  const uint2* row = A.desc + size_t(S0) * stride;
   StageDesc<M> D0, D1;
-  load_stage<M>(D0, row);
-  load_stage<M>(D1, row + stride);
-  row += 2 * stride;
+  load_stage<M>(D0, A.desc + size_t(S0) * (uint32_t(M) * TILE_THREADS));
+  load_stage<M>(D1, A.desc + size_t(S0 + 1) * (uint32_t(M) * TILE_THREADS));
   uint32_t S = S0;
-  for (; S + 2 <= S1; S += 2, row += 2 * stride) {
-    stage_step<W, kBA, M, kClamp>(A, lds, S, row, D0, fb, fe);
-    stage_step<W, kBA, M, kClamp>(A, lds, S + 1, row + stride, D1, fb, fe);
+  for (; S + 2 <= S1; S += 2) {
+    stage_step<W, kBA, M, kClamp, kFix>(A, lds, S, D0, fb, fe);
+    stage_step<W, kBA, M, kClamp, kFix>(A, lds, S + 1, D1, fb, fe);
   }
-  if (S < S1) stage_step<W, kBA, M, kClamp>(A, lds, S, row, D0, fb, fe);
+  if (S < S1) stage_step<W, kBA, M, kClamp, kFix>(A, lds, S, D0, fb, fe);
+}
+
+template <int W, bool kBA, int M>
+__device__ __forceinline__ void run_program(const TileArgs& A, uint32_t* lds, uint32_t S0, uint32_t S1, uint32_t fb,
+                                            uint32_t fe) {
+  if (fe > fb) {
+    if (A.any_fix) run_stages<W, kBA, M, true, true>(A, lds, S0, S1, fb, fe);
+    else run_stages<W, kBA, M, true, false>(A, lds, S0, S1, fb, fe);
+  } else {
+    if (A.any_fix) run_stages<W, kBA, M, false, true>(A, lds, S0, S1, fb, fe);
+    else run_stages<W, kBA, M, false, false>(A, lds, S0, S1, fb, fe);
+  }
 }
 
 template <int W, bool kBA, int M>
@@ -311,14 +318,10 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
     }
     __syncthreads();
     // cycle program, observe before the edge, commit, post program
-    if (fe > fb) run_stages<W, kBA, M, true>(A, lds, 0, A.n_cycle_stages, fb, fe);
-    else run_stages<W, kBA, M, false>(A, lds, 0, A.n_cycle_stages, fb, fe);
+    run_program<W, kBA, M>(A, lds, 0, A.n_cycle_stages, fb, fe);
     if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
     commit_regs<W>(A, lds, fb, fe);
-    if (A.n_post_stages) {
-      if (fe > fb) run_stages<W, kBA, M, true>(A, lds, A.n_cycle_stages, nst, fb, fe);
-      else run_stages<W, kBA, M, false>(A, lds, A.n_cycle_stages, nst, fb, fe);
-    }
+    if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fe);
     if (A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
   }
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {

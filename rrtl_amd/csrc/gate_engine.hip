@@ -158,15 +158,17 @@ GateNet build_net(const Program& P) {
 struct TileProgram {
   uint32_t M = 1;                        // gate positions per thread per stage
   uint32_t n_cycle_stages = 0, n_post_stages = 0;
-  // [stage][depth][thread] x 2 words: a | b << 16, dst | op << 16 | chained << 20
-  // (slot indices; sink slots are sink + k)
+  // [stage][position][thread] x 2 words: a | b << 16, dst | op << 16 (slot
+  // indices; sink slots are sink + k; a is unused at positions > 0, which
+  // take operand a from the previous position); op = ia | ib << 1 | fix << 2 | x << 3
   std::vector<uint32_t> desc;
+  std::vector<uint32_t> stage_fix;       // per stage: bit k = some gate at position k applies its inversion
   uint32_t n_gates = 0;                  // gates evaluated per cycle (cycle and post programs)
   std::vector<uint32_t> load_slot, load_node;     // persistent slots loaded at start, stored at end
   std::vector<uint32_t> stim_slot, stim_mode, stim_stream;
   std::vector<uint64_t> stim_arg;
   std::vector<uint32_t> commit;          // 4 words: reg_slot, val_slot, rsig_slot|NONE, flags(pol | rbit<<1 | has<<2)
-  std::vector<uint32_t> obs_slot;
+  std::vector<uint32_t> obs_slot;        // slot | stored value inverted << 31
   uint32_t n_slots = 0, sink = 0, c0 = 0, c1 = 0;  // live slots [0, n_slots), sink slots [sink, sink + 64)
   // per node: persistent slot, or NONE
   std::unordered_map<uint32_t, uint32_t> persist;
@@ -174,15 +176,13 @@ struct TileProgram {
   std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> clamp_sites;
 };
 
-constexpr uint32_t DESC_CHAINED = 1u << 20;  // operand a is this thread's previous gate result
 
 // Stage schedule of a gate subset.  A stage gives every thread up to M gate
-// positions, evaluated in order between two barriers.  Position 0 and any
-// position a thread cannot chain take a ready gate (every input from an
-// earlier stage), most urgent first (smallest as-late-as-possible level);
-// position k > 0 first tries to chain: a gate whose only same-stage input is
-// the thread's gate at position k - 1, read through exactly one operand and
-// passed in registers.  deps[i] are ordering edges (read-after-write through
+// positions, evaluated in order between two barriers.  Position 0 takes a
+// ready gate (every input from an earlier stage), most urgent first (smallest
+// as-late-as-possible level); position k > 0 takes a chained gate: one whose
+// only same-stage input is the thread's gate at position k - 1, read through
+// exactly one operand and passed in registers (or stays empty).  deps[i] are ordering edges (read-after-write through
 // prod_a / prod_b, plus write-after-read/write on rewritten nodes).
 struct StageSchedule {
   uint32_t stages = 0;
@@ -245,7 +245,6 @@ StageSchedule stage_schedule(size_t n, const std::vector<std::vector<uint32_t>>&
             continue;
           }
         }
-        if (!ready.empty()) { put(ready.top().second, t, k); ready.pop(); }
       }
     }
     for (uint32_t g : placed)
@@ -506,6 +505,52 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
       }
   }
   T.n_slots = slot_end;
+  // Output polarity.  A gate stores (x ? A ^ B : A & B) without its output
+  // inversion; the stored value is the true value XOR pol.  Readers fold the
+  // polarity of what they read into their input inversions (an AND-type reads
+  // a ^ ia ^ pol_a; an XOR-type carries pol_a ^ pol_b into its own polarity),
+  // the commit and the observation undo it, and only a gate whose slot must
+  // hold the true value (a persistent node, a clamped force) applies its
+  // inversion: `fix`.  eff_op = the op bits the kernel uses (ia, ib, fix, x).
+  std::vector<uint8_t> pol(insts.size(), 0);
+  std::vector<uint32_t> eff_op(insts.size(), 0);
+  {
+    std::vector<uint32_t> order(insts.size());
+    for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+      const Inst &a = insts[x], &b = insts[y];
+      return std::make_tuple(a.pos, a.depth) < std::make_tuple(b.pos, b.depth);
+    });
+    auto operand_pol = [&](const Inst& in, int op) -> uint32_t {
+      const uint32_t node = op == 0 ? gl[in.gate]->a : gl[in.gate]->b;
+      if (node == N.c0 || node == N.c1 || pers[node]) return 0;
+      const uint32_t p = op == 0 ? prod_a[in.gate] : prod_b[in.gate];
+      const uint32_t si = p == NONE ? NONE : inst_of(p, in.post);
+      if (si == NONE) throw Error("E_HIP_INTERNAL: node has no producer in the tile program");
+      return pol[si];
+    };
+    for (uint32_t ii : order) {
+      const Inst& in = insts[ii];
+      const uint32_t op = gl[in.gate]->op, d = gl[in.gate]->dst;
+      const uint32_t pa = operand_pol(in, 0), pb = operand_pol(in, 1), io = (op >> 2) & 1u;
+      uint32_t ia = 0, ib = 0, tpol;
+      if (op & GI_X) {
+        tpol = io ^ pa ^ pb;
+      } else {
+        ia = (op & 1u) ^ pa; ib = ((op >> 1) & 1u) ^ pb;
+        tpol = io;
+      }
+      const bool must_be_true = (d < nsig && (pers[d] || forced[d]));
+      const uint32_t fix = must_be_true ? tpol : 0u;
+      pol[ii] = uint8_t(tpol ^ fix);
+      eff_op[ii] = ia | (ib << 1) | (fix << 2) | (op & GI_X);
+    }
+  }
+  auto value_pol = [&](uint32_t node, bool post) -> uint32_t {
+    if (node == N.c0 || node == N.c1 || (node < nsig && pers[node])) return 0;
+    const uint32_t si = src_inst(node, post);
+    return si == NONE ? 0u : pol[si];
+  };
   // commits: a value or reset signal that is itself a committed register would
   // be overwritten mid-commit; stage it through a copy in an extra stage that
   // runs after every cycle stage.
@@ -520,9 +565,13 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
       staging.push_back(slot); staging.push_back(t);
       return t;
     };
+    const uint32_t vpol = value_pol(c.val, false), rpol = c.rsig == NONE ? 0u : value_pol(c.rsig, false);
     if (committed.count(c.val)) vs = stage(vs);
     if (c.rsig != NONE && committed.count(c.rsig)) rs = stage(rs);
-    T.commit.insert(T.commit.end(), {T.persist.at(c.reg), vs, rs == NONE ? 0u : rs, c.pol | (c.rbit << 1) | (c.has_reset << 2)});
+    // flags: reset polarity (with the stored polarity of the reset signal
+    // folded in) | reset value << 1 | has reset << 2 | stored value inverted << 3
+    T.commit.insert(T.commit.end(), {T.persist.at(c.reg), vs, rs == NONE ? 0u : rs,
+                                     (c.pol ^ rpol) | (c.rbit << 1) | (c.has_reset << 2) | (vpol << 3)});
   }
   T.sink = (T.n_slots + RES - 1) & ~(RES - 1);
   if (T.sink + SINK_SLOTS > 65535) throw Error("E_HIP_TOO_LARGE: gate program needs more than 65535 live slots");
@@ -533,6 +582,7 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
   const uint32_t KS = T.n_cycle_stages + T.n_post_stages;
   const size_t row_words = size_t(M) * TILE_THREADS * 2;
   T.desc.assign(size_t(KS) * row_words, NONE);
+  T.stage_fix.assign(KS, 0);
   auto entry = [&](uint32_t kst, uint32_t k, uint32_t lane) -> uint32_t* { return &T.desc[kst * row_words + (size_t(k) * TILE_THREADS + lane) * 2]; };
   std::vector<uint32_t> kstage_mask(size_t(KS) * M * HALVES, 0);
   for (uint32_t ii = 0; ii < insts.size(); ii++) {
@@ -541,15 +591,17 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
     const uint32_t kst = in.post ? T.n_cycle_stages + (in.pos - POS_POST0 - 1) : in.pos - 1;
     uint32_t a = in.chained && !in.chain_b ? NONE : operand_slot(in, 0);
     uint32_t b = in.chained && in.chain_b ? NONE : operand_slot(in, 1);
-    uint32_t op = g.op;
+    uint32_t op = eff_op[ii];
     if (in.chained && in.chain_b) {  // the chained operand goes to a
       std::swap(a, b);
       op = (op & ~(GI_A | GI_B)) | ((op & GI_A) ? GI_B : 0u) | ((op & GI_B) ? GI_A : 0u);
     }
+    if (in.depth > 0 && !in.chained) throw Error("E_HIP_INTERNAL: unchained gate after position 0");
     const uint32_t d = in.slot == NONE - 1 ? NONE : in.slot;
     uint32_t* e = entry(kst, in.depth, lane_of[ii]);
     e[0] = (a == NONE ? 0xFFFFu : a) | ((b == NONE ? 0xFFFFu : b) << 16);
-    e[1] = (d == NONE ? 0xFFFFu : d) | (op << 16) | (in.chained ? DESC_CHAINED : 0u);
+    e[1] = (d == NONE ? 0xFFFFu : d) | (op << 16);
+    if (op & GI_O) T.stage_fix[kst] |= 1u << in.depth;
     kstage_mask[(size_t(kst) * M + in.depth) * HALVES + lane_of[ii] / RES] =
         write_mask[mask_base[in.pos] + in.depth * HALVES + lane_of[ii] / RES];
     if (g.dst < nsig && forced[g.dst] && d != NONE) T.clamp_sites.push_back({g.dst, {kst + 1, d}});
@@ -606,7 +658,7 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
     T.stim_stream.push_back(spec.in_sig[i] << 8);
     T.stim_arg.push_back(spec.in_arg.empty() ? 0 : spec.in_arg[i]);
   }
-  for (uint32_t s : spec.obs_sig) T.obs_slot.push_back(slot_of(s, spec.observe_post));
+  for (uint32_t s : spec.obs_sig) T.obs_slot.push_back(slot_of(s, spec.observe_post) | (value_pol(s, spec.observe_post) << 31));
   return T;
 }
 
@@ -987,14 +1039,18 @@ class GateEngine final : public Engine {
     for (size_t i = 0; i < desc.size(); i++) {
       if (i < size_t(KS) * stride) {
         const uint32_t x = T.desc[2 * i], y = T.desc[2 * i + 1];
-        uint32_t op = (y >> 16) & 0xFu;
-        if (op & GI_X) op &= ~(GI_A | GI_B);  // xor ignores input inversions
-        desc[i] = make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16) | (y & DESC_CHAINED));
+        const uint32_t op = (y >> 16) & 0xFu;
+        desc[i] = make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16));
       } else {
         desc[i] = make_uint2(enc(T.c0) | (enc(T.c0) << 16), enc(sink + uint32_t(i % 32)));
       }
     }
     A.desc = upload_prog(0, desc);
+    std::vector<uint32_t> fix(T.stage_fix);
+    fix.resize(KS + STAGE_AHEAD, 0u);
+    A.stage_fix = upload_prog(1, fix);
+    A.any_fix = 0;
+    for (uint32_t f : T.stage_fix) A.any_fix |= f;
     A.n_cycle_stages = T.n_cycle_stages; A.n_post_stages = T.n_post_stages;
     A.load_slot = upload_prog(2, T.load_slot); A.load_node = upload_prog(3, T.load_node); A.n_load = uint32_t(T.load_slot.size());
     A.stim_slot = upload_prog(4, T.stim_slot); A.stim_mode = upload_prog(5, T.stim_mode);
