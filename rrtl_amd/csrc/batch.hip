@@ -63,10 +63,12 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs B) {
                               [&](auto w, auto ba, auto m) {
                                 gate::tile_body<decltype(w)::value, decltype(ba)::value, decltype(m)::value>(J.u.g, wi.y, lds);
                               });
-    } else if (J.variant) {
-      word::word_body<true>(J.u.w, wi.y, lds);
+    } else if (J.variant == word::WM_ALL_LDS) {
+      word::word_body<word::WM_ALL_LDS>(J.u.w, wi.y);
+    } else if (J.variant == word::WM_SCRATCH_LDS) {
+      word::word_body<word::WM_SCRATCH_LDS>(J.u.w, wi.y);
     } else {
-      word::word_body<false>(J.u.w, wi.y, lds);
+      word::word_body<word::WM_HBM>(J.u.w, wi.y);
     }
     __syncthreads();  // LDS is reused by the next item
   }

@@ -31,229 +31,377 @@ struct WordArgs {
   const uint32_t* obs_desc; uint32_t n_obs, n_obs_limbs;  // per observed: (offset, limbs)
   uint64_t* digest; uint32_t* trace;                      // [lanes] / [cycles][n_obs_limbs][lanes]
   uint32_t halt_offset; uint64_t* halt_cycles;            // NONE = no halting
+  uint32_t n_scratch, n_sig_words;                        // scratch slots; signal words (LDS signal mode)
 };
 
+// Where a block keeps its per-lane scratch and signal values during a launch:
+// WM_HBM both in HBM; WM_SCRATCH_LDS scratch in LDS; WM_ALL_LDS scratch and the
+// block's signal values in LDS (loaded at the start of the launch, written back
+// at the end; memories stay in HBM).  LDS columns are per thread, rows per
+// word, so a block access is conflict-free.
+enum : int { WM_HBM = 0, WM_SCRATCH_LDS = 1, WM_ALL_LDS = 2 };
+
+// A read of launch-constant data (records, constant pool, memory table)
+// through the constant address space, so uniform reads become scalar loads.
+typedef const __attribute__((address_space(4))) uint32_t const_u32;
+__device__ __forceinline__ uint32_t cload(const uint32_t* p, uint32_t i) {
+  return ((const_u32*)p)[i];  // address-space cast
+}
+
+__device__ __forceinline__ uint32_t* lds_words() {
+  extern __shared__ __align__(16) uint32_t word_lds[];
+  return word_lds;
+}
+
+// Per-lane view of the run: operand spaces and this lane's index.  Held by
+// value (never a pointer to the kernel arguments, which would force them into
+// private memory); LDS areas are addressed from the block's LDS base so every
+// access compiles to an LDS instruction.
+template <int kMode>
 struct Ctx {
-  const WordArgs* A;
-  uint32_t lane;
-  uint32_t* scr; uint32_t sstride;
+  uint32_t* values; uint32_t* mems; const uint32_t* cpool; const uint32_t* mem_table;
+  uint32_t lanes, lane;
+  uint32_t* scr; uint32_t sstride;   // HBM scratch (WM_HBM)
+  uint32_t col, stride, sig0;        // LDS: thread column, block stride, first signal row offset
   __device__ __forceinline__ uint32_t ld(uint32_t code, uint32_t limb) const {
     const uint32_t idx = code & ~SP_MASK;
     switch (code & SP_MASK) {
-      case SP_SCRATCH: return scr[size_t(idx + limb) * sstride];
-      case SP_SIGNAL: return A->values[size_t(idx + limb) * A->lanes + lane];
-      default: return A->cpool[idx + limb];
+      case SP_SCRATCH:
+        if constexpr (kMode >= WM_SCRATCH_LDS) return lds_words()[(idx + limb) * stride + col];
+        else return scr[size_t(idx + limb) * sstride];
+      case SP_SIGNAL:
+        if constexpr (kMode == WM_ALL_LDS) return lds_words()[sig0 + (idx + limb) * stride + col];
+        else return values[size_t(idx + limb) * lanes + lane];
+      default: return cload(cpool, idx + limb);
     }
   }
-  // operand of `have` limbs zero-extended to `n`
-  __device__ __forceinline__ void ldv(uint32_t code, uint32_t have, uint32_t n, uint32_t* v) const {
-    for (uint32_t l = 0; l < n; l++) v[l] = l < have ? ld(code, l) : 0u;
+  __device__ __forceinline__ void st(uint32_t slot, uint32_t limb, uint32_t v) const {
+    if constexpr (kMode >= WM_SCRATCH_LDS) lds_words()[(slot + limb) * stride + col] = v;
+    else scr[size_t(slot + limb) * sstride] = v;
   }
-  __device__ __forceinline__ void st(uint32_t slot, uint32_t limb, uint32_t v) const { scr[size_t(slot + limb) * sstride] = v; }
-  __device__ __forceinline__ void stsig(uint32_t off, uint32_t limb, uint32_t v) const { A->values[size_t(off + limb) * A->lanes + lane] = v; }
+  __device__ __forceinline__ void stsig(uint32_t off, uint32_t limb, uint32_t v) const {
+    if constexpr (kMode == WM_ALL_LDS) lds_words()[sig0 + (off + limb) * stride + col] = v;
+    else values[size_t(off + limb) * lanes + lane] = v;
+  }
+  __device__ __forceinline__ uint32_t& mem(uint32_t base, uint64_t word) const { return mems[(size_t(base) + word) * lanes + lane]; }
 };
 
 __device__ __forceinline__ uint32_t fmask(uint32_t w) { uint32_t r = w & 31; return r ? ((1u << r) - 1) : 0xFFFFFFFFu; }
-__device__ __forceinline__ uint32_t get_bit(const uint32_t* v, uint32_t n, uint32_t bit) { uint32_t l = bit >> 5; return l < n ? (v[l] >> (bit & 31)) & 1u : 0u; }
-__device__ __forceinline__ uint64_t decode_usize(const uint32_t* v, uint32_t n) {
+
+// Multi-limb values live in fixed MAXL-limb register arrays: every loop runs
+// to the constant MAXL with a guard and every variable index goes through
+// pick(), so the arrays never need private memory.
+using Limbs = uint32_t[MAXL];
+__device__ __forceinline__ uint32_t pick(const Limbs& v, uint32_t i) {
+  uint32_t x = 0;
+#pragma unroll
+  for (uint32_t m = 0; m < MAXL; m++) x = m == i ? v[m] : x;
+  return x;
+}
+__device__ __forceinline__ uint32_t get_bit(const Limbs& v, uint32_t n, uint32_t bit) {
+  const uint32_t l = bit >> 5;
+  return l < n ? (pick(v, l) >> (bit & 31)) & 1u : 0u;
+}
+// operand of `have` limbs zero-extended to MAXL
+template <class Cx>
+__device__ __forceinline__ void ldv(const Cx& C, uint32_t code, uint32_t have, Limbs& v) {
+#pragma unroll
+  for (uint32_t l = 0; l < MAXL; l++) v[l] = l < have ? C.ld(code, l) : 0u;
+}
+__device__ __forceinline__ uint64_t decode_usize(const Limbs& v, uint32_t n) {
   uint64_t out = v[0] | (n > 1 ? uint64_t(v[1]) << 32 : 0);
-  for (uint32_t l = 2; l < n; l++) if (v[l]) return ~0ull;
+#pragma unroll
+  for (uint32_t l = 2; l < MAXL; l++) if (l < n && v[l]) out = ~0ull;
+  return out;
+}
+// address operand of `n` limbs as usize: a high limb set saturates
+template <class Cx>
+__device__ __forceinline__ uint64_t ld_usize(const Cx& C, uint32_t code, uint32_t n) {
+  uint64_t out = C.ld(code, 0) | (n > 1 ? uint64_t(C.ld(code, 1)) << 32 : 0);
+  for (uint32_t l = 2; l < n; l++) if (C.ld(code, l)) return ~0ull;
   return out;
 }
 
-__device__ void exec_rec(const Ctx& C, const Rec& R) {
-  const uint32_t op = R.w[0] & 0xFF, L = (R.w[0] >> 8) & 0xFF, width = R.w[5];
-  uint32_t a[MAXL], b[MAXL], r[MAXL];
-  if (L == 1 && op <= OP_MEMREAD && op != OP_CONCAT && op != OP_MEMREAD && op != OP_SLICE && op != OP_EQ && op != OP_NE &&
-      op != OP_LT && op != OP_SEXT) {
-    // one-limb fast path
-    const uint32_t x = C.ld(R.w[2], 0);
-    uint32_t v;
-    switch (op) {
-      case OP_NOT: v = ~x; break;
-      case OP_AND: v = x & C.ld(R.w[3], 0); break;
-      case OP_OR: v = x | C.ld(R.w[3], 0); break;
-      case OP_XOR: v = x ^ C.ld(R.w[3], 0); break;
-      case OP_ADD: v = x + C.ld(R.w[3], 0); break;
-      case OP_SUB: v = x - C.ld(R.w[3], 0); break;
-      case OP_MUL: v = x * C.ld(R.w[3], 0); break;
-      case OP_MUX: v = (C.ld(R.w[4], 0) & 1u) ? x : C.ld(R.w[3], 0); break;
-      default: v = x; break;  // ZEXT / TRUNC / CAST
-    }
-    C.st(R.w[1], 0, v & fmask(width));
-    return;
-  }
+// One-limb results (every value of at most 32 bits): straight scalar code.
+template <class Cx>
+__device__ __forceinline__ bool exec_rec1(const Cx& C, const Rec& R) {
+  const uint32_t op = R.w[0] & 0xFF, width = R.w[5];
+  uint32_t v;
   switch (op) {
-    case R_SIGLOAD: {
-      C.ldv(R.w[2], R.w[7], L, r);
+    case OP_NOT: v = ~C.ld(R.w[2], 0); break;
+    case OP_AND: v = C.ld(R.w[2], 0) & C.ld(R.w[3], 0); break;
+    case OP_OR: v = C.ld(R.w[2], 0) | C.ld(R.w[3], 0); break;
+    case OP_XOR: v = C.ld(R.w[2], 0) ^ C.ld(R.w[3], 0); break;
+    case OP_ADD: v = C.ld(R.w[2], 0) + C.ld(R.w[3], 0); break;
+    case OP_SUB: v = C.ld(R.w[2], 0) - C.ld(R.w[3], 0); break;
+    case OP_MUL: v = C.ld(R.w[2], 0) * C.ld(R.w[3], 0); break;
+    case OP_MUX: v = (C.ld(R.w[4], 0) & 1u) ? C.ld(R.w[2], 0) : C.ld(R.w[3], 0); break;
+    case OP_ZEXT: case OP_TRUNC: case OP_CAST: case R_SIGLOAD: v = C.ld(R.w[2], 0); break;
+    case OP_EQ: case OP_NE: {
+      const uint32_t la = R.w[7] & 0xFF, lb = (R.w[7] >> 8) & 0xFF, n = la > lb ? la : lb;
+      uint32_t eq = 1;
+      for (uint32_t l = 0; l < n; l++) eq &= (l < la ? C.ld(R.w[2], l) : 0u) == (l < lb ? C.ld(R.w[3], l) : 0u);
+      v = op == OP_EQ ? eq : (eq ^ 1u);
       break;
     }
-    case OP_NOT: C.ldv(R.w[2], R.w[7], L, a); for (uint32_t l = 0; l < L; l++) r[l] = ~a[l]; break;
+    case OP_LT: {
+      const uint32_t wl = R.w[6];
+      if (wl > 32) return false;
+      const uint32_t a = C.ld(R.w[2], 0) & fmask(wl), b = C.ld(R.w[3], 0) & fmask(wl);
+      if (R.w[7] >> 24) {
+        const uint32_t sa = (a >> (wl - 1)) & 1u, sb = (b >> (wl - 1)) & 1u;
+        v = sa != sb ? sa : uint32_t(a < b);
+      } else {
+        v = a < b;
+      }
+      break;
+    }
+    case OP_SLICE: {
+      const uint32_t n = R.w[7], lsb = R.w[6], li = lsb >> 5, sh = lsb & 31;
+      const uint32_t lo = li < n ? C.ld(R.w[2], li) : 0u, hi = (sh && li + 1 < n) ? C.ld(R.w[2], li + 1) : 0u;
+      v = sh ? ((lo >> sh) | (hi << (32 - sh))) : lo;
+      break;
+    }
+    case OP_SEXT: {
+      const uint32_t iw = R.w[6], x = C.ld(R.w[2], 0);
+      const uint32_t keep = iw >= 32 ? ~0u : ((1u << iw) - 1), sign = (x >> (iw - 1)) & 1u;
+      v = (x & keep) | (sign ? ~keep : 0u);
+      break;
+    }
+    case OP_CONCAT: {
+      v = 0;
+      uint32_t off = 0;
+      for (int k = int(R.w[3]) - 1; k >= 0; k--) {
+        const uint32_t code = cload(C.cpool, R.w[2] + 2 * k), wd = cload(C.cpool, R.w[2] + 2 * k + 1) & 0xFFFF;
+        v |= (C.ld(code, 0) & fmask(wd)) << off;
+        off += wd;
+      }
+      break;
+    }
+    case OP_MEMREAD: {
+      const uint32_t* mt = &C.mem_table[4 * R.w[6]];
+      const uint64_t addr = ld_usize(C, R.w[2], R.w[7]);
+      v = addr < mt[1] ? C.mem(mt[0], addr * mt[2]) & (mt[2] == 1 ? fmask(mt[3]) : ~0u) : 0u;
+      break;
+    }
+    default: return false;
+  }
+  C.st(R.w[1], 0, v & fmask(width));
+  return true;
+}
+
+template <class Cx>
+__device__ __forceinline__ void exec_rec(const Cx& C, const Rec& R) {
+  const uint32_t op = R.w[0] & 0xFF, L = (R.w[0] >> 8) & 0xFF, width = R.w[5];
+  if (L == 1 && exec_rec1(C, R)) return;
+  Limbs a, b, r;
+#pragma unroll
+  for (uint32_t l = 0; l < MAXL; l++) r[l] = 0;
+  switch (op) {
+    case R_SIGLOAD: ldv(C, R.w[2], R.w[7], r); break;
+    case OP_NOT: {
+      ldv(C, R.w[2], R.w[7], a);
+#pragma unroll
+      for (uint32_t l = 0; l < MAXL; l++) r[l] = ~a[l];
+      break;
+    }
     case OP_AND: case OP_OR: case OP_XOR: case OP_ADD: case OP_SUB: case OP_MUL: {
-      C.ldv(R.w[2], R.w[7] & 0xFF, L, a);
-      C.ldv(R.w[3], (R.w[7] >> 8) & 0xFF, L, b);
-      if (op == OP_AND) for (uint32_t l = 0; l < L; l++) r[l] = a[l] & b[l];
-      else if (op == OP_OR) for (uint32_t l = 0; l < L; l++) r[l] = a[l] | b[l];
-      else if (op == OP_XOR) for (uint32_t l = 0; l < L; l++) r[l] = a[l] ^ b[l];
-      else if (op == OP_ADD) { uint64_t c = 0; for (uint32_t l = 0; l < L; l++) { uint64_t t = uint64_t(a[l]) + b[l] + c; r[l] = uint32_t(t); c = t >> 32; } }
-      else if (op == OP_SUB) { uint64_t br = 0; for (uint32_t l = 0; l < L; l++) { uint64_t s = uint64_t(b[l]) + br, x = a[l]; r[l] = uint32_t(x - s); br = x < s; } }
-      else {
-        for (uint32_t l = 0; l < L; l++) r[l] = 0;
-        for (uint32_t j = 0; j < L; j++) {
+      ldv(C, R.w[2], R.w[7] & 0xFF, a);
+      ldv(C, R.w[3], (R.w[7] >> 8) & 0xFF, b);
+      if (op == OP_AND) {
+#pragma unroll
+        for (uint32_t l = 0; l < MAXL; l++) r[l] = a[l] & b[l];
+      } else if (op == OP_OR) {
+#pragma unroll
+        for (uint32_t l = 0; l < MAXL; l++) r[l] = a[l] | b[l];
+      } else if (op == OP_XOR) {
+#pragma unroll
+        for (uint32_t l = 0; l < MAXL; l++) r[l] = a[l] ^ b[l];
+      } else if (op == OP_ADD) {
+        uint64_t c = 0;
+#pragma unroll
+        for (uint32_t l = 0; l < MAXL; l++) { const uint64_t t = uint64_t(a[l]) + b[l] + c; r[l] = uint32_t(t); c = t >> 32; }
+      } else if (op == OP_SUB) {
+        uint64_t br = 0;
+#pragma unroll
+        for (uint32_t l = 0; l < MAXL; l++) { const uint64_t s = uint64_t(b[l]) + br, x = a[l]; r[l] = uint32_t(x - s); br = x < s; }
+      } else {
+#pragma unroll
+        for (uint32_t j = 0; j < MAXL; j++) {
           uint64_t c = 0;
-          for (uint32_t k = 0; j + k < L; k++) { uint64_t t = uint64_t(a[j]) * b[k] + r[j + k] + c; r[j + k] = uint32_t(t); c = t >> 32; }
+#pragma unroll
+          for (uint32_t k = 0; j + k < MAXL; k++) {
+            const uint64_t t = uint64_t(a[j]) * b[k] + r[j + k] + c;
+            r[j + k] = uint32_t(t); c = t >> 32;
+          }
         }
       }
       break;
     }
     case OP_EQ: case OP_NE: {
-      uint32_t la = R.w[7] & 0xFF, lb = (R.w[7] >> 8) & 0xFF, n = la > lb ? la : lb;
+      const uint32_t la = R.w[7] & 0xFF, lb = (R.w[7] >> 8) & 0xFF, n = la > lb ? la : lb;
       uint32_t eq = 1;
       for (uint32_t l = 0; l < n; l++) eq &= (l < la ? C.ld(R.w[2], l) : 0u) == (l < lb ? C.ld(R.w[3], l) : 0u);
       r[0] = op == OP_EQ ? eq : (eq ^ 1u);
-      for (uint32_t l = 1; l < L; l++) r[l] = 0;
       break;
     }
     case OP_LT: {
       const uint32_t wl = R.w[6], n = (wl + 31) / 32;
-      C.ldv(R.w[2], R.w[7] & 0xFF, n, a);
-      C.ldv(R.w[3], (R.w[7] >> 8) & 0xFF, n, b);
-      a[n - 1] &= fmask(wl); b[n - 1] &= fmask(wl);
+      ldv(C, R.w[2], min(R.w[7] & 0xFF, n), a);
+      ldv(C, R.w[3], min((R.w[7] >> 8) & 0xFF, n), b);
+#pragma unroll
+      for (uint32_t l = 0; l < MAXL; l++)
+        if (l + 1 == n) { a[l] &= fmask(wl); b[l] &= fmask(wl); }
       uint32_t res = 0, decided = 0;
       if (R.w[7] >> 24) {
-        uint32_t sa = get_bit(a, n, wl - 1), sb = get_bit(b, n, wl - 1);
+        const uint32_t sa = get_bit(a, n, wl - 1), sb = get_bit(b, n, wl - 1);
         if (sa != sb) { res = sa; decided = 1; }
       }
-      for (int l = int(n) - 1; l >= 0 && !decided; l--) if (a[l] != b[l]) { res = a[l] < b[l]; decided = 1; }
+#pragma unroll
+      for (int l = int(MAXL) - 1; l >= 0; l--)
+        if (uint32_t(l) < n && !decided && a[l] != b[l]) { res = a[l] < b[l]; decided = 1; }
       r[0] = res;
-      for (uint32_t l = 1; l < L; l++) r[l] = 0;
       break;
     }
     case OP_MUX: {
       const bool c = C.ld(R.w[4], 0) & 1u;
-      C.ldv(c ? R.w[2] : R.w[3], c ? (R.w[7] & 0xFF) : ((R.w[7] >> 8) & 0xFF), L, r);
+      ldv(C, c ? R.w[2] : R.w[3], c ? (R.w[7] & 0xFF) : ((R.w[7] >> 8) & 0xFF), r);
       break;
     }
     case OP_SLICE: {
       const uint32_t n = R.w[7], lsb = R.w[6];
-      C.ldv(R.w[2], n, n, a);
-      for (uint32_t l = 0; l < L; l++) {
+      ldv(C, R.w[2], n, a);
+#pragma unroll
+      for (uint32_t l = 0; l < MAXL; l++) {
         const uint32_t bit = lsb + 32 * l, li = bit >> 5, sh = bit & 31;
-        uint32_t lo = li < n ? a[li] : 0u, hi = li + 1 < n ? a[li + 1] : 0u;
+        const uint32_t lo = li < n ? pick(a, li) : 0u, hi = li + 1 < n ? pick(a, li + 1) : 0u;
         r[l] = sh ? ((lo >> sh) | (hi << (32 - sh))) : lo;
       }
       break;
     }
-    case OP_ZEXT: case OP_TRUNC: case OP_CAST: C.ldv(R.w[2], R.w[7], L, r); break;
+    case OP_ZEXT: case OP_TRUNC: case OP_CAST: ldv(C, R.w[2], R.w[7], r); break;
     case OP_SEXT: {
-      const uint32_t iw = R.w[6], n = R.w[7];
-      C.ldv(R.w[2], n, L, r);
-      const uint32_t sign = get_bit(r, L, iw - 1);
-      // clear above iw, then fill with the sign
-      for (uint32_t l = 0; l < L; l++) {
-        uint32_t lo_bit = 32 * l;
+      const uint32_t iw = R.w[6];
+      ldv(C, R.w[2], R.w[7], r);
+      const uint32_t sign = get_bit(r, MAXL, iw - 1);
+#pragma unroll
+      for (uint32_t l = 0; l < MAXL; l++) {
+        const uint32_t lo_bit = 32 * l;
         if (lo_bit >= iw) r[l] = sign ? ~0u : 0u;
         else if (lo_bit + 32 > iw) {
-          uint32_t keep = (1u << (iw - lo_bit)) - 1;
+          const uint32_t keep = (1u << (iw - lo_bit)) - 1;
           r[l] = (r[l] & keep) | (sign ? ~keep : 0u);
         }
       }
       break;
     }
     case OP_CONCAT: {
-      for (uint32_t l = 0; l < L; l++) r[l] = 0;
       uint32_t off = 0;
       for (int k = int(R.w[3]) - 1; k >= 0; k--) {
-        const uint32_t code = C.A->cpool[R.w[2] + 2 * k], wd = C.A->cpool[R.w[2] + 2 * k + 1] & 0xFFFF,
-                       n = C.A->cpool[R.w[2] + 2 * k + 1] >> 16;
+        const uint32_t code = cload(C.cpool, R.w[2] + 2 * k), wd = cload(C.cpool, R.w[2] + 2 * k + 1) & 0xFFFF,
+                       n = cload(C.cpool, R.w[2] + 2 * k + 1) >> 16;
         for (uint32_t l = 0; l < n; l++) {
           uint32_t v = C.ld(code, l);
           if (l == n - 1) v &= fmask(wd);
           const uint32_t bit = off + 32 * l, li = bit >> 5, sh = bit & 31;
-          if (li < L) r[li] |= v << sh;
-          if (sh && li + 1 < L) r[li + 1] |= v >> (32 - sh);
+#pragma unroll
+          for (uint32_t j = 0; j < MAXL; j++) {
+            if (j == li) r[j] |= v << sh;
+            if (sh && j == li + 1) r[j] |= v >> (32 - sh);
+          }
         }
         off += wd;
       }
       break;
     }
     case OP_MEMREAD: {
-      const uint32_t* mt = &C.A->mem_table[4 * R.w[6]];
-      C.ldv(R.w[2], R.w[7], R.w[7], a);
+      const uint32_t* mt = &C.mem_table[4 * R.w[6]];
+      ldv(C, R.w[2], R.w[7], a);
       const uint64_t addr = decode_usize(a, R.w[7]);
-      for (uint32_t l = 0; l < L; l++) r[l] = 0;
       if (addr < mt[1]) {
-        for (uint32_t l = 0; l < mt[2] && l < L; l++)
-          r[l] = C.A->mems[(size_t(mt[0]) + addr * mt[2] + l) * C.A->lanes + C.lane];
-        if (mt[2] <= L) r[mt[2] - 1] &= fmask(mt[3]);
+#pragma unroll
+        for (uint32_t l = 0; l < MAXL; l++)
+          if (l < mt[2] && l < L) r[l] = C.mem(mt[0], addr * mt[2] + l) & (l + 1 == mt[2] ? fmask(mt[3]) : ~0u);
       }
       break;
     }
   }
-  r[L - 1] &= fmask(width);
-  for (uint32_t l = 0; l < L; l++) C.st(R.w[1], l, r[l]);
+#pragma unroll
+  for (uint32_t l = 0; l < MAXL; l++)
+    if (l < L) C.st(R.w[1], l, l + 1 == L ? r[l] & fmask(width) : r[l]);
 }
 
-__device__ __forceinline__ bool reset_asserted(const Ctx& C, uint32_t rsig_code, uint32_t pol) {
+template <class Cx>
+__device__ __forceinline__ bool reset_asserted(const Cx& C, uint32_t rsig_code, uint32_t pol) {
   const uint32_t v = C.ld(rsig_code, 0) & 1u;
   return pol == 0 ? v != 0 : v == 0;
 }
 
-__device__ void exec_effect(const Ctx& C, const Rec& R) {
+template <class Cx>
+__device__ __forceinline__ void exec_effect(const Cx& C, const Rec& R) {
   const uint32_t op = R.w[0] & 0xFF, L = (R.w[0] >> 8) & 0xFF;
-  uint32_t v[MAXL];
   switch (op) {
     case R_STORE:
-      C.ldv(R.w[2], R.w[7], L, v);
-      v[L - 1] &= fmask(R.w[5]);
-      for (uint32_t l = 0; l < L; l++) C.stsig(R.w[1], l, v[l]);
-      break;
-    case R_ASYNC_CAPTURE:
-      if (reset_asserted(C, R.w[3], R.w[6])) {
-        C.ldv(R.w[2], R.w[7], L, v);
-        v[L - 1] &= fmask(R.w[5]);
-        for (uint32_t l = 0; l < L; l++) C.stsig(R.w[1], l, v[l]);
+      for (uint32_t l = 0; l < L; l++) {
+        const uint32_t v = l < R.w[7] ? C.ld(R.w[2], l) : 0u;
+        C.stsig(R.w[1], l, l + 1 == L ? v & fmask(R.w[5]) : v);
       }
       break;
-    case R_CAPTURE:
-      if ((R.w[6] & 2u) && reset_asserted(C, R.w[3], R.w[6] & 1u)) for (uint32_t l = 0; l < L; l++) v[l] = C.ld(R.w[4], l);
-      else C.ldv(R.w[2], R.w[7], L, v);
-      v[L - 1] &= fmask(R.w[5]);
-      for (uint32_t l = 0; l < L; l++) C.st(R.w[1], l, v[l]);
+    case R_ASYNC_CAPTURE:
+      if (reset_asserted(C, R.w[3], R.w[6]))
+        for (uint32_t l = 0; l < L; l++) {
+          const uint32_t v = l < R.w[7] ? C.ld(R.w[2], l) : 0u;
+          C.stsig(R.w[1], l, l + 1 == L ? v & fmask(R.w[5]) : v);
+        }
       break;
+    case R_CAPTURE: {
+      const bool rst = (R.w[6] & 2u) && reset_asserted(C, R.w[3], R.w[6] & 1u);
+      for (uint32_t l = 0; l < L; l++) {
+        const uint32_t v = rst ? C.ld(R.w[4], l) : (l < R.w[7] ? C.ld(R.w[2], l) : 0u);
+        C.st(R.w[1], l, l + 1 == L ? v & fmask(R.w[5]) : v);
+      }
+      break;
+    }
     case R_COMMIT:
       for (uint32_t l = 0; l < L; l++) C.stsig(R.w[1], l, C.ld(R.w[2], l));
       break;
     case R_MEMWRITE: {
       if (!(C.ld(R.w[2], 0) & 1u)) break;
-      const uint32_t* mt = &C.A->mem_table[4 * R.w[1]];
-      uint32_t la = R.w[7] & 0xFF, ld = (R.w[7] >> 8) & 0xFF;
-      C.ldv(R.w[3], la, la, v);
-      const uint64_t addr = decode_usize(v, la);
+      const uint32_t* mt = &C.mem_table[4 * R.w[1]];
+      const uint32_t la = R.w[7] & 0xFF, ld = (R.w[7] >> 8) & 0xFF;
+      const uint64_t addr = ld_usize(C, R.w[3], la);
       if (addr >= mt[1]) break;
-      C.ldv(R.w[4], ld, L, v);
-      v[L - 1] &= fmask(mt[3]);
-      for (uint32_t l = 0; l < L; l++) C.A->mems[(size_t(mt[0]) + addr * mt[2] + l) * C.A->lanes + C.lane] = v[l];
+      for (uint32_t l = 0; l < L; l++) {
+        const uint32_t v = l < ld ? C.ld(R.w[4], l) : 0u;
+        C.mem(mt[0], addr * mt[2] + l) = l + 1 == L ? v & fmask(mt[3]) : v;
+      }
       break;
     }
   }
 }
 
-__device__ void exec_block(const Ctx& C, int b) {
-  const Rec* recs = C.A->blk[b];
-  const uint32_t n = C.A->nrec[b];
+// The interpreter loop over one block's records.  Every lane runs the same
+// record, and records never change during a launch, so they are read through
+// the constant address space: scalar loads and scalar branches on the opcode.
+__device__ __forceinline__ Rec load_rec(const Rec* recs, uint32_t i) {
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(recs + i);
+  Rec R;
+#pragma unroll
+  for (int k = 0; k < 8; k++) R.w[k] = cload(p, k);
+  return R;
+}
+
+template <class Cx>
+__device__ __forceinline__ void run_records(const Rec* recs, uint32_t n, const Cx& C) {
   for (uint32_t i = 0; i < n; i++) {
-    const Rec R = recs[i];  // uniform: every lane runs the same record
+    const Rec R = load_rec(recs, i);
     if ((R.w[0] & 0xFF) >= R_STORE && (R.w[0] & 0xFF) != R_SIGLOAD) exec_effect(C, R);
     else exec_rec(C, R);
   }
 }
 
-__device__ __forceinline__ void eval_comb(const Ctx& C) { exec_block(C, B_COMB); exec_block(C, B_ASYNC); }
-__device__ __forceinline__ void edge(const Ctx& C) { exec_block(C, B_NEXT); exec_block(C, B_COMMIT); }
-
-__device__ void apply_stimulus(const Ctx& C, uint64_t c, uint64_t cyc) {
-  const WordArgs& A = *C.A;
+template <class Cx>
+__device__ __forceinline__ void apply_stimulus(const WordArgs& A, const Cx& C, uint64_t c, uint64_t cyc) {
   uint32_t trace_limb = 0;
   for (uint32_t i = 0; i < A.n_in; i++) {
     const uint32_t off = A.in_desc[4 * i], limbs = A.in_desc[4 * i + 1], width = A.in_desc[4 * i + 2], mode = A.in_desc[4 * i + 3];
@@ -284,53 +432,83 @@ __device__ void apply_stimulus(const Ctx& C, uint64_t c, uint64_t cyc) {
   }
 }
 
-__device__ void observe(const Ctx& C, uint64_t c, uint64_t& d) {
-  const WordArgs& A = *C.A;
+template <class Cx>
+__device__ __forceinline__ void observe(const WordArgs& A, const Cx& C, uint64_t c, uint64_t& d) {
   uint32_t k = 0;
   for (uint32_t j = 0; j < A.n_obs; j++) {
     const uint32_t off = A.obs_desc[2 * j], limbs = A.obs_desc[2 * j + 1];
     for (uint32_t l = 0; l < limbs; l++, k++) {
-      const uint32_t v = A.values[size_t(off + l) * A.lanes + C.lane];
+      const uint32_t v = C.ld(SP_SIGNAL | off, l);
       if (A.trace) A.trace[(size_t(c) * A.n_obs_limbs + k) * A.lanes + C.lane] = v;
       if (A.digest) d = lane_digest(d, v, j, l);
     }
   }
 }
 
-// One block of lanes; lanes past the batch skip (no barriers inside).
-template <bool kLds>
-__device__ void word_body(const WordArgs& A, uint32_t block, uint32_t* lds) {
+// One block of lanes; lanes past the batch skip (no barriers inside).  The
+// launch is one loop over steps -- the first settle, every cycle, the final
+// settle -- and each step runs a uniform sequence of record blocks and
+// observation points through a single inlined copy of the interpreter.
+constexpr uint32_t STEP_OBS = 15;  // sequence entry: observe (else a block id)
+
+template <int kMode>
+__device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
   const uint32_t lane = block * blockDim.x + threadIdx.x;
   if (lane >= A.lanes) return;
-  Ctx C;
-  C.A = &A; C.lane = lane;
-  if (kLds) { C.scr = lds + threadIdx.x; C.sstride = blockDim.x; }
-  else { C.scr = A.scratch + lane; C.sstride = A.lanes; }
+  Ctx<kMode> C;
+  C.values = A.values; C.mems = A.mems; C.cpool = A.cpool; C.mem_table = A.mem_table;
+  C.lanes = A.lanes; C.lane = lane;
+  C.scr = A.scratch + lane; C.sstride = A.lanes;
+  C.col = threadIdx.x; C.stride = blockDim.x; C.sig0 = A.n_scratch * blockDim.x;
+  if constexpr (kMode == WM_ALL_LDS)
+    for (uint32_t w = 0; w < A.n_sig_words; w++) lds_words()[C.sig0 + w * C.stride + C.col] = A.values[size_t(w) * A.lanes + lane];
   uint64_t d = A.digest ? A.digest[lane] : 0;
   uint64_t done = 0;
   bool halted = false;
   if (A.halt_cycles) { done = A.halt_cycles[lane] >> 1; halted = A.halt_cycles[lane] & 1; }
-  if (A.flags & F_EVAL_FIRST) eval_comb(C);
-  for (uint64_t c = 0; c < A.cycles && !halted; c++) {
-    const uint64_t cyc = A.cycle_base + c;
-    if (A.n_in) apply_stimulus(C, c, cyc);
-    if (A.flags & F_EVAL_PRE) eval_comb(C);
-    if (A.n_obs && !(A.flags & F_OBS_POST)) observe(C, c, d);
-    if (A.flags & F_EDGE) edge(C);
-    if (A.flags & F_EVAL_POST) eval_comb(C);
-    if (A.n_obs && (A.flags & F_OBS_POST)) observe(C, c, d);
-    done++;
-    if (A.halt_offset != NONE && (A.values[size_t(A.halt_offset) * A.lanes + lane] & 1u)) halted = true;
+  // per-cycle sequence, 4 bits per entry
+  uint32_t cyc_seq = 0, cyc_n = 0;
+  auto push = [&](uint32_t x) { cyc_seq |= x << (4 * cyc_n); cyc_n++; };
+  if (A.flags & F_EVAL_PRE) { push(B_COMB); push(B_ASYNC); }
+  if (A.n_obs && !(A.flags & F_OBS_POST)) push(STEP_OBS);
+  if (A.flags & F_EDGE) { push(B_NEXT); push(B_COMMIT); }
+  if (A.flags & F_EVAL_POST) { push(B_COMB); push(B_ASYNC); }
+  if (A.n_obs && (A.flags & F_OBS_POST)) push(STEP_OBS);
+  const uint32_t settle_seq = B_COMB | (B_ASYNC << 4);
+  const uint64_t steps = A.cycles + 2;
+  for (uint64_t s = 0; s < steps; s++) {
+    const bool first = s == 0, last = s == steps - 1;
+    uint32_t seq, n;
+    uint64_t c = 0;
+    if (first) { seq = settle_seq; n = (A.flags & F_EVAL_FIRST) ? 2 : 0; }
+    else if (last) { seq = settle_seq; n = (A.flags & F_FINAL_EVAL) ? 2 : 0; }
+    else {
+      if (halted) continue;
+      c = s - 1;
+      if (A.n_in) apply_stimulus(A, C, c, A.cycle_base + c);
+      seq = cyc_seq; n = cyc_n;
+    }
+    // the sequence is the same on every active lane; make that visible
+    const uint32_t seq_u = __builtin_amdgcn_readfirstlane(seq), n_u = __builtin_amdgcn_readfirstlane(n);
+    for (uint32_t k = 0; k < n_u; k++) {
+      const uint32_t id = (seq_u >> (4 * k)) & 15u;
+      if (id == STEP_OBS) observe(A, C, c, d);
+      else run_records(A.blk[id], A.nrec[id], C);
+    }
+    if (!first && !last) {
+      done++;
+      if (A.halt_offset != NONE && (C.ld(SP_SIGNAL | A.halt_offset, 0) & 1u)) halted = true;
+    }
   }
-  if (A.flags & F_FINAL_EVAL) eval_comb(C);
+  if constexpr (kMode == WM_ALL_LDS)
+    for (uint32_t w = 0; w < A.n_sig_words; w++) A.values[size_t(w) * A.lanes + lane] = lds_words()[C.sig0 + w * C.stride + C.col];
   if (A.digest) A.digest[lane] = d;
   if (A.halt_cycles) A.halt_cycles[lane] = (done << 1) | (halted ? 1u : 0u);
 }
 
-template <bool kLds>
+template <int kMode>
 __global__ void __launch_bounds__(256) word_kernel(WordArgs A) {
-  extern __shared__ uint32_t lds[];
-  word_body<kLds>(A, blockIdx.x, lds);
+  word_body<kMode>(A, blockIdx.x);
 }
 
 }  // namespace word

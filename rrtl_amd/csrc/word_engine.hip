@@ -232,8 +232,17 @@ class WordEngine final : public Engine {
     RRTL_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     values_.alloc(size_t(P.total_signal_words) * lanes); values_.zero(stream_);
     mems_.alloc(size_t(P.total_memory_words) * lanes); mems_.zero(stream_);
-    block_ = opt.block_threads ? opt.block_threads : 256;
-    use_lds_ = size_t(WP.scratch_slots) * block_ * 4 <= 64 * 1024;
+    // One lane per thread: blocks of 256 lanes, smaller (down to one wave)
+    // when that is what it takes to give every CU a block.
+    if (opt.block_threads) {
+      block_ = opt.block_threads;
+    } else {
+      const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
+      const uint32_t per_cu = (lanes + cus - 1) / cus;
+      block_ = std::min<uint32_t>(256, std::max<uint32_t>(64, (per_cu + 63) / 64 * 64));
+    }
+    mode_ = lds_mode(block_);
+    use_lds_ = mode_ != WM_HBM;
     if (!use_lds_) { scratch_.alloc(size_t(WP.scratch_slots) * lanes); scratch_.zero(stream_); }
     for (int b = 0; b < 4; b++) {
       std::vector<uint32_t> flat;
@@ -308,15 +317,15 @@ class WordEngine final : public Engine {
   void run(const RunSpec& spec) override { launch(run_flags(spec), spec.cycles, &spec, true); }
 
   BatchJob batch_prepare(const RunSpec& spec) override {
-    // batch launches use 256-lane blocks; scratch stays in LDS when it fits
-    const bool lds_ok = size_t(WP.scratch_slots) * 256 * 4 <= 64 * 1024;
-    if (!lds_ok && !scratch_.p) { scratch_.alloc(size_t(WP.scratch_slots) * lanes_); scratch_.zero(stream_); }
+    // batch launches use 256-lane blocks; scratch (and signals) stay in LDS when they fit
+    const int mode = lds_mode(256);
+    if (mode == WM_HBM && !scratch_.p) { scratch_.alloc(size_t(WP.scratch_slots) * lanes_); scratch_.zero(stream_); }
     WordArgs A = make_args(run_flags(spec), spec.cycles, &spec, true);
-    A.scratch = lds_ok ? nullptr : scratch_.p;
+    A.scratch = mode != WM_HBM ? nullptr : scratch_.p;
     BatchJob J;
-    J.kind = ENGINE_WORD; J.variant = lds_ok ? 1 : 0;
+    J.kind = ENGINE_WORD; J.variant = uint32_t(mode);
     J.n_items = (lanes_ + 255) / 256;
-    J.lds_bytes = lds_ok ? size_t(WP.scratch_slots) * 256 * 4 : 0;
+    J.lds_bytes = lds_bytes(mode, 256);
     size_t recs = 0;
     for (auto& b : WP.blk) recs += b.size();
     J.item_cost = double(spec.cycles) * double(recs) * 4.0;
@@ -402,8 +411,22 @@ class WordEngine final : public Engine {
     return reinterpret_cast<T*>(b.p);
   }
 
+  // LDS bytes a block of `block` lanes needs in `mode`.
+  size_t lds_bytes(int mode, uint32_t block) const {
+    if (mode == WM_HBM) return 0;
+    const size_t words = WP.scratch_slots + (mode == WM_ALL_LDS ? P.total_signal_words : 0);
+    return words * block * 4;
+  }
+  // Signals join scratch in LDS when both fit in 64 KiB per block.
+  int lds_mode(uint32_t block) const {
+    if (lds_bytes(WM_ALL_LDS, block) <= 64 * 1024) return WM_ALL_LDS;
+    if (lds_bytes(WM_SCRATCH_LDS, block) <= 64 * 1024) return WM_SCRATCH_LDS;
+    return WM_HBM;
+  }
+
   WordArgs make_args(uint32_t flags, uint64_t cycles, const RunSpec* spec, bool outputs) {
     WordArgs A{};
+    A.n_scratch = WP.scratch_slots; A.n_sig_words = P.total_signal_words;
     A.values = values_.p; A.mems = mems_.p; A.scratch = use_lds_ ? nullptr : scratch_.p;
     for (int b = 0; b < 4; b++) { A.blk[b] = reinterpret_cast<const Rec*>(blk_[b].p); A.nrec[b] = uint32_t(WP.blk[b].size()); }
     A.cpool = cpool_.p; A.mem_table = mtab_.p; A.lanes = lanes_; A.flags = flags; A.cycles = cycles;
@@ -461,7 +484,7 @@ class WordEngine final : public Engine {
     const WordArgs A = make_args(flags, cycles, spec, outputs);
     const uint64_t chunk = (opt_.ticks_per_launch && outputs) ? opt_.ticks_per_launch : std::max<uint64_t>(cycles, 1);
     const uint32_t grid = (lanes_ + block_ - 1) / block_;
-    const size_t lds = use_lds_ ? size_t(WP.scratch_slots) * block_ * 4 : 0;
+    const size_t lds = lds_bytes(mode_, block_);
     stats_ = RunStats{};
     RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
     uint64_t done = 0;
@@ -473,8 +496,9 @@ class WordEngine final : public Engine {
       if (done + B.cycles < cycles) B.flags &= ~uint32_t(F_FINAL_EVAL);
       if (A.trace) B.trace = A.trace + done * A.n_obs_limbs * lanes_;
       if (A.trace_in) B.trace_in = A.trace_in + done * A.trace_in_limbs * lanes_;
-      if (use_lds_) hipLaunchKernelGGL(word_kernel<true>, dim3(grid), dim3(block_), lds, stream_, B);
-      else hipLaunchKernelGGL(word_kernel<false>, dim3(grid), dim3(block_), 0, stream_, B);
+      if (mode_ == WM_ALL_LDS) hipLaunchKernelGGL(word_kernel<WM_ALL_LDS>, dim3(grid), dim3(block_), lds, stream_, B);
+      else if (mode_ == WM_SCRATCH_LDS) hipLaunchKernelGGL(word_kernel<WM_SCRATCH_LDS>, dim3(grid), dim3(block_), lds, stream_, B);
+      else hipLaunchKernelGGL(word_kernel<WM_HBM>, dim3(grid), dim3(block_), 0, stream_, B);
       RRTL_HIP_CHECK(hipGetLastError());
       stats_.launches++;
       done += B.cycles;
@@ -497,6 +521,7 @@ class WordEngine final : public Engine {
   WordProgram WP;
   size_t lds_max_ = 65536;
   uint32_t block_ = 256;
+  int mode_ = WM_SCRATCH_LDS;
   bool use_lds_ = true, has_digest_ = false, has_halt_ = false, batch_spec_halt_ = false, batch_digest_ = false;
   size_t trace_bytes_ = 0;
   DevBuf<uint32_t> values_, mems_, scratch_, cpool_, mtab_, blk_[4], run_bufs_[4], trace_;

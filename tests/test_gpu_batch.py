@@ -32,6 +32,9 @@ def mixed_jobs():
                                                (f.signal_index("out_ready"), IN_RANDOM, 0),
                                                (f.signal_index("in_data"), IN_RANDOM, 0)],
                                        observe=[f.signal_index(n) for n in ("out_valid", "out_data", "count")])))
+    from rrtl_amd import workloads
+    cpu = workloads.rv32i_job(700, 900, 5, n_insns=96, distinct=64)
+    jobs.append((cpu.program, cpu.lanes, "word", dict(cpu.run, observe=[cpu.program.signal_index("pc")]), cpu.setup))
     r = lower.lower_module(random_design(77, async_resets=False))
     jobs.append((r, 300, "word", dict(cycles=150, seed=4,
                                       inputs=[(i, IN_RANDOM, 0) for i, s in enumerate(r.signals) if s.kind == KIND_INPUT],
@@ -39,20 +42,29 @@ def mixed_jobs():
     return jobs
 
 
+def make(job):
+    sim = HipSimulator(job[0], job[1], engine=job[2])
+    if len(job) > 4 and job[4]:
+        job[4](sim)
+    return sim
+
+
 def test_batch_matches_individual_runs():
     flush_print("case: mixed batch persistent vs individual")
     jobs = mixed_jobs()
-    solo = [HipSimulator(p, lanes, engine=eng) for p, lanes, eng, _ in jobs]
-    both = [HipSimulator(p, lanes, engine=eng) for p, lanes, eng, _ in jobs]
-    batch = Batch([(s, kw) for s, (_, _, _, kw) in zip(both, jobs)])
+    solo = [make(j) for j in jobs]
+    both = [make(j) for j in jobs]
+    batch = Batch([(s, j[3]) for s, j in zip(both, jobs)])
     for rep in range(2):  # the second launch replays the captured graph
-        for s, (_, _, _, kw) in zip(solo, jobs):
-            s.run(**kw)
+        for s, j in zip(solo, jobs):
+            s.run(**j[3])
         st = batch.run(persistent=True)
         assert st["launches"] == 1
-        assert st["lane_cycles"] == sum(lanes * kw["cycles"] for _, lanes, _, kw in jobs)
-        for a, b in zip(solo, both):
+        assert st["lane_cycles"] == sum(s.stats()["lane_cycles"] for s in solo)
+        for a, b, j in zip(solo, both, jobs):
             assert np.array_equal(a.digest(), b.digest()), (rep, a.kind)
+            if j[3].get("halt"):
+                assert np.array_equal(a.halt_cycles(), b.halt_cycles()), rep
             va, ma = a.snapshot()
             vb, mb = b.snapshot()
             assert np.array_equal(va, vb) and np.array_equal(ma, mb), (rep, a.kind)
@@ -60,12 +72,12 @@ def test_batch_matches_individual_runs():
 
 def test_batch_per_design_mode_matches():
     jobs = mixed_jobs()[:2]
-    sims = [HipSimulator(p, lanes, engine=eng) for p, lanes, eng, _ in jobs]
-    ref = [HipSimulator(p, lanes, engine=eng) for p, lanes, eng, _ in jobs]
-    batch = Batch([(s, kw) for s, (_, _, _, kw) in zip(sims, jobs)])
+    sims = [make(j) for j in jobs]
+    ref = [make(j) for j in jobs]
+    batch = Batch([(s, j[3]) for s, j in zip(sims, jobs)])
     st = batch.run(persistent=False)
     assert st["launches"] == 2
-    for s, (_, _, _, kw) in zip(ref, jobs):
-        s.run(**kw)
+    for s, j in zip(ref, jobs):
+        s.run(**j[3])
     for a, b in zip(sims, ref):
         assert np.array_equal(a.digest(), b.digest())
