@@ -100,6 +100,9 @@ rrtl_status rrtl_hip_restore(rrtl_hip_engine* e, const uint32_t* values, const u
 /* Stuck-at fault injection, BitParallelSimulator::set_force / clear_forces
  * (crates/rrtl-sim-ir/src/bitparallel.rs:96-101); gate engine only. */
 rrtl_status rrtl_hip_set_force(rrtl_hip_engine* e, uint32_t signal, uint32_t lane, int value);
+/* n set_force calls at once (the per-fault loop of fault_sim.rs:56-58); value[i] != 0 is stuck-at-1. */
+rrtl_status rrtl_hip_set_forces(rrtl_hip_engine* e, const uint32_t* signal, const uint32_t* lane, const uint8_t* value,
+                                size_t n);
 rrtl_status rrtl_hip_clear_forces(rrtl_hip_engine* e);
 
 rrtl_status rrtl_hip_synchronize(rrtl_hip_engine* e);         /* InterpGpuSimulator::synchronize, interp.rs:2044 */
@@ -110,7 +113,8 @@ rrtl_status rrtl_hip_synchronize(rrtl_hip_engine* e);         /* InterpGpuSimula
  * (capture, memory writes, commit, forces), then either a settle + observe
  * (POST) or nothing; the state after the run equals the reference's after its
  * trailing settle. */
-enum { RRTL_IN_HOLD = 0, RRTL_IN_RANDOM = 1, RRTL_IN_CONST = 2, RRTL_IN_PULSE = 3, RRTL_IN_TRACE = 4 };
+enum { RRTL_IN_HOLD = 0, RRTL_IN_RANDOM = 1, RRTL_IN_CONST = 2, RRTL_IN_PULSE = 3, RRTL_IN_TRACE = 4,
+       RRTL_IN_GROUP_RANDOM = 5 };
 enum { RRTL_OBSERVE_PRE = 0, RRTL_OBSERVE_POST = 1 };
 typedef struct rrtl_hip_run_spec {
   uint32_t struct_size;
@@ -126,6 +130,16 @@ typedef struct rrtl_hip_run_spec {
   uint32_t observe_point;       /* RRTL_OBSERVE_PRE / RRTL_OBSERVE_POST */
   uint32_t record_trace;        /* also keep every observed value: [cycles][...][...] */
   uint32_t halt_signal;         /* word engine: a lane stops after the cycle this bit is 1; UINT32_MAX = none */
+  /* Fields below are read only when struct_size covers them. */
+  /* Gate engine fault simulation (fault_sim.rs:66-97 moved on chip): lane 0 of
+   * every 64-lane group is the fault-free reference; a lane is flagged once any
+   * observed value differs from its group's lane 0 (rrtl_hip_get_detect).
+   * Replaces the digest of the run. */
+  uint32_t detect;
+  /* RRTL_IN_GROUP_RANDOM: one bit per 64-lane group g, shared by its lanes:
+   * bit 0 of stim_word(seed, signal << 8, cycle, group_key[g]) (group_key null:
+   * key g).  ceil(lanes/64) keys. */
+  const uint64_t* group_key;
 } rrtl_hip_run_spec;
 
 typedef struct rrtl_hip_run_stats {
@@ -144,6 +158,8 @@ size_t rrtl_hip_trace_bytes(const rrtl_hip_engine* e);
 rrtl_status rrtl_hip_get_trace(rrtl_hip_engine* e, void* out);
 /* Word engine runs with a halt signal: cycles simulated per lane. */
 rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out);
+/* Gate engine runs with detect set: lane flags packed 32 per u32 (ceil(lanes/32) words). */
+rrtl_status rrtl_hip_get_detect(rrtl_hip_engine* e, uint32_t* out_words);
 /* Gate engine: [tile_words, live_slots, levels, gates] of the last compiled run. */
 rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4);
 

@@ -141,7 +141,7 @@ class PackedRef:
 
 
 # ------------------------------------------------------------ bit-parallel
-IN_HOLD, IN_RANDOM, IN_CONST, IN_PULSE = 0, 1, 2, 3
+IN_HOLD, IN_RANDOM, IN_CONST, IN_PULSE, IN_GROUP_RANDOM = 0, 1, 2, 3, 5
 
 
 class RunSpec(ctypes.Structure):
@@ -150,6 +150,7 @@ class RunSpec(ctypes.Structure):
         ("input_sig", ctypes.c_void_p), ("input_mode", ctypes.c_void_p), ("input_arg", ctypes.c_void_p),
         ("n_obs", ctypes.c_uint32), ("obs_sig", ctypes.c_void_p), ("observe_post", ctypes.c_uint32),
         ("word_base", ctypes.c_uint64), ("digest", ctypes.c_void_p), ("trace", ctypes.c_void_p),
+        ("group_key", ctypes.c_void_p),
     ]
 
 
@@ -190,8 +191,9 @@ class _BPBase:
     words: int
 
     def _spec(self, *, seed, cycles, inputs, observe, observe_post=False, word_base=0,
-              digest=True, trace=False):
-        """``inputs``: list of (signal, mode, arg); ``observe``: signal list."""
+              digest=True, trace=False, group_key=None):
+        """``inputs``: list of (signal, mode, arg); ``observe``: signal list;
+        ``group_key``: IN_GROUP_RANDOM key per u64 word of this engine."""
         self._keep = []
         isig = np.array([i[0] for i in inputs] or [0], dtype=np.uint32)
         imode = np.array([i[1] for i in inputs] or [0], dtype=np.uint32)
@@ -199,10 +201,14 @@ class _BPBase:
         osig = np.array(list(observe) or [0], dtype=np.uint32)
         dg = np.zeros((64, self.words), dtype=np.uint64) if digest else None
         tr = np.zeros((cycles, len(observe), self.words), dtype=np.uint64) if trace else None
-        self._keep += [isig, imode, iarg, osig, dg, tr]
+        gk = None if group_key is None else np.ascontiguousarray(group_key, dtype=np.uint64)
+        if gk is not None and gk.size < self.words:
+            raise ValueError("group_key needs one key per u64 word")
+        self._keep += [isig, imode, iarg, osig, dg, tr, gk]
         spec = RunSpec(seed, cycles, len(inputs), isig.ctypes.data, imode.ctypes.data, iarg.ctypes.data,
                        len(observe), osig.ctypes.data, int(observe_post), word_base,
-                       dg.ctypes.data if dg is not None else None, tr.ctypes.data if tr is not None else None)
+                       dg.ctypes.data if dg is not None else None, tr.ctypes.data if tr is not None else None,
+                       gk.ctypes.data if gk is not None else None)
         return spec, dg, tr
 
 

@@ -36,7 +36,7 @@
 #define G 8 /* u64 words per group in the fast engine */
 
 /* ------------------------------------------------------------ run spec */
-enum { IN_HOLD = 0, IN_RANDOM = 1, IN_CONST = 2, IN_PULSE = 3 };
+enum { IN_HOLD = 0, IN_RANDOM = 1, IN_CONST = 2, IN_PULSE = 3, IN_GROUP_RANDOM = 5 };
 typedef struct {
     uint64_t seed;
     uint32_t cycles;
@@ -50,12 +50,17 @@ typedef struct {
     uint64_t word_base;          /* global index of this engine's word 0 (stimulus is per global word) */
     uint64_t *digest;            /* [64][words] or NULL */
     uint64_t *trace;             /* [cycles][n_obs][words] or NULL */
+    const uint64_t *group_key;   /* IN_GROUP_RANDOM: key per local u64 word (one 64-lane group), or NULL = global word */
 } bp_run_spec;
 
-static uint64_t input_word(const bp_run_spec *r, uint32_t i, uint32_t cyc, uint64_t key, uint64_t gword) {
+/* gword: global u64 word (64-lane group) index; lword: its index in this engine */
+static uint64_t input_word(const bp_run_spec *r, uint32_t i, uint32_t cyc, uint64_t key, uint64_t gword, uint64_t lword) {
     uint64_t arg = r->input_arg ? r->input_arg[i] : 0;
     switch (r->input_mode[i]) {
     case IN_RANDOM: return stim_word_from_key(key, gword);
+    case IN_GROUP_RANDOM:
+        /* one bit per group, shared by its 64 lanes */
+        return (stim_word_from_key(key, r->group_key ? r->group_key[lword] : gword) & 1) ? ~0ull : 0ull;
     case IN_CONST: return (arg & 1) ? ~0ull : 0ull;
     case IN_PULSE: {
         uint32_t n = (uint32_t)arg;
@@ -228,7 +233,7 @@ int bp_run(bp_sim *s, const bp_run_spec *r) {
             if (r->input_mode[i] == IN_HOLD) continue;
             uint64_t key = stim_stream_key(r->seed, (uint64_t)r->input_sig[i] << 8, c);
             uint64_t *d = &s->state[(size_t)r->input_sig[i] * W];
-            for (size_t w = 0; w < W; w++) d[w] = input_word(r, i, c, key, r->word_base + w);
+            for (size_t w = 0; w < W; w++) d[w] = input_word(r, i, c, key, r->word_base + w, w);
         }
         bp_settle(s);
         if (!r->observe_post) bp_observe(s, r, c);
@@ -476,7 +481,7 @@ double bpg_run(bpg_engine *e, const bp_run_spec *r) {
                 if (r->input_mode[i] == IN_HOLD) continue;
                 uint64_t key = stim_stream_key(r->seed, (uint64_t)r->input_sig[i] << 8, c);
                 for (uint32_t w = 0; w < G; w++)
-                    st[(size_t)r->input_sig[i] * G + w] = input_word(r, i, c, key, r->word_base + wlo + w);
+                    st[(size_t)r->input_sig[i] * G + w] = input_word(r, i, c, key, r->word_base + wlo + w, wlo + w < e->words ? wlo + w : 0);
             }
             group_settle(e, st, g);
             if (r->observe_post) group_edge(e, st, g), group_settle(e, st, g);

@@ -75,3 +75,24 @@ def u32_rows_to_u64(a: np.ndarray) -> np.ndarray:
     lo = a[..., 0::2].astype(U64)
     hi = a[..., 1::2].astype(U64)
     return lo | (hi << U64(32))
+
+
+def group_random_words(seed: int, sig_index: int, cycle: int, keys: np.ndarray) -> np.ndarray:
+    """u64 lane words of a 1-bit GROUP_RANDOM input for one cycle: bit 0 of
+    ``stim_word(seed, s << 8, c, key)`` for each 64-lane group, copied to all of
+    its lanes."""
+    w = stim_words(seed, sig_index << 8, cycle, np.asarray(keys, dtype=U64))
+    return np.where((w & U64(1)) != 0, U64(0xFFFFFFFFFFFFFFFF), U64(0))
+
+
+def detect_from_trace(trace: np.ndarray) -> np.ndarray:
+    """Fault-detection flags from a bit-sliced observation trace
+    [cycles][n_obs][words] u64: lane l of word w is flagged when any observed
+    bit differs from lane 0 of the same word at any cycle (the golden-lane
+    comparison of crates/rrtl-sim-ir/examples/fault_sim.rs:86-95).  Returns a
+    bool array [words * 64]."""
+    t = np.asarray(trace, dtype=U64)
+    gold = np.where((t & U64(1)) != 0, U64(0xFFFFFFFFFFFFFFFF), U64(0))
+    d = np.bitwise_or.reduce((t ^ gold).reshape(-1, t.shape[-1]), axis=0)
+    bits = np.unpackbits(d.view(np.uint8), bitorder="little")
+    return bits.astype(bool)

@@ -25,7 +25,7 @@ from .program import Program
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librrtl_hip.so")
 
 RRTL_HIP_AUTO, RRTL_HIP_WORD, RRTL_HIP_GATE = 0, 1, 2
-IN_HOLD, IN_RANDOM, IN_CONST, IN_PULSE, IN_TRACE = 0, 1, 2, 3, 4
+IN_HOLD, IN_RANDOM, IN_CONST, IN_PULSE, IN_TRACE, IN_GROUP_RANDOM = 0, 1, 2, 3, 4, 5
 OBSERVE_PRE, OBSERVE_POST = 0, 1
 NO_HALT = 0xFFFFFFFF
 ENGINE_NAMES = {"auto": RRTL_HIP_AUTO, "word": RRTL_HIP_WORD, "gate": RRTL_HIP_GATE}
@@ -36,9 +36,10 @@ EXPORTS = [
     "rrtl_hip_lanes", "rrtl_hip_set_signal_limbs", "rrtl_hip_set_signal_limbs_raw", "rrtl_hip_get_signal_limbs",
     "rrtl_hip_get_signal_bits", "rrtl_hip_set_memory_limbs", "rrtl_hip_get_memory_limbs",
     "rrtl_hip_eval_combinational", "rrtl_hip_tick", "rrtl_hip_tick_from_evaluated", "rrtl_hip_tick_many",
-    "rrtl_hip_snapshot", "rrtl_hip_restore", "rrtl_hip_set_force", "rrtl_hip_clear_forces",
+    "rrtl_hip_snapshot", "rrtl_hip_restore", "rrtl_hip_set_force", "rrtl_hip_set_forces", "rrtl_hip_clear_forces",
     "rrtl_hip_synchronize", "rrtl_hip_run", "rrtl_hip_last_run_stats", "rrtl_hip_digest_bytes",
     "rrtl_hip_get_digest", "rrtl_hip_trace_bytes", "rrtl_hip_get_trace", "rrtl_hip_get_halt_cycles",
+    "rrtl_hip_get_detect",
     "rrtl_hip_gate_tile_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
     "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy",
 ]
@@ -64,7 +65,8 @@ class RunSpec(ctypes.Structure):
                 ("n_inputs", ctypes.c_uint32), ("input_signal", ctypes.c_void_p), ("input_mode", ctypes.c_void_p),
                 ("input_arg", ctypes.c_void_p), ("trace_inputs", ctypes.c_void_p), ("n_observe", ctypes.c_uint32),
                 ("observe_signal", ctypes.c_void_p), ("observe_point", ctypes.c_uint32),
-                ("record_trace", ctypes.c_uint32), ("halt_signal", ctypes.c_uint32)]
+                ("record_trace", ctypes.c_uint32), ("halt_signal", ctypes.c_uint32), ("detect", ctypes.c_uint32),
+                ("group_key", ctypes.c_void_p)]
 
 
 class RunStats(ctypes.Structure):
@@ -101,6 +103,8 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_snapshot.argtypes = [vp, vp, vp]
     L.rrtl_hip_restore.argtypes = [vp, vp, vp]
     L.rrtl_hip_set_force.argtypes = [vp, u32, u32, ctypes.c_int]
+    L.rrtl_hip_set_forces.argtypes = [vp, vp, vp, vp, sz]
+    L.rrtl_hip_get_detect.argtypes = [vp, vp]
     L.rrtl_hip_run.argtypes = [vp, ctypes.POINTER(RunSpec)]
     L.rrtl_hip_last_run_stats.argtypes = [vp, ctypes.POINTER(RunStats)]
     L.rrtl_hip_digest_bytes.restype = sz
@@ -268,32 +272,51 @@ class HipSimulator:
     def set_force(self, s: SignalRef, lane: int, value: int) -> None:
         self._check(self.L.rrtl_hip_set_force(self.h, self.sig(s), lane, int(value)))
 
+    def set_forces(self, signals, lanes, values) -> None:
+        """Many forces at once: arrays of signal index, lane and stuck value."""
+        s = np.ascontiguousarray(signals, dtype=np.uint32)
+        ln = np.ascontiguousarray(lanes, dtype=np.uint32)
+        v = np.ascontiguousarray(values, dtype=np.uint8)
+        if not (s.size == ln.size == v.size):
+            raise ValueError("set_forces: arrays differ in length")
+        self._check(self.L.rrtl_hip_set_forces(self.h, _ptr(s), _ptr(ln), _ptr(v), s.size))
+
     def clear_forces(self) -> None:
         self._check(self.L.rrtl_hip_clear_forces(self.h))
 
     # ------------------------------------------------------------ runs
     def run(self, cycles: int, seed: int = 0, inputs: Sequence[Tuple] = (), observe: Sequence[SignalRef] = (),
             observe_post: bool = False, trace: bool = False, halt: Optional[SignalRef] = None,
-            trace_inputs: Optional[np.ndarray] = None) -> None:
+            trace_inputs: Optional[np.ndarray] = None, detect: bool = False,
+            group_key: Optional[np.ndarray] = None) -> None:
         """``inputs``: (signal, mode, arg) triples with modes IN_*; ``observe``: signals
-        folded into the digest (and the trace when ``trace``)."""
-        spec = self.run_spec(cycles, seed, inputs, observe, observe_post, trace, halt, trace_inputs)
+        folded into the digest (and the trace when ``trace``).  ``detect``: flag
+        lanes whose observed values leave lane 0 of their 64-lane group (read
+        with ``detect_flags``); ``group_key``: IN_GROUP_RANDOM key per group."""
+        spec = self.run_spec(cycles, seed, inputs, observe, observe_post, trace, halt, trace_inputs, detect, group_key)
         self._check(self.L.rrtl_hip_run(self.h, ctypes.byref(spec)))
 
     def run_spec(self, cycles: int, seed: int = 0, inputs: Sequence[Tuple] = (), observe: Sequence[SignalRef] = (),
                  observe_post: bool = False, trace: bool = False, halt: Optional[SignalRef] = None,
-                 trace_inputs: Optional[np.ndarray] = None) -> RunSpec:
+                 trace_inputs: Optional[np.ndarray] = None, detect: bool = False,
+                 group_key: Optional[np.ndarray] = None) -> RunSpec:
         """Builds the C run spec; its arrays stay alive on this object until the next call."""
         isig = np.array([self.sig(i[0]) for i in inputs] or [0], dtype=np.uint32)
         imode = np.array([i[1] for i in inputs] or [0], dtype=np.uint32)
         iarg = np.array([int(i[2]) if len(i) > 2 else 0 for i in inputs] or [0], dtype=np.uint64)
         osig = np.array([self.sig(o) for o in observe] or [0], dtype=np.uint32)
         tin = None if trace_inputs is None else np.ascontiguousarray(trace_inputs, dtype=np.uint32)
-        self._keep = (isig, imode, iarg, osig, tin)
+        gk = None
+        if group_key is not None:
+            gk = np.ascontiguousarray(group_key, dtype=np.uint64)
+            if gk.size != (self.lanes + 63) // 64:
+                raise ValueError("group_key needs one key per 64-lane group")
+        self._keep = (isig, imode, iarg, osig, tin, gk)
         spec = RunSpec(ctypes.sizeof(RunSpec), seed, cycles, len(inputs), isig.ctypes.data, imode.ctypes.data,
                        iarg.ctypes.data, tin.ctypes.data if tin is not None else None, len(observe),
                        osig.ctypes.data, OBSERVE_POST if observe_post else OBSERVE_PRE, int(trace),
-                       NO_HALT if halt is None else self.sig(halt))
+                       NO_HALT if halt is None else self.sig(halt), int(bool(detect)),
+                       gk.ctypes.data if gk is not None else None)
         return spec
 
     def stats(self) -> dict:
@@ -316,6 +339,13 @@ class HipSimulator:
         out = np.zeros(n // 4, dtype=np.uint32)
         self._check(self.L.rrtl_hip_get_trace(self.h, _ptr(out)))
         return out
+
+    def detect_flags(self) -> np.ndarray:
+        """Per-lane detection flags of the last detect run (bool array)."""
+        out = np.zeros(self.words32, dtype=np.uint32)
+        self._check(self.L.rrtl_hip_get_detect(self.h, _ptr(out)))
+        bits = np.unpackbits(out.view(np.uint8), bitorder="little")
+        return bits[: self.lanes].astype(bool)
 
     def halt_cycles(self) -> np.ndarray:
         out = np.zeros(self.lanes, dtype=np.uint64)

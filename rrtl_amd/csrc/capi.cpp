@@ -54,8 +54,9 @@ rrtl_status guard(rrtl_hip_engine* e, F&& f) {
   }
 }
 
-rrtl::RunSpec to_spec(const rrtl_hip_run_spec* s) {
-  if (!s || s->struct_size < sizeof(rrtl_hip_run_spec)) throw rrtl::Error("E_HIP_RUN_SPEC: run spec is null or too small");
+rrtl::RunSpec to_spec(const rrtl_hip_run_spec* s, uint32_t lanes) {
+  if (!s || s->struct_size < offsetof(rrtl_hip_run_spec, detect))
+    throw rrtl::Error("E_HIP_RUN_SPEC: run spec is null or too small");
   rrtl::RunSpec r;
   r.seed = s->seed; r.cycles = s->cycles;
   if (s->n_inputs && (!s->input_signal || !s->input_mode)) throw rrtl::Error("E_HIP_RUN_SPEC: input arrays are null");
@@ -64,12 +65,17 @@ rrtl::RunSpec to_spec(const rrtl_hip_run_spec* s) {
   r.in_mode.assign(s->input_mode, s->input_mode + s->n_inputs);
   if (s->input_arg) r.in_arg.assign(s->input_arg, s->input_arg + s->n_inputs);
   else r.in_arg.assign(s->n_inputs, 0);
-  for (uint32_t m : r.in_mode) if (m > RRTL_IN_TRACE) throw rrtl::Error("E_HIP_RUN_SPEC: unknown input mode");
+  for (uint32_t m : r.in_mode) if (m > RRTL_IN_GROUP_RANDOM) throw rrtl::Error("E_HIP_RUN_SPEC: unknown input mode");
   r.obs_sig.assign(s->observe_signal, s->observe_signal + s->n_observe);
   r.observe_post = s->observe_point == RRTL_OBSERVE_POST;
   r.trace = s->record_trace != 0;
   r.halt_sig = s->halt_signal;
   r.trace_inputs = s->trace_inputs;
+  if (s->struct_size >= offsetof(rrtl_hip_run_spec, detect) + sizeof(uint32_t)) r.detect = s->detect != 0;
+  if (s->struct_size >= offsetof(rrtl_hip_run_spec, group_key) + sizeof(void*) && s->group_key) {
+    // one key per 64-lane group of the engine
+    r.group_key.assign(s->group_key, s->group_key + (uint64_t(lanes) + 63) / 64);
+  }
   return r;
 }
 }  // namespace
@@ -167,10 +173,16 @@ rrtl_status rrtl_hip_restore(rrtl_hip_engine* e, const uint32_t* v, const uint32
 rrtl_status rrtl_hip_set_force(rrtl_hip_engine* e, uint32_t s, uint32_t lane, int value) {
   return guard(e, [&](rrtl::Engine& x) { x.set_force(s, lane, value); });
 }
+rrtl_status rrtl_hip_set_forces(rrtl_hip_engine* e, const uint32_t* s, const uint32_t* lane, const uint8_t* value, size_t n) {
+  return guard(e, [&](rrtl::Engine& x) {
+    if (n && (!s || !lane || !value)) throw rrtl::Error("E_HIP_NULL: null buffer");
+    x.set_forces(s, lane, value, n);
+  });
+}
 rrtl_status rrtl_hip_clear_forces(rrtl_hip_engine* e) { return guard(e, [](rrtl::Engine& x) { x.clear_forces(); }); }
 rrtl_status rrtl_hip_synchronize(rrtl_hip_engine* e) { return guard(e, [](rrtl::Engine& x) { x.synchronize(); }); }
 rrtl_status rrtl_hip_run(rrtl_hip_engine* e, const rrtl_hip_run_spec* spec) {
-  return guard(e, [&](rrtl::Engine& x) { x.run(to_spec(spec)); });
+  return guard(e, [&](rrtl::Engine& x) { x.run(to_spec(spec, x.lanes())); });
 }
 rrtl_status rrtl_hip_last_run_stats(const rrtl_hip_engine* e, rrtl_hip_run_stats* out) {
   if (!e || !e->impl || !out) return RRTL_E_INVALID;
@@ -188,6 +200,9 @@ rrtl_status rrtl_hip_get_trace(rrtl_hip_engine* e, void* out) {
 }
 rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out) {
   return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_halt_cycles(out); });
+}
+rrtl_status rrtl_hip_get_detect(rrtl_hip_engine* e, uint32_t* out) {
+  return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_detect(out); });
 }
 rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4) {
   return guard(e, [&](rrtl::Engine& x) {
@@ -208,7 +223,7 @@ rrtl_status rrtl_hip_batch_create(rrtl_hip_engine* const* engines, const rrtl_hi
     for (uint32_t i = 0; i < n; i++) {
       if (!engines[i] || !engines[i]->impl) throw rrtl::Error("E_HIP_NULL: null engine in batch");
       es.push_back(engines[i]->impl.get());
-      ss.push_back(to_spec(&specs[i]));
+      ss.push_back(to_spec(&specs[i], engines[i]->impl->lanes()));
     }
     auto* b = new rrtl_hip_batch;
     try { b->impl = rrtl::make_batch(std::move(es), std::move(ss)); } catch (...) { delete b; throw; }

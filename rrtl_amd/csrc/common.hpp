@@ -35,7 +35,7 @@ __host__ __device__ inline uint64_t lane_digest(uint64_t d, uint32_t limb, uint3
 }
 
 // Stimulus modes (rrtl_hip_input_mode in the C header).
-enum : uint32_t { IN_HOLD = 0, IN_RANDOM = 1, IN_CONST = 2, IN_PULSE = 3, IN_TRACE = 4 };
+enum : uint32_t { IN_HOLD = 0, IN_RANDOM = 1, IN_CONST = 2, IN_PULSE = 3, IN_TRACE = 4, IN_GROUP_RANDOM = 5 };
 
 struct RunSpec {
   uint64_t seed = 0, cycles = 0;
@@ -46,6 +46,8 @@ struct RunSpec {
   bool trace = false;
   uint32_t halt_sig = 0xFFFFFFFFu;
   const uint32_t* trace_inputs = nullptr;  // IN_TRACE values, host [cycles][n_trace_limbs][lanes]
+  bool detect = false;                     // flag lanes whose observed values leave lane 0 of their 64-lane group
+  std::vector<uint64_t> group_key;         // IN_GROUP_RANDOM key per 64-lane group (empty: the group index)
 };
 
 struct RunStats {

@@ -56,6 +56,10 @@ class Engine {
   virtual void snapshot(uint32_t* values, uint32_t* memories) = 0;
   virtual void restore(const uint32_t* values, const uint32_t* memories) = 0;
   virtual void set_force(uint32_t sig, uint32_t lane, int value) = 0;
+  // n forces at once, in order, as n set_force calls
+  virtual void set_forces(const uint32_t* sig, const uint32_t* lane, const uint8_t* value, size_t n) {
+    for (size_t i = 0; i < n; i++) set_force(sig[i], lane[i], value[i]);
+  }
   virtual void clear_forces() = 0;
   virtual void run(const RunSpec& spec) = 0;
   // Results of the last run: digest (gate engine [64][words32] u32 bit-sliced,
@@ -65,6 +69,8 @@ class Engine {
   virtual size_t trace_bytes() const = 0;
   virtual void get_trace(void* out) = 0;
   virtual void get_halt_cycles(uint64_t* out) { (void)out; throw Error("E_HIP_UNSUPPORTED: engine has no halting"); }
+  // Runs with detect set: lane flags packed 32 per u32 word (ceil(lanes/32) words).
+  virtual void get_detect(uint32_t* out) { (void)out; throw Error("E_HIP_UNSUPPORTED: engine has no fault detection"); }
   virtual void synchronize() { RRTL_HIP_CHECK(hipStreamSynchronize(stream_)); }
   // Multi-design batches: compile and upload a run, return its kernel
   // arguments; reset its outputs before every launch; book-keep after it.

@@ -37,7 +37,11 @@ struct TileArgs {
   const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
   const uint4* commit; uint32_t n_commit;
   const uint32_t* obs_slot; uint32_t n_obs;     // slot | stored value inverted << 31
-  const uint32_t* force; const uint32_t* force_start;  // per tile: 5-word entries (pos, slot, word, and, or)
+  // per tile: 5-word entries (pos, slot, word, and, or); tile t owns
+  // [force_start[t], force_start[t+1]), of which [force_mid[t], ...) clamp at the commit
+  const uint32_t* force; const uint32_t* force_start; const uint32_t* force_mid;
+  const uint64_t* group_key;                           // IN_GROUP_RANDOM key per 64-lane group, or null (= group index)
+  uint32_t* detect;                                    // [nwords] lanes whose observed values left lane 0 of their group, or null
   uint32_t* digest;                                    // [64][nwords] or null
   uint32_t* trace;                                     // [cycles][n_obs][nwords] or null
   uint64_t seed, cycle_base, cycles;
@@ -168,9 +172,23 @@ __device__ __forceinline__ void apply_clamps(const TileArgs& A, uint32_t* lds, u
 // Signature of the observed outputs: per cycle c, S[(j + c) & 63] ^= v_j for every
 // observed word v_j, then one nonlinear round S'[p] = S[p] ^ (S[p+17] & ~S[p+41]).
 // Bitwise, so every lane keeps its own 64-bit signature.
+//
+// Detection (fault simulation): lane 0 of every 64-lane group (bit 0 of the
+// group's even u32 word) is the reference lane; a lane is flagged once any
+// observed value differs from its group's lane 0.  An inverted stored value
+// flips both sides of the comparison, so the inversion flag drops out.  The
+// flags accumulate in the thread's registers (thread j owns observed signals
+// j, j + 256, ...) and reach HBM once per tile run.
 template <int W>
-__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32_t* sig, uint64_t c, uint64_t cyc, uint64_t w0) {
+__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32_t* sig, uint64_t c, uint64_t cyc, uint64_t w0,
+                                        Words<W>& det) {
   const uint32_t tid = threadIdx.x;
+  if (A.detect)
+    for (uint32_t j = tid; j < A.n_obs; j += TILE_THREADS) {
+      const Words<W> v = ld_words<W>(lds, A.obs_slot[j] & 0x7FFFFFFFu);
+#pragma unroll
+      for (int k = 0; k < W; k++) det.v[k] |= v.v[k] ^ (0u - (v.v[k & ~1] & 1u));
+    }
   if (A.trace)
     for (uint32_t i = tid; i < A.n_obs * W; i += TILE_THREADS) {
       uint32_t j = i / W, w = i % W;
@@ -229,6 +247,39 @@ __device__ __forceinline__ void commit_regs(const TileArgs& A, uint32_t* lds, ui
   if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
 }
 
+// Input words of one tile for one cycle, for the thread that drives input s.
+template <int W>
+__device__ __forceinline__ Words<W> stimulus(const TileArgs& A, uint32_t s, uint64_t cyc, uint64_t w0, const uint64_t* gkey) {
+  const uint32_t mode = A.stim_mode[s];
+  const uint64_t arg = A.stim_arg[s];
+  Words<W> v;
+  if (mode == IN_RANDOM) {
+    const uint64_t key = stim_key(A.seed, A.stim_stream[s], cyc);
+#pragma unroll
+    for (int k = 0; k < W; k += 2) {
+      const uint64_t gw = w0 + k;
+      const uint64_t word = stim_from_key(key, gw >> 1);
+      v.v[k] = uint32_t(word >> (32 * (gw & 1)));
+      if (k + 1 < W) v.v[k + 1] = uint32_t(word >> 32);
+    }
+  } else if (mode == IN_GROUP_RANDOM) {
+    // one bit per 64-lane group, the same in all of its lanes
+    const uint64_t key = stim_key(A.seed, A.stim_stream[s], cyc);
+#pragma unroll
+    for (int k = 0; k < W; k += 2) {
+      const uint32_t b = 0u - uint32_t(stim_from_key(key, gkey[k / 2]) & 1u);
+      v.v[k] = b;
+      if (k + 1 < W) v.v[k + 1] = b;
+    }
+  } else {
+    const uint32_t bit = mode == IN_CONST ? uint32_t(arg & 1)
+                                          : uint32_t((cyc < (arg & 0xFFFFFFFFull)) ? (arg >> 32) & 1 : (arg >> 33) & 1);
+#pragma unroll
+    for (int k = 0; k < W; k++) v.v[k] = bit ? ~0u : 0u;
+  }
+  return v;
+}
+
 // One stage: this thread's positions (descriptors fetched STAGE_AHEAD stages
 // earlier), the fetch for STAGE_AHEAD stages later into the same registers
 // (the table carries STAGE_AHEAD padding stages, so the fetch needs no bound
@@ -280,7 +331,18 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
   const uint32_t tid = threadIdx.x;
   const uint64_t w0 = uint64_t(tile) * W;
   const uint32_t nst = A.n_cycle_stages + A.n_post_stages;
+  // stage clamps [fb, fm), commit clamps [fm, fe)
   const uint32_t fb = A.force_start ? A.force_start[tile] : 0, fe = A.force_start ? A.force_start[tile + 1] : 0;
+  const uint32_t fm = A.force_start ? A.force_mid[tile] : 0;
+  uint64_t gkey[(W + 1) / 2];
+#pragma unroll
+  for (int k = 0; k < W; k += 2) {
+    const uint64_t g = (w0 + k) >> 1;
+    gkey[k / 2] = A.group_key && w0 + k < A.nwords ? A.group_key[g] : g;
+  }
+  Words<W> det;
+#pragma unroll
+  for (int k = 0; k < W; k++) det.v[k] = 0u;
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
     uint32_t s = i / W, w = i % W;
     uint64_t gw = w0 + w;
@@ -296,33 +358,14 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
   for (uint64_t c = 0; c < A.cycles; c++) {
     const uint64_t cyc = A.cycle_base + c;
     // primary inputs for this cycle (thread per input, all words of the tile)
-    for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS) {
-      const uint64_t arg = A.stim_arg[s];
-      Words<W> v;
-      if (A.stim_mode[s] == IN_RANDOM) {
-        const uint64_t key = stim_key(A.seed, A.stim_stream[s], cyc);
-#pragma unroll
-        for (int k = 0; k < W; k += 2) {
-          const uint64_t gw = w0 + k;
-          const uint64_t word = stim_from_key(key, gw >> 1);
-          v.v[k] = uint32_t(word >> (32 * (gw & 1)));
-          if (k + 1 < W) v.v[k + 1] = uint32_t(word >> 32);
-        }
-      } else {
-        const uint32_t bit = A.stim_mode[s] == IN_CONST ? uint32_t(arg & 1)
-                                                        : uint32_t((cyc < (arg & 0xFFFFFFFFull)) ? (arg >> 32) & 1 : (arg >> 33) & 1);
-#pragma unroll
-        for (int k = 0; k < W; k++) v.v[k] = bit ? ~0u : 0u;
-      }
-      st_words<W>(lds, A.stim_slot[s], v);
-    }
+    for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS) st_words<W>(lds, A.stim_slot[s], stimulus<W>(A, s, cyc, w0, gkey));
     __syncthreads();
     // cycle program, observe before the edge, commit, post program
-    run_program<W, kBA, M>(A, lds, 0, A.n_cycle_stages, fb, fe);
-    if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
-    commit_regs<W>(A, lds, fb, fe);
-    if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fe);
-    if (A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0);
+    run_program<W, kBA, M>(A, lds, 0, A.n_cycle_stages, fb, fm);
+    if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0, det);
+    commit_regs<W>(A, lds, fm, fe);
+    if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fm);
+    if (A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0, det);
   }
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
     uint32_t s = i / W, w = i % W;
@@ -335,6 +378,10 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
       uint64_t gw = w0 + w;
       if (gw < A.nwords) A.digest[p * A.nwords + gw] = sig[i];
     }
+  if (A.detect && tid < A.n_obs)
+#pragma unroll
+    for (int k = 0; k < W; k++)
+      if (det.v[k] && w0 + k < A.nwords) atomicOr(&A.detect[w0 + k], det.v[k]);
 }
 
 template <int W, bool kBA, int M>

@@ -40,6 +40,15 @@ __global__ void queue_reset_kernel(uint32_t* ctl, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0u;
 }
 
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  const uint32_t grid = uint32_t(std::min<uint64_t>((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, p, n, v);
+  RRTL_HIP_CHECK(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ compile
 struct GateNet {
   uint32_t nsig = 0, n_nodes = 0, c0 = 0, c1 = 0;
@@ -653,6 +662,7 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
   for (size_t i = 0; i < spec.in_sig.size(); i++) {
     if (spec.in_mode[i] == IN_HOLD) continue;
     if (spec.in_mode[i] == IN_TRACE) throw Error("E_HIP_UNSUPPORTED: trace-driven inputs are not supported by the gate engine run");
+    if (spec.in_mode[i] > IN_GROUP_RANDOM) throw Error("E_HIP_RUN_SPEC: unknown input mode");
     T.stim_slot.push_back(T.persist.at(spec.in_sig[i]));
     T.stim_mode.push_back(spec.in_mode[i]);
     T.stim_stream.push_back(spec.in_sig[i] << 8);
@@ -756,6 +766,7 @@ class GateEngine final : public Engine {
     if (ev1_) (void)hipEventDestroy(ev1_);
     state_.release(); settle_g_.release(); next_g_.release(); commit_g_.release();
     fstart_.release(); fent_.release(); tile_force_.release(); tile_fstart_.release(); digest_.release(); trace_.release();
+    detect_.release(); group_key_.release();
     for (auto& b : prog_bufs_) b.release();
     if (stream_) (void)hipStreamDestroy(stream_);
   }
@@ -827,7 +838,24 @@ class GateEngine final : public Engine {
     forces_dirty_ = true;
     forces_version_++;
   }
+  void set_forces(const uint32_t* sig, const uint32_t* lane, const uint8_t* value, size_t n) override {
+    for (size_t i = 0; i < n; i++) {
+      check_sig(sig[i]);
+      if (lane[i] >= lanes_) throw Error("E_SIM_IR_LANE: lane is out of range");
+      if (P.signals[sig[i]].kind == K_INPUT) throw Error("E_HIP_FORCE_INPUT: inputs are driven by the caller; force the net they feed instead");
+    }
+    forces_.reserve(forces_.size() + n);
+    for (size_t i = 0; i < n; i++) forces_.push_back({sig[i], lane[i], value[i] ? 1u : 0u});
+    forces_dirty_ = true;
+    forces_version_++;
+  }
   void clear_forces() override { forces_.clear(); forces_dirty_ = true; forces_version_++; }
+
+  void get_detect(uint32_t* out) override {
+    if (!has_detect_) throw Error("E_HIP_NO_RUN: no run has produced detection flags");
+    RRTL_HIP_CHECK(hipMemcpyAsync(out, detect_.p, nwords_ * 4, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+  }
 
   size_t digest_bytes() const override { return has_digest_ ? size_t(64) * nwords_ * 4 : 0; }
   void get_digest(void* out) override {
@@ -877,21 +905,41 @@ class GateEngine final : public Engine {
     synchronize();
   }
 
+  // Forces merged per (key, u32 lane-word) into (and, or) masks, later forces
+  // winning: entries sorted by key, word and arrival order, then folded.
+  struct MergedForce { uint64_t key; uint32_t word, andm, orm; };
+  template <class KeyOf>
+  std::vector<MergedForce> merge_forces(KeyOf key_of) const {
+    struct E { uint64_t key; uint32_t word, idx; };
+    std::vector<E> e;
+    e.reserve(forces_.size());
+    for (uint32_t i = 0; i < forces_.size(); i++) {
+      uint64_t k;
+      uint32_t w;
+      if (key_of(forces_[i], k, w)) e.push_back({k, w, i});
+    }
+    std::sort(e.begin(), e.end(), [](const E& a, const E& b) {
+      return a.key != b.key ? a.key < b.key : a.word != b.word ? a.word < b.word : a.idx < b.idx;
+    });
+    std::vector<MergedForce> out;
+    for (size_t i = 0; i < e.size(); i++) {
+      if (i == 0 || e[i].key != e[i - 1].key || e[i].word != e[i - 1].word) out.push_back({e[i].key, e[i].word, ~0u, 0u});
+      const Force& f = forces_[e[i].idx];
+      const uint32_t bit = 1u << (f.lane % 32);
+      out.back().andm &= ~bit;
+      if (f.value) out.back().orm |= bit; else out.back().orm &= ~bit;
+    }
+    return out;
+  }
+
   // per-node force lists for the HBM kernels
   void sync_forces() {
     if (!forces_dirty_) return;
     std::vector<uint32_t> start(N.n_nodes + 1, 0);
-    std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> m;  // (node, word) -> (and, or)
-    for (auto& f : forces_) {
-      auto key = std::make_pair(f.sig, f.lane / 32);
-      auto it = m.find(key);
-      if (it == m.end()) it = m.emplace(key, std::make_pair(~0u, 0u)).first;
-      uint32_t bit = 1u << (f.lane % 32);
-      it->second.first &= ~bit;
-      if (f.value) it->second.second |= bit; else it->second.second &= ~bit;
-    }
+    auto m = merge_forces([](const Force& f, uint64_t& k, uint32_t& w) { k = f.sig; w = f.lane / 32; return true; });
     std::vector<uint32_t> ent;
-    for (auto& kv : m) { start[kv.first.first + 1]++; ent.insert(ent.end(), {kv.first.second, kv.second.first, kv.second.second}); }
+    ent.reserve(m.size() * 3);
+    for (auto& x : m) { start[x.key + 1]++; ent.insert(ent.end(), {x.word, x.andm, x.orm}); }
     for (uint32_t n = 0; n < N.n_nodes; n++) start[n + 1] += start[n];
     fstart_.upload(start, stream_);
     fent_.upload(ent, stream_);
@@ -924,7 +972,14 @@ class GateEngine final : public Engine {
   }
 
   // W lane-words per tile; byte_addr: descriptors hold LDS byte offsets.
-  struct Prepared { TileArgs A; uint32_t W = 0, M = 2, tiles = 0; size_t lds = 0; bool byte_addr = false; };
+  struct Prepared {
+    TileArgs A;
+    uint32_t W = 0, M = 2, tiles = 0;
+    size_t lds = 0;
+    bool byte_addr = false;
+    std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> clamp_sites;  // forced node -> (position, slot)
+    uint64_t force_version = ~0ull;                                               // forces the entries were built from
+  };
 
   // The compiled tile program depends on which inputs are driven, what is
   // observed and where forces sit; repeated runs with the same shape reuse it.
@@ -935,7 +990,11 @@ class GateEngine final : public Engine {
     for (size_t i = 0; i < spec.in_sig.size(); i++) { put(spec.in_sig[i]); put(spec.in_mode[i]); put(spec.in_arg.empty() ? 0 : spec.in_arg[i]); }
     put(spec.obs_sig.size());
     for (uint32_t s : spec.obs_sig) put(s);
-    put(spec.observe_post); put(forces_version_); put(opt_.tile_words); put(opt_.stage_gates);
+    put(spec.observe_post); put(opt_.tile_words); put(opt_.stage_gates);
+    // which signals carry a force shapes the program; the lanes do not
+    std::vector<uint8_t> forced(N.nsig, 0);
+    for (auto& f : forces_) forced[f.sig] = 1;
+    for (uint32_t s = 0; s < N.nsig; s++) if (forced[s]) put(s);
     return k;
   }
 
@@ -948,14 +1007,35 @@ class GateEngine final : public Engine {
     // wire it reads, so stale wire rows stay stale until an API read.
     const std::string key = plan_key(spec);
     Prepared R;
-    if (key == plan_key_ && have_plan_) {
-      R = plan_;
-    } else {
-      R = compile_and_upload(spec);
-      plan_ = R; plan_key_ = key; have_plan_ = true;
+    if (!(key == plan_key_ && have_plan_)) {
+      plan_ = compile_and_upload(spec);
+      plan_key_ = key; have_plan_ = true;
     }
+    if (plan_.force_version != forces_version_) {
+      upload_force_entries(plan_);
+      plan_.force_version = forces_version_;
+    }
+    R = plan_;
     TileArgs& A = R.A;
-    has_digest_ = with_outputs && !spec.obs_sig.empty();
+    // detection: lanes compared with lane 0 of their 64-lane group, which
+    // needs both u32 words of every group in one tile
+    const bool detect = with_outputs && spec.detect;
+    if (spec.detect && spec.obs_sig.empty()) throw Error("E_HIP_RUN_SPEC: detection needs observed signals");
+    if (detect && R.W < 2) throw Error("E_HIP_UNSUPPORTED: detection needs tiles of at least two lane-words (64 lanes)");
+    has_detect_ = detect;
+    if (detect) {
+      detect_.alloc(nwords_);
+      fill_u32(detect_.p, nwords_, 0u, stream_);
+    }
+    A.detect = detect ? detect_.p : nullptr;
+    const uint64_t groups = (nwords_ + 1) / 2;
+    if (!spec.group_key.empty()) {
+      if (spec.group_key.size() < groups) throw Error("E_HIP_RUN_SPEC: group_key needs one key per 64-lane group");
+      group_key_.alloc(groups);
+      RRTL_HIP_CHECK(hipMemcpyAsync(group_key_.p, spec.group_key.data(), groups * 8, hipMemcpyHostToDevice, stream_));
+    }
+    A.group_key = spec.group_key.empty() ? nullptr : group_key_.p;
+    has_digest_ = with_outputs && !spec.obs_sig.empty() && !detect;
     if (has_digest_) { digest_.alloc(size_t(64) * nwords_); digest_.zero(stream_); }
     A.digest = has_digest_ ? digest_.p : nullptr;
     if (!with_outputs) A.n_obs = 0;
@@ -1001,30 +1081,6 @@ class GateEngine final : public Engine {
     const uint32_t M = T.M;
     W_ = W; M_ = M; last_slots_ = T.n_slots; last_levels_ = T.n_cycle_stages + T.n_post_stages; last_gates_ = T.n_gates;
     const uint32_t tiles = uint32_t((nwords_ + W - 1) / W);
-    // per-tile force clamp entries (pos, slot, word, and, or), merged per LDS word
-    std::vector<uint32_t> fent, fstart(tiles + 1, 0);
-    if (!forces_.empty()) {
-      std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> m;  // (tile,pos,slot,word)
-      std::unordered_map<uint32_t, std::vector<std::pair<uint32_t, uint32_t>>> sites;
-      for (auto& cs : T.clamp_sites) sites[cs.first].push_back(cs.second);
-      for (auto& f : forces_) {
-        auto it = sites.find(f.sig);
-        if (it == sites.end()) continue;
-        uint32_t gw = f.lane / 32, tile = gw / W, word = gw % W, bit = 1u << (f.lane % 32);
-        for (auto& ps : it->second) {
-          auto key = std::make_tuple(tile, ps.first, ps.second, word);
-          auto e = m.find(key);
-          if (e == m.end()) e = m.emplace(key, std::make_pair(~0u, 0u)).first;
-          e->second.first &= ~bit;
-          if (f.value) e->second.second |= bit; else e->second.second &= ~bit;
-        }
-      }
-      for (auto& kv : m) {
-        fstart[std::get<0>(kv.first) + 1]++;
-        fent.insert(fent.end(), {std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), kv.second.first, kv.second.second});
-      }
-      for (uint32_t t = 0; t < tiles; t++) fstart[t + 1] += fstart[t];
-    }
     TileArgs A{};
     A.state = state_.p; A.nwords = nwords_;
     // descriptors [stage][position][thread] as compiled, STAGE_AHEAD padding
@@ -1059,12 +1115,69 @@ class GateEngine final : public Engine {
     for (size_t k = 0; k < cm.size(); k++) cm[k] = make_uint4(T.commit[4 * k], T.commit[4 * k + 1], T.commit[4 * k + 2], T.commit[4 * k + 3]);
     A.commit = upload_prog(8, cm); A.n_commit = uint32_t(cm.size());
     A.obs_slot = upload_prog(9, T.obs_slot); A.n_obs = uint32_t(T.obs_slot.size());
-    A.force = fent.empty() ? nullptr : upload_prog(10, fent);
-    A.force_start = fent.empty() ? nullptr : upload_prog(11, fstart);
     A.n_slots = n_slots; A.c0 = T.c0; A.c1 = T.c1;
     Prepared R;
     R.A = A; R.W = W; R.M = M; R.tiles = tiles; R.lds = lds; R.byte_addr = byte_addr;
+    R.clamp_sites = std::move(T.clamp_sites);
     return R;
+  }
+
+  // Per-tile force clamp entries (pos, slot, word, and, or), merged per LDS
+  // word; within a tile the stage clamps come first (ordered by position) and
+  // the commit clamps (position CLAMP_COMMIT) last, from force_mid on.
+  void upload_force_entries(Prepared& R) {
+    TileArgs& A = R.A;
+    A.force = A.force_start = A.force_mid = nullptr;
+    if (forces_.empty()) return;
+    const uint32_t W = R.W, tiles = R.tiles;
+    // clamp sites ranked by (position, slot); each forced signal's site ranks
+    std::vector<std::pair<uint32_t, uint32_t>> site;  // (position, slot) by rank
+    std::unordered_map<uint32_t, std::vector<uint32_t>> ranks_of;
+    {
+      std::vector<uint32_t> ord(R.clamp_sites.size());
+      for (uint32_t i = 0; i < ord.size(); i++) ord[i] = i;
+      std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return R.clamp_sites[x].second < R.clamp_sites[y].second; });
+      for (uint32_t r = 0; r < ord.size(); r++) {
+        site.push_back(R.clamp_sites[ord[r]].second);
+        ranks_of[R.clamp_sites[ord[r]].first].push_back(r);
+      }
+    }
+    // one item per (force, site), keyed tile | rank, then word, then arrival
+    struct E { uint64_t key; uint32_t word, idx; };
+    std::vector<E> e;
+    e.reserve(forces_.size());
+    for (uint32_t i = 0; i < forces_.size(); i++) {
+      auto it = ranks_of.find(forces_[i].sig);
+      if (it == ranks_of.end()) continue;
+      const uint32_t gw = forces_[i].lane / 32;
+      for (uint32_t r : it->second) e.push_back({(uint64_t(gw / W) << 32) | r, gw % W, i});
+    }
+    std::sort(e.begin(), e.end(), [](const E& x, const E& y) {
+      return x.key != y.key ? x.key < y.key : x.word != y.word ? x.word < y.word : x.idx < y.idx;
+    });
+    std::vector<uint32_t> fent, fstart(tiles + 1, 0), fmid(tiles, 0);
+    for (size_t i = 0; i < e.size(); i++) {
+      if (i == 0 || e[i].key != e[i - 1].key || e[i].word != e[i - 1].word) {
+        const auto& ps = site[uint32_t(e[i].key)];
+        fent.insert(fent.end(), {ps.first, ps.second, e[i].word, ~0u, 0u});
+        fstart[uint32_t(e[i].key >> 32) + 1]++;
+      }
+      const Force& f = forces_[e[i].idx];
+      uint32_t* x = &fent[fent.size() - 5];
+      const uint32_t bit = 1u << (f.lane % 32);
+      x[3] &= ~bit;
+      if (f.value) x[4] |= bit; else x[4] &= ~bit;
+    }
+    if (fent.empty()) return;
+    for (uint32_t t = 0; t < tiles; t++) fstart[t + 1] += fstart[t];
+    for (uint32_t t = 0; t < tiles; t++) {
+      uint32_t m = fstart[t];
+      while (m < fstart[t + 1] && fent[size_t(m) * 5] != CLAMP_COMMIT) m++;
+      fmid[t] = m;
+    }
+    A.force = upload_prog(10, fent);
+    A.force_start = upload_prog(11, fstart);
+    A.force_mid = upload_prog(12, fmid);
   }
 
   // Chunks per tile for the work-queue schedule: the K in 2..16 whose rounds
@@ -1130,6 +1243,7 @@ class GateEngine final : public Engine {
   BatchJob batch_prepare(const RunSpec& spec) override {
     Prepared R = prepare(spec, true);
     batch_digest_ = R.A.digest != nullptr;
+    batch_detect_ = R.A.detect != nullptr;
     BatchJob J;
     J.kind = ENGINE_GATE;
     J.variant = R.W | (R.byte_addr ? GATE_VARIANT_BYTE_ADDR : 0u) | (R.M << GATE_VARIANT_M_SHIFT);
@@ -1144,6 +1258,7 @@ class GateEngine final : public Engine {
   }
   void batch_reset(hipStream_t s) override {
     if (batch_digest_) RRTL_HIP_CHECK(hipMemsetAsync(digest_.p, 0, digest_.n * 4, s));
+    if (batch_detect_) fill_u32(detect_.p, nwords_, 0u, s);
   }
   void batch_finish(const RunSpec& spec) override {
     stats_ = RunStats{};
@@ -1197,7 +1312,8 @@ class GateEngine final : public Engine {
   size_t lds_max_ = 65536;
   DevBuf<uint32_t> state_;
   DevBuf<uint4> settle_g_, next_g_, commit_g_;
-  DevBuf<uint32_t> fstart_, fent_, tile_force_, tile_fstart_, digest_, trace_;
+  DevBuf<uint32_t> fstart_, fent_, tile_force_, tile_fstart_, digest_, trace_, detect_;
+  DevBuf<uint64_t> group_key_;
   DevBuf<uint32_t> prog_bufs_[16];
   std::vector<Force> forces_;
   bool forces_dirty_ = true, wires_stale_ = false, has_digest_ = false;
@@ -1207,7 +1323,7 @@ class GateEngine final : public Engine {
   DevBuf<uint32_t> qctl_;  // work-queue counter, error flag, per-tile chunk counts
   Prepared plan_;
   std::string plan_key_;
-  bool have_plan_ = false, batch_digest_ = false;
+  bool have_plan_ = false, batch_digest_ = false, batch_detect_ = false, has_detect_ = false;
   uint64_t forces_version_ = 0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };

@@ -432,6 +432,9 @@ class WordEngine final : public Engine {
     A.cpool = cpool_.p; A.mem_table = mtab_.p; A.lanes = lanes_; A.flags = flags; A.cycles = cycles;
     A.halt_offset = NONE;
     if (spec && outputs) {
+      if (spec->detect) throw Error("E_HIP_UNSUPPORTED: fault detection runs on the bit-sliced gate engine");
+      for (uint32_t m : spec->in_mode)
+        if (m == IN_GROUP_RANDOM) throw Error("E_HIP_UNSUPPORTED: group-shared random inputs run on the bit-sliced gate engine");
       A.seed = spec->seed;
       std::vector<uint32_t> in_desc; std::vector<uint64_t> in_arg;
       uint32_t trace_limbs = 0;
