@@ -18,7 +18,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "librrtl_hip.so")
-SOURCES = ["gate_engine.hip", "word_engine.hip", "batch.hip", "capi.cpp"]
+SOURCES = ["gate_engine.hip", "word_engine.hip", "batch.hip", "mlp.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RRTL_OFFLOAD_ARCH", "gfx950")

@@ -78,6 +78,34 @@ def test_library_exports_every_header_symbol():
         assert hasattr(lib, n), n
 
 
+def test_library_exports_every_surrogate_symbol():
+    from rrtl_amd import surrogate
+    lib = backend.load_library()
+    hdr = open(os.path.join(ROOT, "include", "rrtl_mlp.h")).read()
+    names = set(re.findall(r"\b(rrtl_mlp_[a-z_0-9]+)\s*\(", hdr))
+    assert names, "no declarations parsed"
+    assert set(surrogate.EXPORTS) == names
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_surrogate_config_rejected_without_gpu_work():
+    """Shape checks run before any device call."""
+    from rrtl_amd import surrogate
+    lib = surrogate._lib()
+    c = surrogate.MlpConfig()
+    c.struct_size = ctypes.sizeof(c)
+    c.n_layers = 2
+    c.dims[0], c.dims[1], c.dims[2] = 100, 128, 1  # input width not a multiple of 128
+    c.max_batch = 128
+    h = ctypes.c_void_p()
+    assert lib.rrtl_mlp_create(ctypes.byref(c), ctypes.byref(h)) != 0
+    assert lib.rrtl_mlp_create_error().decode().startswith("E_MLP_CONFIG")
+    c.dims[0], c.dims[2] = 128, 2  # more than one output
+    assert lib.rrtl_mlp_create(ctypes.byref(c), ctypes.byref(h)) != 0
+    assert lib.rrtl_mlp_create_error().decode().startswith("E_MLP_CONFIG")
+
+
 def test_create_rejects_bad_blob_without_gpu_work():
     lib = backend.load_library()
     h = ctypes.c_void_p()
