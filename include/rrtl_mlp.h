@@ -73,18 +73,22 @@ rrtl_status rrtl_mlp_get_grads(rrtl_mlp* m, float* out);
 /* Logits of n host rows (any n). */
 rrtl_status rrtl_mlp_forward(rrtl_mlp* m, const uint16_t* x, uint64_t n, float* logits);
 
-/* Two-table rows: row r = concat(table_a[ia[r]], table_b[ib[r]]) with
- * da + db == dims[0].  The tables live on the device; rows are assembled on
- * the fly, so a dataset of N rows costs 8 bytes per row plus the tables. */
+/* Table rows: row r = concat(table_a[ia[r]], table_b[ib[r]], extra[r]) with
+ * da + db + de == dims[0] (each a multiple of 8; de may be 0, then extra is
+ * not read).  The tables live on the device and rows are assembled on the
+ * fly, so a dataset of N rows costs 8 + 2 de bytes per row plus the tables. */
 rrtl_status rrtl_mlp_set_tables(rrtl_mlp* m, const uint16_t* table_a, uint32_t na, uint32_t da,
-                                const uint16_t* table_b, uint32_t nb, uint32_t db);
-/* Trains `epochs` passes over n (ia, ib, y) rows in a seeded shuffled order,
- * batch rows per Adam step (multiple of 128, <= max_batch; the last partial
- * batch is used with its true size).  loss_per_epoch: epochs floats or null. */
-rrtl_status rrtl_mlp_train_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32_t* ib, const float* y, uint64_t n,
-                                 uint32_t epochs, uint32_t batch, uint64_t shuffle_seed, float* loss_per_epoch);
-/* Sigmoid outputs for n (ia, ib) rows. */
-rrtl_status rrtl_mlp_predict_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32_t* ib, uint64_t n, float* prob);
+                                const uint16_t* table_b, uint32_t nb, uint32_t db, uint32_t de);
+/* Trains `epochs` passes over n (ia, ib, extra, y) rows in a seeded shuffled
+ * order, batch rows per Adam step (multiple of 128, <= max_batch; the last
+ * partial batch is used with its true size).  extra: [n][de] bf16 patterns.
+ * loss_per_epoch: epochs floats or null. */
+rrtl_status rrtl_mlp_train_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32_t* ib, const uint16_t* extra,
+                                 const float* y, uint64_t n, uint32_t epochs, uint32_t batch, uint64_t shuffle_seed,
+                                 float* loss_per_epoch);
+/* Sigmoid outputs for n (ia, ib, extra) rows. */
+rrtl_status rrtl_mlp_predict_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32_t* ib, const uint16_t* extra,
+                                   uint64_t n, float* prob);
 rrtl_status rrtl_mlp_last_stats(const rrtl_mlp* m, rrtl_mlp_stats* out);
 /* The library's matrix-core product on its own: C[M][N] = A[M][K] . B[N][K]^T
  * (bf16 patterns in, fp32 out; M, N multiples of 128, K of 64), launched

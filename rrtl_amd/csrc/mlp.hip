@@ -261,29 +261,30 @@ __global__ void __launch_bounds__(256) transpose_kernel(const u16* in, uint32_t 
   (void)R;
 }
 
-// Batch assembly: row r of X = concat(ta[ia[p]], tb[ib[p]]) with p = perm[base + r]
-// (or base + r), zeros past n_valid; labels gathered alongside.
+// Batch assembly: row r of X = concat(ta[ia[p]], tb[ib[p]], ex[p]) with
+// p = perm[base + r] (or base + r), zeros past n_valid; labels gathered alongside.
 struct GatherArgs {
-  const u16* ta; const u16* tb; uint32_t da, db;
+  const u16* ta; const u16* tb; const u16* ex; uint32_t da, db, de;
   const uint32_t* ia; const uint32_t* ib; const float* y; const uint32_t* perm;
   uint64_t base; uint32_t n_valid, rows;
   u16* X; float* yb;
 };
 __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
-  const uint32_t w = (a.da + a.db) / 8u;
+  const uint32_t d = a.da + a.db + a.de, w = d / 8u;
   const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
   if (i >= uint64_t(a.rows) * w) return;
   const uint32_t r = uint32_t(i / w), c = uint32_t(i % w) * 8u;
   uint4 v = make_uint4(0, 0, 0, 0);
   if (r < a.n_valid) {
     const uint64_t p = a.perm ? a.perm[a.base + r] : a.base + r;
-    v = c < a.da ? *reinterpret_cast<const uint4*>(a.ta + size_t(a.ia[p]) * a.da + c)
-                 : *reinterpret_cast<const uint4*>(a.tb + size_t(a.ib[p]) * a.db + (c - a.da));
+    if (c < a.da) v = *reinterpret_cast<const uint4*>(a.ta + size_t(a.ia[p]) * a.da + c);
+    else if (c < a.da + a.db) v = *reinterpret_cast<const uint4*>(a.tb + size_t(a.ib[p]) * a.db + (c - a.da));
+    else v = *reinterpret_cast<const uint4*>(a.ex + size_t(p) * a.de + (c - a.da - a.db));
     if (c == 0 && a.yb) a.yb[r] = a.y ? a.y[p] : 0.f;
   } else if (c == 0 && a.yb) {
     a.yb[r] = 0.f;
   }
-  *reinterpret_cast<uint4*>(a.X + size_t(r) * (a.da + a.db) + c) = v;
+  *reinterpret_cast<uint4*>(a.X + size_t(r) * d + c) = v;
 }
 
 // Adam with decoupled weight decay over the flat parameter vector; writes the
@@ -432,22 +433,23 @@ class Mlp {
     end(steps);
   }
 
-  void set_tables(const u16* ta, uint32_t na, uint32_t da, const u16* tb, uint32_t nb, uint32_t db) {
-    if (da + db != cfg_.dims[0] || da % 8 || db % 8) throw Error("E_MLP_TABLES: widths must be multiples of 8 summing to the input width");
+  void set_tables(const u16* ta, uint32_t na, uint32_t da, const u16* tb, uint32_t nb, uint32_t db, uint32_t de) {
+    if (da + db + de != cfg_.dims[0] || da % 8 || db % 8 || de % 8)
+      throw Error("E_MLP_TABLES: widths must be multiples of 8 summing to the input width");
     if (!na || !nb) throw Error("E_MLP_TABLES: empty table");
     ta_.alloc(size_t(na) * da); tb_.alloc(size_t(nb) * db);
     RRTL_HIP_CHECK(hipMemcpyAsync(ta_.p, ta, size_t(na) * da * 2, hipMemcpyHostToDevice, s_));
     RRTL_HIP_CHECK(hipMemcpyAsync(tb_.p, tb, size_t(nb) * db * 2, hipMemcpyHostToDevice, s_));
-    na_ = na; nb_ = nb; da_ = da; db_ = db;
+    na_ = na; nb_ = nb; da_ = da; db_ = db; de_ = de;
     sync();
   }
 
-  void train_pairs(const uint32_t* ia, const uint32_t* ib, const float* y, uint64_t n, uint32_t epochs, uint32_t batch,
-                   uint64_t seed, float* loss_per_epoch) {
+  void train_pairs(const uint32_t* ia, const uint32_t* ib, const u16* ex, const float* y, uint64_t n, uint32_t epochs,
+                   uint32_t batch, uint64_t seed, float* loss_per_epoch) {
     if (!na_) throw Error("E_MLP_TABLES: no tables set");
     if (batch == 0 || batch % 128 || batch > cfg_.max_batch) throw Error("E_MLP_BATCH: batch must be a multiple of 128 up to max_batch");
     if (n == 0) throw Error("E_MLP_BATCH: no rows");
-    upload_pairs(ia, ib, y, n);
+    upload_pairs(ia, ib, ex, y, n);
     std::vector<uint32_t> perm(n);
     perm_.alloc(n);
     begin();
@@ -473,9 +475,9 @@ class Mlp {
     end(steps);
   }
 
-  void predict_pairs(const uint32_t* ia, const uint32_t* ib, uint64_t n, float* prob) {
+  void predict_pairs(const uint32_t* ia, const uint32_t* ib, const u16* ex, uint64_t n, float* prob) {
     if (!na_) throw Error("E_MLP_TABLES: no tables set");
-    upload_pairs(ia, ib, nullptr, n);
+    upload_pairs(ia, ib, ex, nullptr, n);
     out_.alloc(n);
     begin();
     uint64_t steps = 0;
@@ -538,13 +540,18 @@ class Mlp {
     stats_.steps = steps;
   }
 
-  void upload_pairs(const uint32_t* ia, const uint32_t* ib, const float* y, uint64_t n) {
+  void upload_pairs(const uint32_t* ia, const uint32_t* ib, const u16* ex, const float* y, uint64_t n) {
     for (uint64_t i = 0; i < n; i++)
       if (ia[i] >= na_ || ib[i] >= nb_) throw Error("E_MLP_TABLES: row index out of range");
     if (n > 0xFFFFFFFFull) throw Error("E_MLP_BATCH: too many rows");
+    if (de_ && !ex) throw Error("E_HIP_NULL: the tables declare per-row features but none were given");
     ia_.alloc(n); ib_.alloc(n);
     RRTL_HIP_CHECK(hipMemcpyAsync(ia_.p, ia, n * 4, hipMemcpyHostToDevice, s_));
     RRTL_HIP_CHECK(hipMemcpyAsync(ib_.p, ib, n * 4, hipMemcpyHostToDevice, s_));
+    if (de_) {
+      ex_.alloc(n * de_);
+      RRTL_HIP_CHECK(hipMemcpyAsync(ex_.p, ex, n * de_ * 2, hipMemcpyHostToDevice, s_));
+    }
     if (y) {
       y_.alloc(n);
       RRTL_HIP_CHECK(hipMemcpyAsync(y_.p, y, n * 4, hipMemcpyHostToDevice, s_));
@@ -554,11 +561,11 @@ class Mlp {
 
   void gather(const uint32_t* perm, uint64_t base, uint32_t nb, uint32_t rows, bool labels) {
     GatherArgs a{};
-    a.ta = ta_.p; a.tb = tb_.p; a.da = da_; a.db = db_;
+    a.ta = ta_.p; a.tb = tb_.p; a.ex = de_ ? ex_.p : nullptr; a.da = da_; a.db = db_; a.de = de_;
     a.ia = ia_.p; a.ib = ib_.p; a.y = labels ? y_.p : nullptr; a.perm = perm;
     a.base = base; a.n_valid = nb; a.rows = rows;
     a.X = H_[0].p; a.yb = labels ? yb_.p : nullptr;
-    const uint64_t work = uint64_t(rows) * (da_ + db_) / 8;
+    const uint64_t work = uint64_t(rows) * (da_ + db_ + de_) / 8;
     hipLaunchKernelGGL(gather_kernel, dim3(uint32_t((work + 255) / 256)), dim3(256), 0, s_, a);
     RRTL_HIP_CHECK(hipGetLastError());
     if (labels) launch_transpose(H_[0].p, cfg_.dims[0], HT_[0].p, rows, rows, cfg_.dims[0]);
@@ -657,10 +664,10 @@ class Mlp {
   Buf<u16> w16_, wt16_;
   Buf<u16> H_[RRTL_MLP_MAX_LAYERS], HT_[RRTL_MLP_MAX_LAYERS], dZ_[RRTL_MLP_MAX_LAYERS], dZT_[RRTL_MLP_MAX_LAYERS];
   Buf<float> logit_, dlogit_, yb_, scal_, y_, out_;
-  Buf<u16> ta_, tb_, ga_, gb_;
+  Buf<u16> ta_, tb_, ex_, ga_, gb_;
   Buf<float> gc_;
   Buf<uint32_t> ia_, ib_, perm_;
-  uint32_t na_ = 0, nb_ = 0, da_ = 0, db_ = 0;
+  uint32_t na_ = 0, nb_ = 0, da_ = 0, db_ = 0, de_ = 0;
   uint64_t t_ = 0;
   rrtl_mlp_stats stats_{};
 };
@@ -738,23 +745,25 @@ rrtl_status rrtl_mlp_forward(rrtl_mlp* m, const uint16_t* xin, uint64_t n, float
   });
 }
 rrtl_status rrtl_mlp_set_tables(rrtl_mlp* m, const uint16_t* ta, uint32_t na, uint32_t da, const uint16_t* tb,
-                                uint32_t nb, uint32_t db) {
+                                uint32_t nb, uint32_t db, uint32_t de) {
   return mlp_guard(m, [&](rrtl::mlp::Mlp& x) {
     if (!ta || !tb) throw rrtl::Error("E_HIP_NULL: null buffer");
-    x.set_tables(ta, na, da, tb, nb, db);
+    x.set_tables(ta, na, da, tb, nb, db, de);
   });
 }
-rrtl_status rrtl_mlp_train_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32_t* ib, const float* y, uint64_t n,
-                                 uint32_t epochs, uint32_t batch, uint64_t seed, float* loss_per_epoch) {
+rrtl_status rrtl_mlp_train_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32_t* ib, const uint16_t* extra,
+                                 const float* y, uint64_t n, uint32_t epochs, uint32_t batch, uint64_t seed,
+                                 float* loss_per_epoch) {
   return mlp_guard(m, [&](rrtl::mlp::Mlp& x) {
     if (!ia || !ib || !y) throw rrtl::Error("E_HIP_NULL: null buffer");
-    x.train_pairs(ia, ib, y, n, epochs, batch, seed, loss_per_epoch);
+    x.train_pairs(ia, ib, extra, y, n, epochs, batch, seed, loss_per_epoch);
   });
 }
-rrtl_status rrtl_mlp_predict_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32_t* ib, uint64_t n, float* prob) {
+rrtl_status rrtl_mlp_predict_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32_t* ib, const uint16_t* extra,
+                                   uint64_t n, float* prob) {
   return mlp_guard(m, [&](rrtl::mlp::Mlp& x) {
     if (n && (!ia || !ib || !prob)) throw rrtl::Error("E_HIP_NULL: null buffer");
-    x.predict_pairs(ia, ib, n, prob);
+    x.predict_pairs(ia, ib, extra, n, prob);
   });
 }
 rrtl_status rrtl_mlp_gemm_nt(rrtl_mlp* m, const uint16_t* a, const uint16_t* b, uint32_t M, uint32_t N, uint32_t K,

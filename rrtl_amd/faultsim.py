@@ -18,12 +18,12 @@ tick.  Every detection reported by this module comes from such a run.
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Optional, Sequence
+from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
 from .backend import IN_GROUP_RANDOM, HipSimulator
-from .program import KIND_INPUT, KIND_OUTPUT, KIND_REG, Program
+from .program import EF_CAPTURE, EF_STORE, KIND_INPUT, KIND_OUTPUT, KIND_REG, OP_LIT, OP_SIGNAL, Program
 
 FAULTS_PER_GROUP = 63  # lane 0 of each 64-lane group is the golden lane
 
@@ -33,6 +33,44 @@ def register_faults(program: Program) -> np.ndarray:
     (signal, stuck value) -- the fault list of fault_sim.rs:41-46."""
     regs = [i for i, s in enumerate(program.signals) if s.kind == KIND_REG]
     return np.array([(r, v) for r in regs for v in (0, 1)], dtype=np.uint32).reshape(-1, 2)
+
+
+def observable_signals(program: Program, observe: Optional[Sequence[int]] = None) -> np.ndarray:
+    """[signals] bool: the signal can influence an observed signal (default:
+    every output port) through stores and register captures, over any number
+    of cycles.  A register outside this set cannot be detected by any stimulus:
+    a fault on it never reaches what the comparison reads."""
+    n = len(program.signals)
+    srcs: List[set] = [set() for _ in range(n)]   # dst signal -> signals its value reads
+    for blk in program.blocks:
+        val: Dict[int, frozenset] = {}   # value ids are numbered per block
+        for pk in blk.packets:
+            for ins in pk.instrs:
+                if ins.op == OP_SIGNAL:
+                    val[ins.dst] = frozenset((ins.args[0],))
+                elif ins.op == OP_LIT:
+                    val[ins.dst] = frozenset()
+                else:
+                    acc = set()
+                    for a in ins.args:
+                        acc |= val.get(a, frozenset())
+                    val[ins.dst] = frozenset(acc)
+            for e in pk.effects:
+                if e.kind in (EF_STORE, EF_CAPTURE):
+                    srcs[e.dst] |= val.get(e.value, frozenset())
+                    if e.kind == EF_CAPTURE and e.reset is not None:
+                        srcs[e.dst].add(e.reset.signal)
+    seen = np.zeros(n, dtype=bool)
+    stack = list(observe) if observe is not None else [i for i, s in enumerate(program.signals) if s.kind == KIND_OUTPUT]
+    for s in stack:
+        seen[s] = True
+    while stack:
+        d = stack.pop()
+        for s in srcs[d]:
+            if not seen[s]:
+                seen[s] = True
+                stack.append(s)
+    return seen
 
 
 @dataclass
@@ -116,3 +154,20 @@ class FaultSim:
         finally:
             sim.close()
         return RunResult(flags[pk.lane], st["kernel_ms"], pk.groups, st["lane_cycles"])
+
+    def golden_register_bits(self, stim_ids: np.ndarray, regs: Sequence[int]):
+        """Fault-free value of each register after every tick under each
+        stimulus, from one launch with one 64-lane group per stimulus and the
+        registers traced: ([stimuli][registers][cycles] uint8, kernel ms)."""
+        ids = np.asarray(stim_ids, dtype=np.uint64)
+        regs = [int(r) for r in regs]
+        sim = HipSimulator(self.program, 64 * ids.size, engine="gate", **self.opts)
+        try:
+            sim.run(self.cycles, seed=self.seed, inputs=[(i, IN_GROUP_RANDOM, 0) for i in self.inputs],
+                    observe=regs, observe_post=True, trace=True, group_key=ids)
+            tr = sim.trace().reshape(self.cycles, len(regs), -1)
+            st = sim.stats()
+        finally:
+            sim.close()
+        bits = (tr[:, :, 0::2] & np.uint32(1)).astype(np.uint8)     # lane 0 of each group
+        return np.ascontiguousarray(bits.transpose(2, 1, 0)), st["kernel_ms"]

@@ -52,9 +52,9 @@ def _lib():
         L.rrtl_mlp_forward_backward.argtypes = [vp, vp, vp, u32, ctypes.POINTER(ctypes.c_float)]
         L.rrtl_mlp_get_grads.argtypes = [vp, vp]
         L.rrtl_mlp_forward.argtypes = [vp, vp, u64, vp]
-        L.rrtl_mlp_set_tables.argtypes = [vp, vp, u32, u32, vp, u32, u32]
-        L.rrtl_mlp_train_pairs.argtypes = [vp, vp, vp, vp, u64, u32, u32, u64, vp]
-        L.rrtl_mlp_predict_pairs.argtypes = [vp, vp, vp, u64, vp]
+        L.rrtl_mlp_set_tables.argtypes = [vp, vp, u32, u32, vp, u32, u32, u32]
+        L.rrtl_mlp_train_pairs.argtypes = [vp, vp, vp, vp, vp, u64, u32, u32, u64, vp]
+        L.rrtl_mlp_predict_pairs.argtypes = [vp, vp, vp, vp, u64, vp]
         L.rrtl_mlp_last_stats.argtypes = [vp, ctypes.POINTER(MlpStats)]
         L.rrtl_mlp_gemm_nt.argtypes = [vp, vp, vp, u32, u32, u32, vp, u32, ctypes.POINTER(ctypes.c_double)]
         _bound = True
@@ -129,27 +129,39 @@ class Surrogate:
         self._check(self.L.rrtl_mlp_forward(self.h, _ptr(x), x.shape[0], _ptr(out)))
         return out
 
-    def set_tables(self, table_a: np.ndarray, table_b: np.ndarray) -> None:
-        """Feature rows are concat(table_a[ia], table_b[ib]) (uint16 bf16 patterns)."""
+    def set_tables(self, table_a: np.ndarray, table_b: np.ndarray, extra_width: int = 0) -> None:
+        """Feature rows are concat(table_a[ia], table_b[ib], extra[row]) (uint16
+        bf16 patterns); ``extra_width`` columns come with each row."""
         ta = np.ascontiguousarray(table_a, dtype=np.uint16)
         tb = np.ascontiguousarray(table_b, dtype=np.uint16)
+        self.extra_width = extra_width
         self._check(self.L.rrtl_mlp_set_tables(self.h, _ptr(ta), ta.shape[0], ta.shape[1], _ptr(tb), tb.shape[0],
-                                               tb.shape[1]))
+                                               tb.shape[1], extra_width))
 
-    def train_pairs(self, ia, ib, y, epochs: int = 1, batch: int = 8192, seed: int = 1) -> np.ndarray:
+    def _extra(self, extra, n):
+        if not getattr(self, "extra_width", 0):
+            return None, None
+        ex = np.ascontiguousarray(extra, dtype=np.uint16)
+        if ex.shape != (n, self.extra_width):
+            raise ValueError("extra features must be [rows][extra_width]")
+        return ex, _ptr(ex)
+
+    def train_pairs(self, ia, ib, y, epochs: int = 1, batch: int = 8192, seed: int = 1, extra=None) -> np.ndarray:
         ia = np.ascontiguousarray(ia, dtype=np.uint32)
         ib = np.ascontiguousarray(ib, dtype=np.uint32)
         yv = np.ascontiguousarray(y, dtype=np.float32)
+        ex, exp = self._extra(extra, ia.size)
         losses = np.zeros(epochs, dtype=np.float32)
-        self._check(self.L.rrtl_mlp_train_pairs(self.h, _ptr(ia), _ptr(ib), _ptr(yv), ia.size, epochs, batch, seed,
-                                                _ptr(losses)))
+        self._check(self.L.rrtl_mlp_train_pairs(self.h, _ptr(ia), _ptr(ib), exp, _ptr(yv), ia.size, epochs, batch,
+                                                seed, _ptr(losses)))
         return losses
 
-    def predict_pairs(self, ia, ib) -> np.ndarray:
+    def predict_pairs(self, ia, ib, extra=None) -> np.ndarray:
         ia = np.ascontiguousarray(ia, dtype=np.uint32)
         ib = np.ascontiguousarray(ib, dtype=np.uint32)
+        ex, exp = self._extra(extra, ia.size)
         out = np.zeros(ia.size, dtype=np.float32)
-        self._check(self.L.rrtl_mlp_predict_pairs(self.h, _ptr(ia), _ptr(ib), ia.size, _ptr(out)))
+        self._check(self.L.rrtl_mlp_predict_pairs(self.h, _ptr(ia), _ptr(ib), exp, ia.size, _ptr(out)))
         return out
 
     def gemm_nt(self, a_bits: np.ndarray, b_bits: np.ndarray, iters: int = 1, want_c: bool = True):
