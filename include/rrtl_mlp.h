@@ -93,9 +93,11 @@ rrtl_status rrtl_mlp_last_stats(const rrtl_mlp* m, rrtl_mlp_stats* out);
 /* The library's matrix-core product on its own: C[M][N] = A[M][K] . B[N][K]^T
  * (bf16 patterns in, fp32 out; M, N multiples of 128, K of 64), launched
  * `iters` times on device copies of A and B; c (null: skip the copy back)
- * receives C; ms_per_launch the mean device time of one launch. */
+ * receives C; ms_per_launch the mean device time of one launch (split-K
+ * reduction included).  tile: 0 automatic, 1 = 128 x 128 workgroup tiles,
+ * 2 = 256 x 128 (M a multiple of 256). */
 rrtl_status rrtl_mlp_gemm_nt(rrtl_mlp* m, const uint16_t* a, const uint16_t* b, uint32_t M, uint32_t N, uint32_t K,
-                             float* c, uint32_t iters, double* ms_per_launch);
+                             uint32_t tile, float* c, uint32_t iters, double* ms_per_launch);
 
 #ifdef __cplusplus
 }

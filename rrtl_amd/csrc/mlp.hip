@@ -26,6 +26,7 @@ namespace mlp {
 typedef uint16_t u16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // fp32 -> bf16 pattern, round to nearest even (finite inputs)
 __host__ __device__ inline u16 f2bf(float f) {
@@ -36,12 +37,16 @@ __host__ __device__ inline u16 f2bf(float f) {
 __host__ __device__ inline float bf2f(u16 h) { return __builtin_bit_cast(float, uint32_t(h) << 16); }
 
 // ------------------------------------------------------------------ GEMM
-// Workgroup tile 128 x 128, K step 64, four waves in a 2 x 2 grid each owning
-// a 64 x 64 block as 2 x 2 MFMA 32x32 tiles.  LDS rows are padded to 72 bf16
-// (144 bytes): the eight lanes of a 128-byte LDS pass read rows 36 banks
-// apart, which covers all 64 banks once.  The next K step's global loads are
-// in flight while the current one computes.
-constexpr uint32_t BM = 128, BN = 128, BK = 64, LDK = BK + 8, GT = 256;
+// Four waves in a 2 x 2 grid; each wave owns TM x TN MFMA 32x32 tiles, so the
+// workgroup tile is (64 TM) x (64 TN) with K steps of 64: 128 x 128 (TM = TN =
+// 2) for short products, 256 x 128 (TM = 4) when M is large -- per 16-deep K
+// slice a wave then reads 6 fragments for 8 MFMAs instead of 4 for 4.  LDS
+// rows are padded to 72 bf16 (144 bytes): the eight lanes of a 128-byte LDS
+// pass read rows 36 banks apart, which covers all 64 banks once.  The next K
+// step's global loads are in flight while the current one computes.  With
+// gridDim.y > 1 each y-slice takes k_chunk of K and writes its own fp32
+// partial product (split K, summed by splitk_reduce_kernel).
+constexpr uint32_t BK = 64, LDK = BK + 8, GT = 256;
 enum : int { EPI_RELU = 0, EPI_DRELU = 1, EPI_F32 = 2 };
 
 struct GemmArgs {
@@ -52,79 +57,85 @@ struct GemmArgs {
   u16* out; uint32_t ldo;      // bf16 [M][ldo] (EPI_RELU, EPI_DRELU)
   u16* out_t; uint32_t ldt;    // bf16 [N][ldt] transposed copy, or null
   float* out_f; uint32_t ldf;  // EPI_F32: [M][ldf]
+  uint32_t k_chunk;            // K per y-slice
+  size_t split_stride;         // EPI_F32: elements between the y-slices' partial products
 };
 
-template <int EPI>
-__global__ void __launch_bounds__(GT) gemm_nt_kernel(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) u16 sA[BM * LDK];
-  __shared__ __attribute__((aligned(16))) u16 sB[BN * LDK];
+template <int EPI, int TM, int TN>
+__global__ void __launch_bounds__(GT, 2) gemm_nt_kernel(GemmArgs g) {
+  constexpr uint32_t BMt = 64 * TM, BNt = 64 * TN, NA = BMt / 32, NB = BNt / 32;
+  __shared__ __attribute__((aligned(16))) u16 sA[BMt * LDK];
+  __shared__ __attribute__((aligned(16))) u16 sB[BNt * LDK];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint32_t tiles_n = g.N / BN, nb = gridDim.x;
+  const uint32_t tiles_n = g.N / BNt, nb = gridDim.x;
   uint32_t bid = blockIdx.x;
   // Workgroups are dealt round-robin over the 8 XCDs; renumber so each XCD
   // (its own L2) works through a contiguous run of tiles sharing A rows.
   if (nb % 8u == 0) bid = (bid % 8u) * (nb / 8u) + bid / 8u;
-  const uint32_t m0 = (bid / tiles_n) * BM, n0 = (bid % tiles_n) * BN;
+  const uint32_t m0 = (bid / tiles_n) * BMt, n0 = (bid % tiles_n) * BNt;
+  const uint32_t kbeg = blockIdx.y * g.k_chunk, kend = min(g.K, kbeg + g.k_chunk);
   const uint32_t wm = wave >> 1, wn = wave & 1u;
   const uint32_t crow = tid >> 3, cch = (tid & 7u) * 8u;  // copy: 16-byte chunk cch of rows crow + 32 i
-  // this thread's copy sources: rows crow + 32 i of the A and B tiles
   const u16* pa = g.A + size_t(m0 + crow) * g.lda + cch;
   const u16* pb = g.B + size_t(n0 + crow) * g.ldb + cch;
   const size_t sa = size_t(32) * g.lda, sb = size_t(32) * g.ldb;
-  uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define RRTL_GEMM_GLOAD(k0)                                                                       \
-  {                                                                                               \
-    ra0 = *reinterpret_cast<const uint4*>(pa + (k0)); rb0 = *reinterpret_cast<const uint4*>(pb + (k0));   \
-    ra1 = *reinterpret_cast<const uint4*>(pa + sa + (k0)); rb1 = *reinterpret_cast<const uint4*>(pb + sb + (k0)); \
-    ra2 = *reinterpret_cast<const uint4*>(pa + 2 * sa + (k0)); rb2 = *reinterpret_cast<const uint4*>(pb + 2 * sb + (k0)); \
-    ra3 = *reinterpret_cast<const uint4*>(pa + 3 * sa + (k0)); rb3 = *reinterpret_cast<const uint4*>(pb + 3 * sb + (k0)); \
-  }
-  f32x16 acc[2][2];
+  u32x4 ra[NA], rb[NB];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (uint32_t i = 0; i < NA; i++) ra[i] = *reinterpret_cast<const u32x4*>(pa + i * sa + kbeg);
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+  for (uint32_t i = 0; i < NB; i++) rb[i] = *reinterpret_cast<const u32x4*>(pb + i * sb + kbeg);
+  f32x16 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
-  RRTL_GEMM_GLOAD(0)
   const uint32_t frow = lane & 31u, fk = (lane >> 5) * 8u;
-  for (uint32_t k0 = 0; k0 < g.K; k0 += BK) {
+  u32x4* da = reinterpret_cast<u32x4*>(&sA[crow * LDK + cch]);
+  u32x4* db = reinterpret_cast<u32x4*>(&sB[crow * LDK + cch]);
+  constexpr uint32_t R32 = 32u * LDK / 8u;  // 32 LDS rows in uint4 units
+  for (uint32_t k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();  // the previous step's fragment reads are done
-    {
-      uint4* da = reinterpret_cast<uint4*>(&sA[crow * LDK + cch]);
-      uint4* db = reinterpret_cast<uint4*>(&sB[crow * LDK + cch]);
-      constexpr uint32_t R32 = 32u * LDK / 8u;  // 32 rows in uint4 units
-      da[0] = ra0; da[R32] = ra1; da[2 * R32] = ra2; da[3 * R32] = ra3;
-      db[0] = rb0; db[R32] = rb1; db[2 * R32] = rb2; db[3 * R32] = rb3;
-    }
+#pragma unroll
+    for (uint32_t i = 0; i < NA; i++) da[i * R32] = ra[i];
+#pragma unroll
+    for (uint32_t i = 0; i < NB; i++) db[i * R32] = rb[i];
     __syncthreads();
-    if (k0 + BK < g.K) { RRTL_GEMM_GLOAD(k0 + BK) }
+    if (k0 + BK < kend) {
+#pragma unroll
+      for (uint32_t i = 0; i < NA; i++) ra[i] = *reinterpret_cast<const u32x4*>(pa + i * sa + k0 + BK);
+#pragma unroll
+      for (uint32_t i = 0; i < NB; i++) rb[i] = *reinterpret_cast<const u32x4*>(pb + i * sb + k0 + BK);
+    }
 #pragma unroll
     for (uint32_t kk = 0; kk < BK; kk += 16) {
-      bf16x8 a[2], b[2];
+      bf16x8 a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < 2; i++) a[i] = *reinterpret_cast<const bf16x8*>(&sA[(wm * 64u + i * 32u + frow) * LDK + kk + fk]);
+      for (int i = 0; i < TM; i++)
+        a[i] = *reinterpret_cast<const bf16x8*>(&sA[(wm * (32u * TM) + i * 32u + frow) * LDK + kk + fk]);
 #pragma unroll
-      for (int j = 0; j < 2; j++) b[j] = *reinterpret_cast<const bf16x8*>(&sB[(wn * 64u + j * 32u + frow) * LDK + kk + fk]);
+      for (int j = 0; j < TN; j++)
+        b[j] = *reinterpret_cast<const bf16x8*>(&sB[(wn * (32u * TN) + j * 32u + frow) * LDK + kk + fk]);
 #pragma unroll
-      for (int i = 0; i < 2; i++)
+      for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-#undef RRTL_GEMM_GLOAD
   // C tile layout: lane holds column (lane & 31) and rows 8 (r / 4) + 4 (lane / 32) + r % 4
+  float* out_f = g.out_f + (EPI == EPI_F32 ? blockIdx.y * g.split_stride : 0);
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < TM; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const uint32_t n = n0 + wn * 64u + j * 32u + (lane & 31u);
+    for (int j = 0; j < TN; j++) {
+      const uint32_t n = n0 + wn * (32u * TN) + j * 32u + (lane & 31u);
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const uint32_t mq = m0 + wm * 64u + i * 32u + 8u * q + 4u * (lane >> 5);
+        const uint32_t mq = m0 + wm * (32u * TM) + i * 32u + 8u * q + 4u * (lane >> 5);
         if constexpr (EPI == EPI_F32) {
 #pragma unroll
-          for (int r = 0; r < 4; r++) g.out_f[size_t(mq + r) * g.ldf + n] = acc[i][j][4 * q + r];
+          for (int r = 0; r < 4; r++) out_f[size_t(mq + r) * g.ldf + n] = acc[i][j][4 * q + r];
         } else {
           u16 h[4];
 #pragma unroll
@@ -150,6 +161,19 @@ __global__ void __launch_bounds__(GT) gemm_nt_kernel(GemmArgs g) {
     }
 }
 
+// out[i] = sum over s < splits of part[s * stride + i], i < n (n, stride multiples of 4)
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(const float* part, size_t stride, uint32_t splits, float* out,
+                                                            size_t n) {
+  for (size_t i = (size_t(blockIdx.x) * 256u + threadIdx.x) * 4u; i < n; i += size_t(gridDim.x) * 1024u) {
+    float4 a = *reinterpret_cast<const float4*>(part + i);
+    for (uint32_t s = 1; s < splits; s++) {
+      const float4 b = *reinterpret_cast<const float4*>(part + s * stride + i);
+      a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
+    }
+    *reinterpret_cast<float4*>(out + i) = a;
+  }
+}
+
 // ------------------------------------------------------------------ small kernels
 // Output layer and loss: one wave per row, logit = H[row] . w + b; with labels,
 // the mean binary cross-entropy and its logit gradient (rows >= n_valid are
@@ -163,34 +187,47 @@ struct HeadArgs {
   float* loss_sum;     // accumulated sum of row losses when y
 };
 __global__ void __launch_bounds__(256) head_fwd_kernel(HeadArgs h) {
-  const uint32_t lane = threadIdx.x & 63u, row = blockIdx.x * 4u + (threadIdx.x >> 6);
-  if (row >= h.rows) return;
-  float s = 0.f;
-  for (uint32_t k = lane * 8u; k < h.K; k += 512u) {
-    const uint4 hv = *reinterpret_cast<const uint4*>(h.H + size_t(row) * h.ldh + k);
-    const uint4 wv = *reinterpret_cast<const uint4*>(h.w + k);
-    const uint32_t hx[4] = {hv.x, hv.y, hv.z, hv.w}, wx[4] = {wv.x, wv.y, wv.z, wv.w};
+  // 64 rows per workgroup, 16 per wave; the loss is summed in the workgroup
+  // and added to the total once
+  __shared__ float part[4];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  float loss = 0.f;
+  for (uint32_t q = 0; q < 16u; q++) {
+    const uint32_t row = blockIdx.x * 64u + wave * 16u + q;
+    if (row >= h.rows) break;
+    float s = 0.f;
+    for (uint32_t k = lane * 8u; k < h.K; k += 512u) {
+      const uint4 hv = *reinterpret_cast<const uint4*>(h.H + size_t(row) * h.ldh + k);
+      const uint4 wv = *reinterpret_cast<const uint4*>(h.w + k);
+      const uint32_t hx[4] = {hv.x, hv.y, hv.z, hv.w}, wx[4] = {wv.x, wv.y, wv.z, wv.w};
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      s = fmaf(bf2f(u16(hx[q])), bf2f(u16(wx[q])), s);
-      s = fmaf(bf2f(u16(hx[q] >> 16)), bf2f(u16(wx[q] >> 16)), s);
+      for (int t = 0; t < 4; t++) {
+        s = fmaf(bf2f(u16(hx[t])), bf2f(u16(wx[t])), s);
+        s = fmaf(bf2f(u16(hx[t] >> 16)), bf2f(u16(wx[t] >> 16)), s);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    const float l = s + h.b[0];
+    if (lane == 0) {
+      h.logit[row] = l;
+      if (h.y) {
+        if (row < h.n_valid) {
+          const float y = h.y[row];
+          const float p = 1.f / (1.f + expf(-l));
+          h.dlogit[row] = (p - y) / float(h.n_valid);
+          // softplus(l) - y l, evaluated without overflow
+          loss += fmaxf(l, 0.f) - y * l + log1pf(expf(-fabsf(l)));
+        } else {
+          h.dlogit[row] = 0.f;
+        }
+      }
     }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane != 0) return;
-  const float l = s + h.b[0];
-  h.logit[row] = l;
   if (!h.y) return;
-  if (row < h.n_valid) {
-    const float y = h.y[row];
-    const float p = 1.f / (1.f + expf(-l));
-    h.dlogit[row] = (p - y) / float(h.n_valid);
-    // softplus(l) - y l, evaluated without overflow
-    atomicAdd(h.loss_sum, fmaxf(l, 0.f) - y * l + log1pf(expf(-fabsf(l))));
-  } else {
-    h.dlogit[row] = 0.f;
-  }
+  if (lane == 0) part[wave] = loss;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(h.loss_sum, part[0] + part[1] + part[2] + part[3]);
 }
 
 // dZ[row][k] = dlogit[row] * w[k] * [H[row][k] > 0], bf16; 8 columns per thread
@@ -494,15 +531,17 @@ class Mlp {
 
   // C[M][N] = A[M][K] . B[N][K]^T in fp32 through the matrix-core kernel,
   // launched iters times on device copies; returns the mean launch time.
-  double gemm_nt(const u16* a, const u16* b, uint32_t M, uint32_t N, uint32_t K, float* c, uint32_t iters) {
-    if (!M || !N || !K || M % BM || N % BN || K % BK) throw Error("E_MLP_GEMM: M and N must be multiples of 128, K of 64");
+  double gemm_nt(const u16* a, const u16* b, uint32_t M, uint32_t N, uint32_t K, float* c, uint32_t iters,
+                 uint32_t tile) {
+    if (tile > 2) throw Error("E_MLP_GEMM: tile must be 0 (automatic), 1 (128 x 128) or 2 (256 x 128)");
+    if (!M || !N || !K || M % 128 || N % 128 || K % BK) throw Error("E_MLP_GEMM: M and N must be multiples of 128, K of 64");
     if (iters == 0) iters = 1;
     ga_.alloc(size_t(M) * K); gb_.alloc(size_t(N) * K); gc_.alloc(size_t(M) * N);
     RRTL_HIP_CHECK(hipMemcpyAsync(ga_.p, a, size_t(M) * K * 2, hipMemcpyHostToDevice, s_));
     RRTL_HIP_CHECK(hipMemcpyAsync(gb_.p, b, size_t(N) * K * 2, hipMemcpyHostToDevice, s_));
     begin();
     for (uint32_t i = 0; i < iters; i++)
-      gemm(EPI_F32, ga_.p, K, gb_.p, K, M, N, K, nullptr, nullptr, nullptr, 0, nullptr, 0, gc_.p, N);
+      gemm(EPI_F32, ga_.p, K, gb_.p, K, M, N, K, nullptr, nullptr, nullptr, 0, nullptr, 0, gc_.p, N, tile);
     end(iters);
     if (c) copy_out(c, gc_.p, size_t(M) * N);
     return stats_.gemm_ms / iters;
@@ -573,9 +612,22 @@ class Mlp {
 
   void gemm(int epi, const u16* A, uint32_t lda, const u16* B, uint32_t ldb, uint32_t M, uint32_t N, uint32_t K,
             const float* bias, const u16* act, u16* out, uint32_t ldo, u16* out_t, uint32_t ldt, float* out_f,
-            uint32_t ldf) {
-    if (M % BM || N % BN || K % BK || lda % 8 || ldb % 8) throw Error("E_MLP_INTERNAL: GEMM shape");
-    GemmArgs g{A, B, M, N, K, lda, ldb, bias, act, out, ldo, out_t, ldt, out_f, ldf};
+            uint32_t ldf, uint32_t tile = 0) {
+    if (M % 128 || N % 128 || K % BK || lda % 8 || ldb % 8) throw Error("E_MLP_INTERNAL: GEMM shape");
+    // 256 x 128 tiles when there are enough of them to fill the device
+    const bool wide = tile ? tile == 2 : (M % 256 == 0 && uint64_t(M / 256) * (N / 128) >= 1024);
+    if (wide && M % 256) throw Error("E_MLP_GEMM: 256-row tiles need M to be a multiple of 256");
+    const uint32_t tiles = wide ? (M / 256) * (N / 128) : (M / 128) * (N / 128);
+    // split K when the tiles alone cannot occupy the CUs (fp32 products only)
+    uint32_t splits = 1;
+    if (epi == EPI_F32)
+      while (tiles * splits < 512 && K / (splits * 2) >= 4 * BK && (K / (splits * 2)) % BK == 0) splits *= 2;
+    GemmArgs g{A, B, M, N, K, lda, ldb, bias, act, out, ldo, out_t, ldt, out_f, ldf, K / splits, 0};
+    float* final_out = out_f;
+    if (splits > 1) {
+      gsplit_.alloc(size_t(splits) * M * N);
+      g.out_f = gsplit_.p; g.ldf = N; g.split_stride = size_t(M) * N;
+    }
     if (n_gemm_ == gemm_ev_.size()) {
       hipEvent_t a, b;
       RRTL_HIP_CHECK(hipEventCreate(&a));
@@ -583,12 +635,25 @@ class Mlp {
       gemm_ev_.push_back({a, b});
     }
     auto& ev = gemm_ev_[n_gemm_++];
-    const dim3 grid((M / BM) * (N / BN));
+    const dim3 grid(tiles, splits);
     RRTL_HIP_CHECK(hipEventRecord(ev.first, s_));
-    if (epi == EPI_RELU) hipLaunchKernelGGL(gemm_nt_kernel<EPI_RELU>, grid, dim3(GT), 0, s_, g);
-    else if (epi == EPI_DRELU) hipLaunchKernelGGL(gemm_nt_kernel<EPI_DRELU>, grid, dim3(GT), 0, s_, g);
-    else hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, dim3(GT), 0, s_, g);
+    if (wide) {
+      if (epi == EPI_RELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_RELU, 4, 2>), grid, dim3(GT), 0, s_, g);
+      else if (epi == EPI_DRELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_DRELU, 4, 2>), grid, dim3(GT), 0, s_, g);
+      else hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 4, 2>), grid, dim3(GT), 0, s_, g);
+    } else {
+      if (epi == EPI_RELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_RELU, 2, 2>), grid, dim3(GT), 0, s_, g);
+      else if (epi == EPI_DRELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_DRELU, 2, 2>), grid, dim3(GT), 0, s_, g);
+      else hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 2, 2>), grid, dim3(GT), 0, s_, g);
+    }
     RRTL_HIP_CHECK(hipGetLastError());
+    if (splits > 1) {
+      if (ldf != N) throw Error("E_MLP_INTERNAL: split-K output must be dense");
+      const size_t n = size_t(M) * N;
+      hipLaunchKernelGGL(splitk_reduce_kernel, dim3(uint32_t(std::min<size_t>((n / 4 + 255) / 256, 2048))), dim3(256), 0,
+                         s_, gsplit_.p, n, splits, final_out, n);
+      RRTL_HIP_CHECK(hipGetLastError());
+    }
     RRTL_HIP_CHECK(hipEventRecord(ev.second, s_));
     stats_.gemm_flop += 2.0 * M * N * K;
   }
@@ -614,7 +679,7 @@ class Mlp {
     h.H = H_[L_ - 1].p; h.ldh = d[L_ - 1]; h.K = d[L_ - 1]; h.rows = rows; h.n_valid = n_valid;
     h.w = w16_.p + woff_[L_ - 1]; h.b = w_.p + boff_[L_ - 1];
     h.y = labels ? yb_.p : nullptr; h.logit = logit_.p; h.dlogit = dlogit_.p; h.loss_sum = scal_.p;
-    hipLaunchKernelGGL(head_fwd_kernel, dim3(rows / 4), dim3(256), 0, s_, h);
+    hipLaunchKernelGGL(head_fwd_kernel, dim3((rows + 63) / 64), dim3(256), 0, s_, h);
     RRTL_HIP_CHECK(hipGetLastError());
   }
 
@@ -665,7 +730,7 @@ class Mlp {
   Buf<u16> H_[RRTL_MLP_MAX_LAYERS], HT_[RRTL_MLP_MAX_LAYERS], dZ_[RRTL_MLP_MAX_LAYERS], dZT_[RRTL_MLP_MAX_LAYERS];
   Buf<float> logit_, dlogit_, yb_, scal_, y_, out_;
   Buf<u16> ta_, tb_, ex_, ga_, gb_;
-  Buf<float> gc_;
+  Buf<float> gc_, gsplit_;
   Buf<uint32_t> ia_, ib_, perm_;
   uint32_t na_ = 0, nb_ = 0, da_ = 0, db_ = 0, de_ = 0;
   uint64_t t_ = 0;
@@ -767,10 +832,10 @@ rrtl_status rrtl_mlp_predict_pairs(rrtl_mlp* m, const uint32_t* ia, const uint32
   });
 }
 rrtl_status rrtl_mlp_gemm_nt(rrtl_mlp* m, const uint16_t* a, const uint16_t* b, uint32_t M, uint32_t N, uint32_t K,
-                             float* c, uint32_t iters, double* ms_per_launch) {
+                             uint32_t tile, float* c, uint32_t iters, double* ms_per_launch) {
   return mlp_guard(m, [&](rrtl::mlp::Mlp& x) {
     if (!a || !b) throw rrtl::Error("E_HIP_NULL: null buffer");
-    const double ms = x.gemm_nt(a, b, M, N, K, c, iters);
+    const double ms = x.gemm_nt(a, b, M, N, K, c, iters, tile);
     if (ms_per_launch) *ms_per_launch = ms;
   });
 }

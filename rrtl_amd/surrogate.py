@@ -56,7 +56,7 @@ def _lib():
         L.rrtl_mlp_train_pairs.argtypes = [vp, vp, vp, vp, vp, u64, u32, u32, u64, vp]
         L.rrtl_mlp_predict_pairs.argtypes = [vp, vp, vp, vp, u64, vp]
         L.rrtl_mlp_last_stats.argtypes = [vp, ctypes.POINTER(MlpStats)]
-        L.rrtl_mlp_gemm_nt.argtypes = [vp, vp, vp, u32, u32, u32, vp, u32, ctypes.POINTER(ctypes.c_double)]
+        L.rrtl_mlp_gemm_nt.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, u32, ctypes.POINTER(ctypes.c_double)]
         _bound = True
     return L
 
@@ -164,8 +164,9 @@ class Surrogate:
         self._check(self.L.rrtl_mlp_predict_pairs(self.h, _ptr(ia), _ptr(ib), exp, ia.size, _ptr(out)))
         return out
 
-    def gemm_nt(self, a_bits: np.ndarray, b_bits: np.ndarray, iters: int = 1, want_c: bool = True):
+    def gemm_nt(self, a_bits: np.ndarray, b_bits: np.ndarray, iters: int = 1, want_c: bool = True, tile: int = 0):
         """C = A . B^T on the matrix cores (A [M][K], B [N][K] bf16 patterns).
+        tile: 0 automatic, 1 = 128 x 128, 2 = 256 x 128 workgroup tiles.
         Returns (C or None, mean ms per launch)."""
         a = np.ascontiguousarray(a_bits, dtype=np.uint16)
         b = np.ascontiguousarray(b_bits, dtype=np.uint16)
@@ -175,8 +176,8 @@ class Surrogate:
             raise ValueError("inner dimensions differ")
         c = np.zeros((M, N), dtype=np.float32) if want_c else None
         ms = ctypes.c_double()
-        self._check(self.L.rrtl_mlp_gemm_nt(self.h, _ptr(a), _ptr(b), M, N, K, _ptr(c) if want_c else None, iters,
-                                            ctypes.byref(ms)))
+        self._check(self.L.rrtl_mlp_gemm_nt(self.h, _ptr(a), _ptr(b), M, N, K, tile, _ptr(c) if want_c else None,
+                                            iters, ctypes.byref(ms)))
         return c, ms.value
 
     def stats(self) -> dict:

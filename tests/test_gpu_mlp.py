@@ -16,8 +16,9 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("M,N,K", [(128, 128, 64), (512, 1024, 512), (1024, 512, 4096), (2048, 1024, 1024)])
-def test_gemm_matches_float64_product(M, N, K):
+@pytest.mark.parametrize("M,N,K,tile", [(128, 128, 64, 0), (512, 1024, 512, 0), (1024, 512, 4096, 0),
+                                        (2048, 1024, 1024, 0), (512, 256, 1024, 2), (256, 128, 8192, 2)])
+def test_gemm_matches_float64_product(M, N, K, tile):
     """C = A . B^T of bf16 operands: every element within the fp32
     summation bound K * 2^-24 * sum |a_k b_k| of the float64 product."""
     flush_print(f"case: gemm {M}x{N}x{K}")
@@ -25,7 +26,7 @@ def test_gemm_matches_float64_product(M, N, K):
     a = mlp_ref.bf16_bits(rng.normal(size=(M, K)).astype(np.float32))
     b = mlp_ref.bf16_bits(rng.normal(size=(N, K)).astype(np.float32))
     s = Surrogate([128, 128, 1], max_batch=128)
-    c, ms = s.gemm_nt(a, b, iters=2)
+    c, ms = s.gemm_nt(a, b, iters=2, tile=tile)
     A = mlp_ref.from_bf16_bits(a).astype(np.float64)
     B = mlp_ref.from_bf16_bits(b).astype(np.float64)
     want = A @ B.T
