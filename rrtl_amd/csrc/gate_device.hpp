@@ -171,7 +171,10 @@ __device__ __forceinline__ void apply_clamps(const TileArgs& A, uint32_t* lds, u
 
 // Signature of the observed outputs: per cycle c, S[(j + c) & 63] ^= v_j for every
 // observed word v_j, then one nonlinear round S'[p] = S[p] ^ (S[p+17] & ~S[p+41]).
-// Bitwise, so every lane keeps its own 64-bit signature.
+// Bitwise, so every lane keeps its own 64-bit signature.  Lane p of wave 0
+// holds S[p] (all W words) in registers for the whole tile run; the round
+// reads S[p+17] and S[p+41] from other lanes of the same wave with cross-lane
+// moves, so it needs no barrier.
 //
 // Detection (fault simulation): lane 0 of every 64-lane group (bit 0 of the
 // group's even u32 word) is the reference lane; a lane is flagged once any
@@ -180,8 +183,8 @@ __device__ __forceinline__ void apply_clamps(const TileArgs& A, uint32_t* lds, u
 // flags accumulate in the thread's registers (thread j owns observed signals
 // j, j + 256, ...) and reach HBM once per tile run.
 template <int W>
-__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32_t* sig, uint64_t c, uint64_t cyc, uint64_t w0,
-                                        Words<W>& det) {
+__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, Words<W>& sreg, uint64_t c, uint64_t cyc,
+                                        uint64_t w0, Words<W>& det) {
   const uint32_t tid = threadIdx.x;
   if (A.detect)
     for (uint32_t j = tid; j < A.n_obs; j += TILE_THREADS) {
@@ -196,28 +199,22 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, uint32
       const uint32_t o = A.obs_slot[j];
       if (gw < A.nwords) A.trace[(c * A.n_obs + j) * A.nwords + gw] = lds[(o & 0x7FFFFFFFu) * W + w] ^ (0u - (o >> 31));
     }
-  if (A.digest) {
-    Words<W> s;
-    if (tid < 64) {
-      s = ld_words<W>(sig, tid);
-      for (uint32_t j = (tid - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) {
-        const uint32_t o = A.obs_slot[j];
-        const Words<W> v = ld_words<W>(lds, o & 0x7FFFFFFFu);
-        const uint32_t inv = 0u - (o >> 31);
+  if (A.digest && tid < 64) {
+    for (uint32_t j = (tid - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) {
+      const uint32_t o = A.obs_slot[j];
+      const Words<W> v = ld_words<W>(lds, o & 0x7FFFFFFFu);
+      const uint32_t inv = 0u - (o >> 31);
 #pragma unroll
-        for (int k = 0; k < W; k++) s.v[k] ^= v.v[k] ^ inv;
-      }
-      st_words<W>(sig, tid, s);
+      for (int k = 0; k < W; k++) sreg.v[k] ^= v.v[k] ^ inv;
     }
-    __syncthreads();
-    if (tid < 64) {
-      const Words<W> a = ld_words<W>(sig, (tid + 17) & 63), b = ld_words<W>(sig, (tid + 41) & 63);
+    const int pa = int((tid + 17) & 63), pb = int((tid + 41) & 63);
 #pragma unroll
-      for (int k = 0; k < W; k++) s.v[k] ^= a.v[k] & ~b.v[k];
+    for (int k = 0; k < W; k++) {
+      const uint32_t a = uint32_t(__shfl(int(sreg.v[k]), pa, 64)), b = uint32_t(__shfl(int(sreg.v[k]), pb, 64));
+      sreg.v[k] ^= a & ~b;
     }
-    __syncthreads();
-    if (tid < 64) st_words<W>(sig, tid, s);
   }
+  // the commit (or the next cycle's inputs) may overwrite an observed slot
   __syncthreads();
 }
 
@@ -355,6 +352,10 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
     sig[i] = (A.digest && gw < A.nwords) ? A.digest[p * A.nwords + gw] : 0u;
   }
   __syncthreads();
+  Words<W> sreg;  // wave 0: signature word S[tid] for the run
+#pragma unroll
+  for (int k = 0; k < W; k++) sreg.v[k] = 0u;
+  if (A.digest && tid < 64) sreg = ld_words<W>(sig, tid);
   for (uint64_t c = 0; c < A.cycles; c++) {
     const uint64_t cyc = A.cycle_base + c;
     // primary inputs for this cycle (thread per input, all words of the tile)
@@ -362,22 +363,25 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
     __syncthreads();
     // cycle program, observe before the edge, commit, post program
     run_program<W, kBA, M>(A, lds, 0, A.n_cycle_stages, fb, fm);
-    if (!A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0, det);
+    if (!A.observe_post && A.n_obs) observe<W>(A, lds, sreg, c, cyc, w0, det);
     commit_regs<W>(A, lds, fm, fe);
     if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fm);
-    if (A.observe_post && A.n_obs) observe<W>(A, lds, sig, c, cyc, w0, det);
+    if (A.observe_post && A.n_obs) observe<W>(A, lds, sreg, c, cyc, w0, det);
   }
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
     uint32_t s = i / W, w = i % W;
     uint64_t gw = w0 + w;
     if (gw < A.nwords) A.state[A.load_node[s] * A.nwords + gw] = lds[A.load_slot[s] * W + w];
   }
-  if (A.digest)
+  if (A.digest) {
+    if (tid < 64) st_words<W>(sig, tid, sreg);
+    __syncthreads();
     for (uint32_t i = tid; i < 64 * W; i += TILE_THREADS) {
       uint32_t p = i / W, w = i % W;
       uint64_t gw = w0 + w;
       if (gw < A.nwords) A.digest[p * A.nwords + gw] = sig[i];
     }
+  }
   if (A.detect && tid < A.n_obs)
 #pragma unroll
     for (int k = 0; k < W; k++)
