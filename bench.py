@@ -57,6 +57,8 @@ def main() -> int:
                          "launch per design; rv32i: per-lane random RV32I programs on the word engine; "
                          "faultsim: surrogate-triaged stuck-at fault simulation on the 20k-gate netlist")
     ap.add_argument("--epochs", type=int, default=6, help="faultsim: surrogate training epochs")
+    ap.add_argument("--codegen", choices=("auto", "on", "off"), default="auto",
+                    help="rv32i: generated per-design kernel (auto/on) or the record interpreter (off)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
@@ -215,9 +217,9 @@ def bench_rv32i(args) -> int:
     25 %, branch 15 %), 16,384 lanes, halting on ecall, cycle cap --cycles."""
     from rrtl_amd import workloads
     job = workloads.rv32i_job(16384, args.cycles, args.seed)
-    sim = job.make()
+    sim = job.make(codegen=args.codegen)
     sim.run(**dict(job.run, cycles=1))   # compile the run plan
-    sim = job.make()
+    sim = job.make(codegen=args.codegen)
     t0 = time.perf_counter()
     sim.run(**job.run)
     sim.synchronize()
@@ -229,6 +231,7 @@ def bench_rv32i(args) -> int:
             "higher_is_better": True, "data": "synthetic (random RV32I programs)",
             "wall_ms": wall * 1e3, "kernel_ms": st["kernel_ms"],
             "config": {"workload": f"rv32i core, 16384 lanes, 512-instruction random programs, cap {args.cycles}",
+                       "codegen": args.codegen,
                        "halt_cycles_min": int(halt.min()), "halt_cycles_max": int(halt.max()),
                        "lanes_at_cap": int((halt >= args.cycles).sum())}}
     print(json.dumps(line), flush=True)

@@ -61,7 +61,18 @@ typedef struct rrtl_hip_options {
    * position may take the previous position's result through registers.
    * 0 = automatic.  Read only when struct_size covers it. */
   uint32_t stage_gates;
+  /* Word engine: how a design's records are executed.  RRTL_CODEGEN_ON
+   * generates straight-line device code for the design and compiles it for the
+   * GPU when the engine is created (the counterpart of the reference's
+   * per-design code generator, crates/rrtl-sim-ir/src/gpu_codegen.rs:503);
+   * RRTL_CODEGEN_OFF runs the record interpreter; RRTL_CODEGEN_AUTO generates
+   * code for designs of at most 4000 records and 128 live scratch words and
+   * interprets larger ones.  Results are identical.  Read only when
+   * struct_size covers it. */
+  uint32_t codegen;
 } rrtl_hip_options;
+
+enum { RRTL_CODEGEN_AUTO = 0, RRTL_CODEGEN_ON = 1, RRTL_CODEGEN_OFF = 2 };
 
 /* SimdCpuSimulator::new (lib.rs:5529): lanes == 0 is E_SIM_IR_LANES; the
  * constructor settles combinational logic (lib.rs:9058). */
@@ -162,6 +173,21 @@ rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out);
 rrtl_status rrtl_hip_get_detect(rrtl_hip_engine* e, uint32_t* out_words);
 /* Gate engine: [tile_words, live_slots, levels, gates] of the last compiled run. */
 rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4);
+
+/* ---- per-design code generation (gpu_codegen::emit_kernel,
+ * crates/rrtl-sim-ir/src/gpu_codegen.rs:503, which returns a design's CUDA /
+ * OpenCL tick kernel as source text).  The word engine's generated HIP kernel
+ * "rrtl_word_jit" for `storage` (0: scratch and signals in HBM, 1: scratch in
+ * LDS, 2: scratch and signals in LDS).  Writes at most `cap` bytes including
+ * the terminating NUL and sets *len to the full length (without NUL); call
+ * with cap 0 to size the buffer.  Needs no device. */
+rrtl_status rrtl_hip_emit_kernel(const void* program_blob, size_t blob_bytes, uint32_t storage, char* out,
+                                 size_t cap, size_t* len);
+/* Compiles HIP source for gfx950 at run time (hiprtc) against the library's
+ * device headers, as the word engine does for its generated kernels; sets
+ * *code_bytes to the code object's size.  Needs no device.  On failure the
+ * compiler's log is in rrtl_hip_create_error(). */
+rrtl_status rrtl_hip_compile_kernel(const char* source, size_t* code_bytes);
 
 /* ---- multi-design batches: several engines (different designs, lane counts,
  * cycle counts; all on one device) run their rrtl_hip_run specs in ONE

@@ -29,6 +29,8 @@ IN_HOLD, IN_RANDOM, IN_CONST, IN_PULSE, IN_TRACE, IN_GROUP_RANDOM = 0, 1, 2, 3, 
 OBSERVE_PRE, OBSERVE_POST = 0, 1
 NO_HALT = 0xFFFFFFFF
 ENGINE_NAMES = {"auto": RRTL_HIP_AUTO, "word": RRTL_HIP_WORD, "gate": RRTL_HIP_GATE}
+# word engine: generated per-design kernel ("auto", "on") or the record interpreter ("off")
+CODEGEN_NAMES = {"auto": 0, "on": 1, "off": 2}
 
 # every symbol the header declares
 EXPORTS = [
@@ -41,7 +43,7 @@ EXPORTS = [
     "rrtl_hip_get_digest", "rrtl_hip_trace_bytes", "rrtl_hip_get_trace", "rrtl_hip_get_halt_cycles",
     "rrtl_hip_get_detect",
     "rrtl_hip_gate_tile_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
-    "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy",
+    "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy", "rrtl_hip_emit_kernel", "rrtl_hip_compile_kernel",
 ]
 BATCH_PERSISTENT, BATCH_PER_DESIGN = 0, 1
 
@@ -57,7 +59,7 @@ class Options(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("engine", ctypes.c_uint32), ("device", ctypes.c_int32),
                 ("block_threads", ctypes.c_uint32), ("tile_words", ctypes.c_uint32),
                 ("ticks_per_launch", ctypes.c_uint32), ("tile_chunks", ctypes.c_uint32),
-                ("stage_gates", ctypes.c_uint32)]
+                ("stage_gates", ctypes.c_uint32), ("codegen", ctypes.c_uint32)]
 
 
 class RunSpec(ctypes.Structure):
@@ -121,6 +123,8 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_batch_last_error.restype = ctypes.c_char_p
     L.rrtl_hip_batch_last_error.argtypes = [vp]
     L.rrtl_hip_batch_destroy.argtypes = [vp]
+    L.rrtl_hip_emit_kernel.argtypes = [ctypes.c_char_p, sz, u32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+    L.rrtl_hip_compile_kernel.argtypes = [ctypes.c_char_p, ctypes.POINTER(sz)]
     for fn in EXPORTS:
         if fn != "rrtl_hip_destroy" and getattr(L, fn).restype is ctypes.c_int:
             getattr(L, fn).restype = ctypes.c_int
@@ -135,17 +139,48 @@ def _ptr(a: np.ndarray) -> ctypes.c_void_p:
 SignalRef = Union[int, str]
 
 
+STORAGE_HBM, STORAGE_SCRATCH_LDS, STORAGE_ALL_LDS = 0, 1, 2
+
+
+def emit_kernel(program: Program, storage: int = STORAGE_ALL_LDS) -> str:
+    """The word engine's generated HIP kernel for ``program`` (the counterpart
+    of gpu_codegen::emit_kernel, crates/rrtl-sim-ir/src/gpu_codegen.rs:503).
+    Needs no GPU."""
+    L = load_library()
+    blob = program.to_blob()
+    n = ctypes.c_size_t()
+    st = L.rrtl_hip_emit_kernel(blob, len(blob), storage, None, 0, ctypes.byref(n))
+    if st != 0:
+        raise HipError(st, L.rrtl_hip_create_error().decode())
+    buf = ctypes.create_string_buffer(n.value + 1)
+    st = L.rrtl_hip_emit_kernel(blob, len(blob), storage, buf, n.value + 1, ctypes.byref(n))
+    if st != 0:
+        raise HipError(st, L.rrtl_hip_create_error().decode())
+    return buf.value.decode()
+
+
+def compile_kernel(source: str) -> int:
+    """Compiles HIP source for gfx950 with the library's run-time compiler;
+    returns the code object's size.  Needs no GPU."""
+    L = load_library()
+    n = ctypes.c_size_t()
+    st = L.rrtl_hip_compile_kernel(source.encode(), ctypes.byref(n))
+    if st != 0:
+        raise HipError(st, L.rrtl_hip_create_error().decode())
+    return n.value
+
+
 class HipSimulator:
     """A batch of ``lanes`` independent instances of one program on one GPU."""
 
     def __init__(self, program: Program, lanes: int, engine: str = "auto", device: int = 0,
                  tile_words: int = 0, ticks_per_launch: int = 0, block_threads: int = 0, tile_chunks: int = 0,
-                 stage_gates: int = 0):
+                 stage_gates: int = 0, codegen: str = "auto"):
         self.L = load_library()
         self.program = program
         blob = program.to_blob()
         opt = Options(ctypes.sizeof(Options), ENGINE_NAMES[engine], device, block_threads, tile_words,
-                      ticks_per_launch, tile_chunks, stage_gates)
+                      ticks_per_launch, tile_chunks, stage_gates, CODEGEN_NAMES[codegen])
         h = ctypes.c_void_p()
         st = self.L.rrtl_hip_create(blob, len(blob), lanes, ctypes.byref(opt), ctypes.byref(h))
         if st != 0:

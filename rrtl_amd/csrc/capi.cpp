@@ -3,6 +3,8 @@
 #include "../../include/rrtl_hip.h"
 #include <cstddef>
 #include "engine.hpp"
+#include "rtc.hpp"
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -15,6 +17,7 @@ Batch* make_batch(std::vector<Engine*> engines, std::vector<RunSpec> specs);
 void batch_run(Batch* b, bool persistent);
 const RunStats& batch_stats(const Batch* b);
 void batch_destroy(Batch* b);
+std::string word_kernel_source(const Program& prog, int mode);
 }
 
 struct rrtl_hip_batch {
@@ -99,11 +102,14 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
         opt.tile_chunks = options->tile_chunks;
       if (options->struct_size >= offsetof(rrtl_hip_options, stage_gates) + sizeof(uint32_t))
         opt.stage_gates = options->stage_gates;
+      if (options->struct_size >= offsetof(rrtl_hip_options, codegen) + sizeof(uint32_t))
+        opt.codegen = options->codegen;
     }
     if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 1024))
       throw rrtl::Error("E_HIP_OPTIONS: block_threads must be a multiple of 64 up to 1024");
     if (opt.tile_words && opt.tile_words != 1 && opt.tile_words != 2 && opt.tile_words != 4 && opt.tile_words != 8)
       throw rrtl::Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
+    if (opt.codegen > rrtl::CODEGEN_OFF) throw rrtl::Error("E_HIP_OPTIONS: codegen must be 0, 1 or 2");
     if (opt.stage_gates && opt.stage_gates != 2 && opt.stage_gates != 3)
       throw rrtl::Error("E_HIP_OPTIONS: stage_gates must be 2 or 3");
     rrtl::Program prog = rrtl::Program::parse(blob, bytes);
@@ -118,6 +124,37 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
       throw;
     }
     *out = h;
+    g_create_err.clear();
+    return RRTL_OK;
+  } catch (const std::exception& ex) {
+    g_create_err = ex.what();
+    return classify(g_create_err);
+  }
+}
+
+rrtl_status rrtl_hip_emit_kernel(const void* blob, size_t bytes, uint32_t storage, char* out, size_t cap,
+                                 size_t* len) {
+  try {
+    if (!blob || !len || (cap && !out)) throw rrtl::Error("E_HIP_NULL: null argument");
+    const std::string src = rrtl::word_kernel_source(rrtl::Program::parse(blob, bytes), int(storage));
+    *len = src.size();
+    if (cap) {
+      const size_t n = std::min(cap - 1, src.size());
+      std::memcpy(out, src.data(), n);
+      out[n] = '\0';
+    }
+    g_create_err.clear();
+    return RRTL_OK;
+  } catch (const std::exception& ex) {
+    g_create_err = ex.what();
+    return classify(g_create_err);
+  }
+}
+
+rrtl_status rrtl_hip_compile_kernel(const char* source, size_t* code_bytes) {
+  try {
+    if (!source || !code_bytes) throw rrtl::Error("E_HIP_NULL: null argument");
+    *code_bytes = rrtl::rtc_compile(source).size();
     g_create_err.clear();
     return RRTL_OK;
   } catch (const std::exception& ex) {

@@ -15,27 +15,7 @@ namespace rrtl {
       throw ::rrtl::Error(std::string("E_HIP_RUNTIME: ") + #expr + ": " + hipGetErrorString(_e)); \
   } while (0)
 
-// SplitMix64 finaliser (Steele, Lea, Flood, OOPSLA 2014) used as a counter-based
-// hash for on-device stimulus: bit b of input s at cycle c for lane l is bit
-// (l mod 64) of  stim_word(seed, (s << 8) | b, c, l / 64).
-__host__ __device__ inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-__host__ __device__ inline uint64_t stim_key(uint64_t seed, uint64_t stream, uint64_t cycle) {
-  return mix64(seed ^ ((stream + 1) * 0x9E3779B97F4A7C15ull) ^ ((cycle + 1) * 0xD1B54A32D192ED03ull));
-}
-__host__ __device__ inline uint64_t stim_from_key(uint64_t key, uint64_t word64) {
-  return mix64(key + (word64 + 1) * 0x9E3779B97F4A7C15ull);
-}
-// per-lane observation chain of the word engine
-__host__ __device__ inline uint64_t lane_digest(uint64_t d, uint32_t limb, uint32_t j, uint32_t k) {
-  return mix64(d ^ uint64_t(limb) ^ (uint64_t(j * 64u + k + 1u) << 32));
-}
-
-// Stimulus modes (rrtl_hip_input_mode in the C header).
-enum : uint32_t { IN_HOLD = 0, IN_RANDOM = 1, IN_CONST = 2, IN_PULSE = 3, IN_TRACE = 4, IN_GROUP_RANDOM = 5 };
+// mix64, stim_key, stim_from_key, lane_digest and the stimulus modes: device_defs.hpp
 
 struct RunSpec {
   uint64_t seed = 0, cycles = 0;

@@ -19,7 +19,9 @@ struct EngineOptions {
   uint32_t ticks_per_launch = 0;   // 0: whole run in one launch
   uint32_t tile_chunks = 0;        // gate engine: work-queue chunks per tile (0: automatic, 1: off)
   uint32_t stage_gates = 0;        // gate engine: gate positions per thread per stage (0: automatic, 2 or 3)
+  uint32_t codegen = 0;            // word engine: CODEGEN_AUTO / CODEGEN_ON (generated kernel) or CODEGEN_OFF (interpreter)
 };
+enum : uint32_t { CODEGEN_AUTO = 0, CODEGEN_ON = 1, CODEGEN_OFF = 2 };
 
 // One design's share of a multi-design batch launch (batch.hip): the kernel
 // arguments of its run, how many work items (gate tiles or word lane blocks)

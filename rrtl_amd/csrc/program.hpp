@@ -10,16 +10,11 @@
 #include <string>
 #include <vector>
 #include <stdexcept>
+#include "device_defs.hpp"
 
 namespace rrtl {
 
-enum Op : uint32_t {
-  OP_LIT, OP_SIGNAL, OP_NOT, OP_AND, OP_OR, OP_XOR, OP_ADD, OP_SUB, OP_MUL, OP_EQ, OP_NE,
-  OP_LT, OP_MUX, OP_SLICE, OP_ZEXT, OP_SEXT, OP_TRUNC, OP_CAST, OP_CONCAT, OP_MEMREAD, OP_COUNT
-};
-enum EffectKind : uint32_t { EF_STORE = 0, EF_CAPTURE = 1, EF_MEMWRITE = 2 };
-enum SignalKind : uint32_t { K_INPUT = 0, K_OUTPUT = 1, K_WIRE = 2, K_REG = 3 };
-enum BlockId { B_ASYNC = 0, B_COMB = 1, B_NEXT = 2, B_COMMIT = 3 };
+// Op, EffectKind, SignalKind, BlockId: device_defs.hpp
 
 struct Error : std::runtime_error {
   explicit Error(const std::string& m) : std::runtime_error(m) {}

@@ -2,7 +2,7 @@
 // per-lane interpreter body.  Shared by the word engine's own launches and by
 // the multi-design batch kernel.
 #pragma once
-#include "common.hpp"
+#include "device_defs.hpp"
 
 namespace rrtl {
 namespace word {
@@ -400,6 +400,16 @@ __device__ __forceinline__ void run_records(const Rec* recs, uint32_t n, const C
   }
 }
 
+// How a block's records are executed: InterpExec walks the record arrays of
+// the run; a design compiled at run time supplies its own executor with the
+// same run(A, block id, C) entry.
+struct InterpExec {
+  template <class Cx>
+  __device__ __forceinline__ static void run(const WordArgs& A, uint32_t id, const Cx& C) {
+    run_records(A.blk[id], A.nrec[id], C);
+  }
+};
+
 template <class Cx>
 __device__ __forceinline__ void apply_stimulus(const WordArgs& A, const Cx& C, uint64_t c, uint64_t cyc) {
   uint32_t trace_limb = 0;
@@ -451,7 +461,7 @@ __device__ __forceinline__ void observe(const WordArgs& A, const Cx& C, uint64_t
 // observation points through a single inlined copy of the interpreter.
 constexpr uint32_t STEP_OBS = 15;  // sequence entry: observe (else a block id)
 
-template <int kMode>
+template <int kMode, class Exec = InterpExec>
 __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
   const uint32_t lane = block * blockDim.x + threadIdx.x;
   if (lane >= A.lanes) return;
@@ -493,7 +503,7 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
     for (uint32_t k = 0; k < n_u; k++) {
       const uint32_t id = (seq_u >> (4 * k)) & 15u;
       if (id == STEP_OBS) observe(A, C, c, d);
-      else run_records(A.blk[id], A.nrec[id], C);
+      else Exec::run(A, id, C);
     }
     if (!first && !last) {
       done++;

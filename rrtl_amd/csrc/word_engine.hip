@@ -22,7 +22,10 @@
 // caller trace, observations into a per-lane digest or a trace, per-lane halt.
 #include "engine.hpp"
 #include "word_device.hpp"
+#include "rtc.hpp"
 #include <algorithm>
+#include <cstdio>
+#include <memory>
 #include <cstring>
 #include <map>
 #include <unordered_map>
@@ -219,12 +222,311 @@ class WordCompiler {
   uint32_t capture_limbs_ = 0, max_slot_ = 0;
 };
 
+
+// ------------------------------------------------------------------ generated code
+// A design's records rewritten as straight-line device code and compiled for
+// gfx950 at run time (the MI355X counterpart of the reference's per-design GPU
+// code generator, crates/rrtl-gpu-sim/src/gpu_codegen.rs).  One-limb records
+// become one C++ statement each with their value in a register and literals
+// inlined; a signal word is loaded once per evaluation and forwarded after a
+// store; captures stay in registers until the commit.  Records with results
+// wider than one limb keep the interpreter's code: they are emitted as
+// exec_rec calls on a literal record, which the compiler specialises, with
+// their values in scratch slots.  The executor relies on word_body running
+// B_COMB always followed by B_ASYNC and B_NEXT always followed by B_COMMIT: it
+// runs each pair as one function (jit_eval, jit_edge).
+class WordJit {
+ public:
+  explicit WordJit(const WordProgram& WP) : WP(WP) {
+    std::string eval, edge;
+    begin(eval); emit_block(B_COMB); scr_.clear(); emit_block(B_ASYNC);
+    begin(edge); emit_block(B_NEXT); scr_.clear(); emit_block(B_COMMIT);
+    body_ = "namespace rrtl {\nnamespace word {\nnamespace jit {\n\n"
+            "template <class Cx>\n__device__ __forceinline__ void jit_eval(const Cx& C) {\n" + eval + "}\n\n"
+            "template <class Cx>\n__device__ __forceinline__ void jit_edge(const Cx& C) {\n" + edge + "}\n\n"
+            "struct JitExec {\n"
+            "  template <class Cx>\n"
+            "  __device__ __forceinline__ static void run(const WordArgs&, uint32_t id, const Cx& C) {\n"
+            "    if (id == B_COMB) jit_eval(C);\n"
+            "    else if (id == B_COMMIT) jit_edge(C);\n"
+            "  }\n"
+            "};\n\n}  // namespace jit\n}  // namespace word\n}  // namespace rrtl\n";
+  }
+  // Scratch slots the generated code needs (only records kept in interpreted form use them).
+  bool uses_scratch() const { return generic_; }
+  uint32_t generic_records() const { return n_generic_; }
+  // Translation unit with the kernel "rrtl_word_jit" for one storage mode.
+  std::string source(int mode) const {
+    return "#include \"word_device.hpp\"\n\n" + body_ +
+           "\nextern \"C\" __global__ void __launch_bounds__(256) rrtl_word_jit(rrtl::word::WordArgs A) {\n"
+           "  rrtl::word::word_body<" + std::to_string(mode) + ", rrtl::word::jit::JitExec>(A, blockIdx.x);\n}\n";
+  }
+
+ private:
+  static std::string hex(uint32_t v) {
+    char b[16];
+    snprintf(b, sizeof b, "0x%08xu", v);
+    return b;
+  }
+  static std::string num(uint32_t v) { return std::to_string(v) + "u"; }
+  static std::string masked(const std::string& e, uint32_t width) {
+    return width % 32 ? "(" + e + " & " + hex(final_mask(width)) + ")" : e;
+  }
+  void begin(std::string& out) { out_ = &out; sig_.clear(); scr_.clear(); stage_.clear(); }
+  void line(const std::string& s) { *out_ += "  " + s + "\n"; }
+  std::string fresh(char p) { return p + std::to_string(tmp_++); }
+  std::string def(char p, const std::string& e) {
+    std::string n = fresh(p);
+    line("const uint32_t " + n + " = " + e + ";");
+    return n;
+  }
+  uint32_t cp(uint32_t i) const {
+    if (i >= WP.cpool.size()) throw Error("E_PROGRAM: constant pool index out of range");
+    return WP.cpool[i];
+  }
+  const uint32_t* mtab(uint32_t m) const {
+    if (4 * size_t(m) + 4 > WP.mem_table.size()) throw Error("E_PROGRAM: memory index out of range");
+    return &WP.mem_table[4 * size_t(m)];
+  }
+
+  // Limb `limb` of operand `code` as an expression (loads are emitted first).
+  std::string opnd(uint32_t code, uint32_t limb) {
+    const uint32_t idx = code & ~SP_MASK;
+    switch (code & SP_MASK) {
+      case SP_CONST: return hex(cp(idx + limb));
+      case SP_SIGNAL: {
+        const uint32_t w = idx + limb;
+        auto it = sig_.find(w);
+        if (it != sig_.end()) return it->second;
+        return sig_[w] = def('s', "C.ld(" + hex(SP_SIGNAL | w) + ", 0)");
+      }
+      default: {
+        auto it = scr_.find(idx + limb);
+        if (it != scr_.end()) return it->second;
+        return def('t', "C.ld(" + hex(idx + limb) + ", 0)");
+      }
+    }
+  }
+  std::string reset_cond(uint32_t code, uint32_t pol) {
+    const std::string r = opnd(code, 0);
+    std::string n = fresh('r');
+    line("const bool " + n + " = (" + r + " & 1u) " + (pol == 0 ? "!=" : "==") + " 0u;");
+    return n;
+  }
+  // In-range test of an address operand of `n` limbs against `depth`.
+  std::string addr_ok(uint32_t code, uint32_t n, uint32_t depth, std::string& lo) {
+    lo = opnd(code, 0);
+    std::string ok = "(" + lo + " < " + num(depth) + ")";
+    for (uint32_t l = 1; l < n; l++) ok += " && (" + opnd(code, l) + " == 0u)";
+    return ok;
+  }
+
+  // One-limb result expression, or "" when the record keeps interpreted form.
+  std::string expr1(const Rec& R) {
+    const uint32_t op = R.w[0] & 0xFF;
+    switch (op) {
+      case OP_NOT: return "~" + opnd(R.w[2], 0);
+      case OP_AND: case OP_OR: case OP_XOR: case OP_ADD: case OP_SUB: case OP_MUL: {
+        static const char* sym[] = {"&", "|", "^", "+", "-", "*"};
+        const std::string a = opnd(R.w[2], 0), b = opnd(R.w[3], 0);
+        return "(" + a + " " + sym[op - OP_AND] + " " + b + ")";
+      }
+      case OP_MUX: {
+        const std::string c = opnd(R.w[4], 0), a = opnd(R.w[2], 0), b = opnd(R.w[3], 0);
+        return "((" + c + " & 1u) ? " + a + " : " + b + ")";
+      }
+      case OP_ZEXT: case OP_TRUNC: case OP_CAST: case R_SIGLOAD: return opnd(R.w[2], 0);
+      case OP_EQ: case OP_NE: {
+        const uint32_t la = R.w[7] & 0xFF, lb = (R.w[7] >> 8) & 0xFF, n = std::max(la, lb);
+        std::string e;
+        for (uint32_t l = 0; l < n; l++) {
+          const std::string a = l < la ? opnd(R.w[2], l) : "0u", b = l < lb ? opnd(R.w[3], l) : "0u";
+          const std::string t = "uint32_t(" + a + " == " + b + ")";
+          e = e.empty() ? t : e + " & " + t;
+        }
+        if (e.empty()) e = "1u";
+        return op == OP_EQ ? "(" + e + ")" : "((" + e + ") ^ 1u)";
+      }
+      case OP_LT: {
+        const uint32_t wl = R.w[6];
+        if (wl == 0 || wl > 32) return "";
+        const std::string a = opnd(R.w[2], 0), b = opnd(R.w[3], 0);
+        if (R.w[7] >> 24) {  // signed: the wl-bit values moved to the top and compared as int
+          const std::string sh = num(32 - wl);
+          return "uint32_t(int(" + a + " << " + sh + ") < int(" + b + " << " + sh + "))";
+        }
+        return "uint32_t(" + masked(a, wl) + " < " + masked(b, wl) + ")";
+      }
+      case OP_SLICE: {
+        const uint32_t n = R.w[7], lsb = R.w[6], li = lsb >> 5, sh = lsb & 31;
+        const std::string lo = li < n ? opnd(R.w[2], li) : "0u";
+        if (!sh) return lo;
+        const std::string hi = li + 1 < n ? opnd(R.w[2], li + 1) : "";
+        if (hi.empty()) return "(" + lo + " >> " + num(sh) + ")";
+        return "((" + lo + " >> " + num(sh) + ") | (" + hi + " << " + num(32 - sh) + "))";
+      }
+      case OP_SEXT: {
+        const uint32_t iw = R.w[6];
+        if (iw == 0) return "";
+        const std::string x = opnd(R.w[2], 0);
+        if (iw >= 32) return x;
+        const std::string sh = num(32 - iw);
+        return "uint32_t(int(" + x + " << " + sh + ") >> " + sh + ")";
+      }
+      case OP_CONCAT: {
+        std::string e;
+        uint32_t off = 0;
+        for (int k = int(R.w[3]) - 1; k >= 0; k--) {
+          const uint32_t code = cp(R.w[2] + 2 * k), wd = cp(R.w[2] + 2 * k + 1) & 0xFFFF;
+          std::string t = masked(opnd(code, 0), wd);
+          if (off >= 32) return "";
+          if (off) t = "(" + t + " << " + num(off) + ")";
+          e = e.empty() ? t : e + " | " + t;
+          off += wd;
+        }
+        return e.empty() ? "0u" : "(" + e + ")";
+      }
+      case OP_MEMREAD: {
+        const uint32_t* mt = mtab(R.w[6]);
+        if (mt[1] == 0) return "0u";
+        std::string lo;
+        const std::string ok = addr_ok(R.w[2], R.w[7], mt[1], lo);
+        const std::string o = fresh('o');
+        line("const bool " + o + " = " + ok + ";");
+        // out-of-range lanes read word 0 (in bounds) and take 0
+        const std::string m = def('m', "C.mem(" + num(mt[0]) + ", uint64_t(" + o + " ? " + lo + " : 0u) * " + num(mt[2]) + ")");
+        return "(" + o + " ? " + (mt[2] == 1 ? masked(m, mt[3]) : m) + " : 0u)";
+      }
+      default: return "";
+    }
+  }
+
+  void spill(uint32_t code, uint32_t limbs) {
+    if ((code & SP_MASK) != SP_SCRATCH) return;
+    for (uint32_t l = 0; l < limbs; l++) {
+      auto it = scr_.find((code & ~SP_MASK) + l);
+      if (it != scr_.end()) line("C.st(" + hex(it->first) + ", 0, " + it->second + ");");
+    }
+  }
+  void emit_generic(const Rec& R) {
+    const uint32_t op = R.w[0] & 0xFF, L = (R.w[0] >> 8) & 0xFF;
+    // register-held operands go to their slots first
+    switch (op) {
+      case OP_AND: case OP_OR: case OP_XOR: case OP_ADD: case OP_SUB: case OP_MUL: case OP_EQ: case OP_NE: case OP_LT:
+        spill(R.w[2], R.w[7] & 0xFF); spill(R.w[3], (R.w[7] >> 8) & 0xFF); break;
+      case OP_MUX:
+        spill(R.w[2], R.w[7] & 0xFF); spill(R.w[3], (R.w[7] >> 8) & 0xFF); spill(R.w[4], 1); break;
+      case OP_CONCAT:
+        for (uint32_t k = 0; k < R.w[3]; k++) spill(cp(R.w[2] + 2 * k), cp(R.w[2] + 2 * k + 1) >> 16);
+        break;
+      default: spill(R.w[2], R.w[7]); break;
+    }
+    std::string w;
+    for (int k = 0; k < 8; k++) w += (k ? ", " : "") + hex(R.w[k]);
+    line("{ const Rec R = {{" + w + "}}; exec_rec(C, R); }");
+    for (uint32_t l = 0; l < L; l++) scr_.erase(R.w[1] + l);
+    generic_ = true;
+    n_generic_++;
+  }
+
+  void emit_effect(const Rec& R) {
+    const uint32_t op = R.w[0] & 0xFF, L = (R.w[0] >> 8) & 0xFF;
+    switch (op) {
+      case R_STORE:
+        for (uint32_t l = 0; l < L; l++) {
+          std::string v = l < R.w[7] ? opnd(R.w[2], l) : "0u";
+          if (l + 1 == L) v = masked(v, R.w[5]);
+          const std::string n = def('w', v);
+          line("C.stsig(" + num(R.w[1]) + ", " + num(l) + ", " + n + ");");
+          sig_[R.w[1] + l] = n;
+        }
+        break;
+      case R_ASYNC_CAPTURE: {
+        const std::string r = reset_cond(R.w[3], R.w[6]);
+        std::vector<std::string> v(L);
+        for (uint32_t l = 0; l < L; l++) {
+          v[l] = l < R.w[7] ? opnd(R.w[2], l) : "0u";
+          if (l + 1 == L) v[l] = masked(v[l], R.w[5]);
+        }
+        line("if (" + r + ") {");
+        for (uint32_t l = 0; l < L; l++) line("  C.stsig(" + num(R.w[1]) + ", " + num(l) + ", " + v[l] + ");");
+        line("}");
+        for (uint32_t l = 0; l < L; l++) sig_.erase(R.w[1] + l);
+        break;
+      }
+      case R_CAPTURE: {
+        const std::string r = (R.w[6] & 2u) ? reset_cond(R.w[3], R.w[6] & 1u) : "";
+        for (uint32_t l = 0; l < L; l++) {
+          std::string v = l < R.w[7] ? opnd(R.w[2], l) : "0u";
+          if (!r.empty()) v = "(" + r + " ? " + opnd(R.w[4], l) + " : " + v + ")";
+          if (l + 1 == L) v = masked(v, R.w[5]);
+          stage_[R.w[1] + l] = def('c', v);
+        }
+        break;
+      }
+      case R_COMMIT:
+        for (uint32_t l = 0; l < L; l++) {
+          auto it = stage_.find((R.w[2] & ~SP_MASK) + l);
+          if (it == stage_.end()) throw Error("E_PROGRAM: register commit without a capture");
+          line("C.stsig(" + num(R.w[1]) + ", " + num(l) + ", " + it->second + ");");
+          sig_[R.w[1] + l] = it->second;
+        }
+        break;
+      case R_MEMWRITE: {
+        const uint32_t* mt = mtab(R.w[1]);
+        if (mt[1] == 0) break;
+        const uint32_t la = R.w[7] & 0xFF, ld = (R.w[7] >> 8) & 0xFF;
+        const std::string en = opnd(R.w[2], 0);
+        std::string lo;
+        const std::string ok = addr_ok(R.w[3], la, mt[1], lo);
+        std::vector<std::string> v(L);
+        for (uint32_t l = 0; l < L; l++) {
+          v[l] = l < ld ? opnd(R.w[4], l) : "0u";
+          if (l + 1 == L) v[l] = masked(v[l], mt[3]);
+        }
+        line("if ((" + en + " & 1u) && " + ok + ") {");
+        for (uint32_t l = 0; l < L; l++)
+          line("  C.mem(" + num(mt[0]) + ", uint64_t(" + lo + ") * " + num(mt[2]) + " + " + num(l) + ") = " + v[l] + ";");
+        line("}");
+        break;
+      }
+      default: throw Error("E_PROGRAM: unknown effect record");
+    }
+  }
+
+  void emit_block(int b) {
+    for (const Rec& R : WP.blk[b]) {
+      const uint32_t op = R.w[0] & 0xFF, L = (R.w[0] >> 8) & 0xFF;
+      if (op >= R_STORE && op != R_SIGLOAD) { emit_effect(R); continue; }
+      std::string e = L == 1 ? expr1(R) : "";
+      if (e.empty()) { emit_generic(R); continue; }
+      scr_[R.w[1]] = def('v', masked(e, R.w[5]));
+    }
+  }
+
+  const WordProgram& WP;
+  std::string body_;
+  std::string* out_ = nullptr;
+  std::unordered_map<uint32_t, std::string> sig_, scr_, stage_;  // word -> register holding it
+  uint32_t tmp_ = 0, n_generic_ = 0;
+  bool generic_ = false;
+};
+
 // ------------------------------------------------------------------ engine
 class WordEngine final : public Engine {
  public:
   WordEngine(Program prog, uint32_t lanes, const EngineOptions& opt) : Engine(std::move(prog), lanes, opt) {
     if (lanes == 0) throw Error("E_SIM_IR_LANES: packed simulator requires at least one lane");
     WP = WordCompiler(P).compile();
+    // Automatic code generation is for designs whose straight-line code stays
+    // small (compile time grows faster than linearly with its length) and whose
+    // values fit in registers; larger designs run the record interpreter.
+    size_t n_rec = 0;
+    for (auto& b : WP.blk) n_rec += b.size();
+    const bool small = n_rec <= 4000 && WP.scratch_slots <= 128;
+    std::unique_ptr<WordJit> jit;
+    if (opt.codegen == CODEGEN_ON || (opt.codegen == CODEGEN_AUTO && small)) jit.reset(new WordJit(WP));
+    run_slots_ = jit && !jit->uses_scratch() ? 0 : WP.scratch_slots;
     RRTL_HIP_CHECK(hipSetDevice(opt.device));
     hipDeviceProp_t prop;
     RRTL_HIP_CHECK(hipGetDeviceProperties(&prop, opt.device));
@@ -241,8 +543,9 @@ class WordEngine final : public Engine {
       const uint32_t per_cu = (lanes + cus - 1) / cus;
       block_ = std::min<uint32_t>(256, std::max<uint32_t>(64, (per_cu + 63) / 64 * 64));
     }
-    mode_ = lds_mode(block_);
+    mode_ = lds_mode(block_, run_slots_);
     use_lds_ = mode_ != WM_HBM;
+    if (jit) jit_fn_ = rtc_kernel(jit->source(mode_), "rrtl_word_jit", opt.device);
     if (!use_lds_) { scratch_.alloc(size_t(WP.scratch_slots) * lanes); scratch_.zero(stream_); }
     for (int b = 0; b < 4; b++) {
       std::vector<uint32_t> flat;
@@ -318,14 +621,16 @@ class WordEngine final : public Engine {
 
   BatchJob batch_prepare(const RunSpec& spec) override {
     // batch launches use 256-lane blocks; scratch (and signals) stay in LDS when they fit
-    const int mode = lds_mode(256);
+    // (batch items run the record interpreter, whatever the engine's own launches use)
+    const int mode = lds_mode(256, WP.scratch_slots);
     if (mode == WM_HBM && !scratch_.p) { scratch_.alloc(size_t(WP.scratch_slots) * lanes_); scratch_.zero(stream_); }
     WordArgs A = make_args(run_flags(spec), spec.cycles, &spec, true);
+    A.n_scratch = WP.scratch_slots;
     A.scratch = mode != WM_HBM ? nullptr : scratch_.p;
     BatchJob J;
     J.kind = ENGINE_WORD; J.variant = uint32_t(mode);
     J.n_items = (lanes_ + 255) / 256;
-    J.lds_bytes = lds_bytes(mode, 256);
+    J.lds_bytes = lds_bytes(mode, 256, WP.scratch_slots);
     size_t recs = 0;
     for (auto& b : WP.blk) recs += b.size();
     J.item_cost = double(spec.cycles) * double(recs) * 4.0;
@@ -395,7 +700,8 @@ class WordEngine final : public Engine {
     synchronize();
     for (uint32_t l = 0; l < lanes_; l++) out[l] >>= 1;
   }
-  uint32_t scratch_slots() const { return WP.scratch_slots; }
+  uint32_t scratch_slots() const { return run_slots_; }
+  bool compiled() const { return jit_fn_ != nullptr; }
   bool scratch_in_lds() const { return use_lds_; }
 
  private:
@@ -411,22 +717,22 @@ class WordEngine final : public Engine {
     return reinterpret_cast<T*>(b.p);
   }
 
-  // LDS bytes a block of `block` lanes needs in `mode`.
-  size_t lds_bytes(int mode, uint32_t block) const {
+  // LDS bytes a block of `block` lanes with `slots` scratch slots needs in `mode`.
+  size_t lds_bytes(int mode, uint32_t block, uint32_t slots) const {
     if (mode == WM_HBM) return 0;
-    const size_t words = WP.scratch_slots + (mode == WM_ALL_LDS ? P.total_signal_words : 0);
+    const size_t words = slots + (mode == WM_ALL_LDS ? P.total_signal_words : 0);
     return words * block * 4;
   }
   // Signals join scratch in LDS when both fit in 64 KiB per block.
-  int lds_mode(uint32_t block) const {
-    if (lds_bytes(WM_ALL_LDS, block) <= 64 * 1024) return WM_ALL_LDS;
-    if (lds_bytes(WM_SCRATCH_LDS, block) <= 64 * 1024) return WM_SCRATCH_LDS;
+  int lds_mode(uint32_t block, uint32_t slots) const {
+    if (lds_bytes(WM_ALL_LDS, block, slots) <= 64 * 1024) return WM_ALL_LDS;
+    if (lds_bytes(WM_SCRATCH_LDS, block, slots) <= 64 * 1024) return WM_SCRATCH_LDS;
     return WM_HBM;
   }
 
   WordArgs make_args(uint32_t flags, uint64_t cycles, const RunSpec* spec, bool outputs) {
     WordArgs A{};
-    A.n_scratch = WP.scratch_slots; A.n_sig_words = P.total_signal_words;
+    A.n_scratch = run_slots_; A.n_sig_words = P.total_signal_words;
     A.values = values_.p; A.mems = mems_.p; A.scratch = use_lds_ ? nullptr : scratch_.p;
     for (int b = 0; b < 4; b++) { A.blk[b] = reinterpret_cast<const Rec*>(blk_[b].p); A.nrec[b] = uint32_t(WP.blk[b].size()); }
     A.cpool = cpool_.p; A.mem_table = mtab_.p; A.lanes = lanes_; A.flags = flags; A.cycles = cycles;
@@ -487,7 +793,7 @@ class WordEngine final : public Engine {
     const WordArgs A = make_args(flags, cycles, spec, outputs);
     const uint64_t chunk = (opt_.ticks_per_launch && outputs) ? opt_.ticks_per_launch : std::max<uint64_t>(cycles, 1);
     const uint32_t grid = (lanes_ + block_ - 1) / block_;
-    const size_t lds = lds_bytes(mode_, block_);
+    const size_t lds = lds_bytes(mode_, block_, run_slots_);
     stats_ = RunStats{};
     RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
     uint64_t done = 0;
@@ -499,7 +805,10 @@ class WordEngine final : public Engine {
       if (done + B.cycles < cycles) B.flags &= ~uint32_t(F_FINAL_EVAL);
       if (A.trace) B.trace = A.trace + done * A.n_obs_limbs * lanes_;
       if (A.trace_in) B.trace_in = A.trace_in + done * A.trace_in_limbs * lanes_;
-      if (mode_ == WM_ALL_LDS) hipLaunchKernelGGL(word_kernel<WM_ALL_LDS>, dim3(grid), dim3(block_), lds, stream_, B);
+      if (jit_fn_) {
+        void* params[] = {&B};
+        RRTL_HIP_CHECK(hipModuleLaunchKernel(jit_fn_, grid, 1, 1, block_, 1, 1, uint32_t(lds), stream_, params, nullptr));
+      } else if (mode_ == WM_ALL_LDS) hipLaunchKernelGGL(word_kernel<WM_ALL_LDS>, dim3(grid), dim3(block_), lds, stream_, B);
       else if (mode_ == WM_SCRATCH_LDS) hipLaunchKernelGGL(word_kernel<WM_SCRATCH_LDS>, dim3(grid), dim3(block_), lds, stream_, B);
       else hipLaunchKernelGGL(word_kernel<WM_HBM>, dim3(grid), dim3(block_), 0, stream_, B);
       RRTL_HIP_CHECK(hipGetLastError());
@@ -522,6 +831,8 @@ class WordEngine final : public Engine {
   }
 
   WordProgram WP;
+  uint32_t run_slots_ = 0;              // scratch slots of the engine's own launches
+  hipFunction_t jit_fn_ = nullptr;      // the design's generated kernel (null: record interpreter)
   size_t lds_max_ = 65536;
   uint32_t block_ = 256;
   int mode_ = WM_SCRATCH_LDS;
@@ -533,6 +844,12 @@ class WordEngine final : public Engine {
 };
 
 }  // namespace
+
+std::string word_kernel_source(const Program& prog, int mode) {
+  if (mode < WM_HBM || mode > WM_ALL_LDS) throw Error("E_HIP_OPTIONS: storage mode must be 0, 1 or 2");
+  const WordProgram wp = WordCompiler(prog).compile();
+  return WordJit(wp).source(mode);
+}
 
 Engine* make_word_engine(Program prog, uint32_t lanes, const EngineOptions& opt) {
   return new WordEngine(std::move(prog), lanes, opt);

@@ -67,13 +67,15 @@ def test_counter_scalar_oracle_lanes():
             sc.tick()
 
 
+@pytest.mark.parametrize("codegen", ["on", "off"])
 @pytest.mark.parametrize("seed", range(12))
-def test_random_designs_match_packed_oracle(seed):
-    flush_print(f"case: random design seed={seed}")
+def test_random_designs_match_packed_oracle(seed, codegen):
+    """Generated kernel ("on") and record interpreter ("off") alike."""
+    flush_print(f"case: random design seed={seed} codegen={codegen}")
     m = random_design(seed)
     p = lower.lower_module(m)
     lanes = 96
-    sim = HipSimulator(p, lanes, engine="word")
+    sim = HipSimulator(p, lanes, engine="word", codegen=codegen)
     ref = oracle.PackedRef(p, lanes)
     rng = random.Random(seed)
     inputs = [i for i, s in enumerate(p.signals) if s.kind == KIND_INPUT]
@@ -129,9 +131,11 @@ def test_tick_many_equals_repeated_tick(n):
     assert np.array_equal(va, vr) and np.array_equal(ma, mr)
 
 
-def test_random_stimulus_run_matches_oracle_all_widths():
+@pytest.mark.parametrize("codegen", ["on", "off"])
+def test_random_stimulus_run_matches_oracle_all_widths(codegen):
     """Device-side RANDOM/CONST/PULSE stimulus and PRE/POST observation."""
     for seed, post in ((7, False), (8, True)):
+        flush_print(f"case: random stimulus run seed={seed} codegen={codegen}")
         m = random_design(seed, async_resets=False)
         p = lower.lower_module(m)
         lanes, cycles = 130, 24
@@ -146,7 +150,7 @@ def test_random_stimulus_run_matches_oracle_all_widths():
             else:
                 ins.append((i, IN_RANDOM, 0))
         obs = [i for i, s in enumerate(p.signals) if s.kind != KIND_INPUT]
-        sim = HipSimulator(p, lanes, engine="word")
+        sim = HipSimulator(p, lanes, engine="word", codegen=codegen)
         sim.run(cycles, seed=seed, inputs=ins, observe=obs, observe_post=post, trace=True)
         ref, otr, od = packed_run(p, lanes, seed, cycles, ins, obs, observe_post=post)
         assert np.array_equal(sim.trace().reshape(otr.shape), otr)
@@ -156,14 +160,42 @@ def test_random_stimulus_run_matches_oracle_all_widths():
         assert np.array_equal(vs, vr) and np.array_equal(ms, mr)
 
 
-def test_fifo_ready_valid():
+@pytest.mark.parametrize("block_threads", [0, 64])
+def test_storage_modes_generated_equals_interpreter(block_threads):
+    """A design whose signals and scratch exceed a block's LDS share at 256
+    lanes per block runs with both in HBM; at 64 lanes per block its scratch
+    fits in LDS.  The generated kernel and the interpreter give the same trace,
+    digest and final state in either storage mode, and the packed oracle
+    agrees."""
+    flush_print(f"case: storage modes block_threads={block_threads}")
+    m = designs.synthetic_netlist(gates=400, levels=16, dffs=40, inputs=64, outputs=32, seed=4)
+    p = lower.lower_module(m)
+    lanes, cycles, seed = 200, 12, 9
+    ins = [(i, IN_RANDOM, 0) for i, s in enumerate(p.signals) if s.kind == KIND_INPUT]
+    obs = [i for i, s in enumerate(p.signals) if s.kind != KIND_INPUT][:40]
+    out = {}
+    for cg in ("on", "off"):
+        sim = HipSimulator(p, lanes, engine="word", codegen=cg, block_threads=block_threads)
+        sim.run(cycles, seed=seed, inputs=ins, observe=obs, trace=True)
+        out[cg] = (sim.trace(), sim.digest(), sim.snapshot()[0])
+        sim.close()
+    for a, b in zip(out["on"], out["off"]):
+        assert np.array_equal(a, b)
+    ref, otr, od = packed_run(p, lanes, seed, cycles, ins, obs)
+    assert np.array_equal(out["on"][0].reshape(otr.shape), otr)
+    assert np.array_equal(out["on"][1], od)
+
+
+@pytest.mark.parametrize("codegen", ["on", "off"])
+def test_fifo_ready_valid(codegen):
+    flush_print(f"case: fifo codegen={codegen}")
     p = lower.lower_module(designs.ready_valid_fifo())
     lanes, cycles, seed = 512, 300, 21
     ins = [(p.signal_index("clk"), IN_CONST, 1), (p.signal_index("rst"), IN_PULSE, oracle.pulse_arg(1, 1, 0)),
            (p.signal_index("in_valid"), IN_RANDOM, 0), (p.signal_index("out_ready"), IN_RANDOM, 0),
            (p.signal_index("in_data"), IN_RANDOM, 0)]
     obs = [p.signal_index(n) for n in ("in_ready", "out_valid", "out_data", "count")]
-    sim = HipSimulator(p, lanes, engine="word")
+    sim = HipSimulator(p, lanes, engine="word", codegen=codegen)
     sim.run(cycles, seed=seed, inputs=ins, observe=obs, trace=True)
     ref, otr, od = packed_run(p, lanes, seed, cycles, ins, obs)
     tr = sim.trace().reshape(otr.shape)

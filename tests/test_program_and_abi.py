@@ -116,6 +116,41 @@ def test_create_rejects_bad_blob_without_gpu_work():
     assert lib.rrtl_hip_create_error().decode().startswith("E_SIM_IR_LANES")
 
 
+def test_generated_kernels_compile_for_gfx950():
+    """Per-design code generation (gpu_codegen::emit_kernel counterpart) and
+    the run-time compiler, both without a GPU: RV32I is all one-limb records
+    (straight-line code, no interpreted records); a 100-bit counter keeps its
+    wide records in interpreted form; every storage mode compiles."""
+    from rrtl_amd import riscv
+    rv = backend.emit_kernel(lower.lower_module(riscv.rv32i_core()))
+    assert "exec_rec" not in rv and "rrtl_word_jit" in rv
+    assert backend.compile_kernel(rv) > 0
+    wide = lower.lower_module(designs.counter(100))
+    for storage in (backend.STORAGE_HBM, backend.STORAGE_SCRATCH_LDS, backend.STORAGE_ALL_LDS):
+        src = backend.emit_kernel(wide, storage)
+        assert "exec_rec(C, R)" in src and f"word_body<{storage}," in src
+        assert backend.compile_kernel(src) > 0
+    for d in (designs.ready_valid_fifo(), random_design(3)):
+        assert backend.compile_kernel(backend.emit_kernel(lower.lower_module(d))) > 0
+
+
+def test_generated_kernel_abi_errors():
+    lib = backend.load_library()
+    p = lower.lower_module(designs.counter())
+    blob = p.to_blob()
+    n = ctypes.c_size_t()
+    assert lib.rrtl_hip_emit_kernel(blob, len(blob), 3, None, 0, ctypes.byref(n)) != 0
+    assert lib.rrtl_hip_create_error().decode().startswith("E_HIP_OPTIONS")
+    # a short buffer gets a NUL-terminated prefix and the full length
+    full = backend.emit_kernel(p)
+    buf = ctypes.create_string_buffer(17)
+    assert lib.rrtl_hip_emit_kernel(blob, len(blob), 2, buf, 17, ctypes.byref(n)) == 0
+    assert n.value == len(full) and buf.value.decode() == full[:16]
+    with pytest.raises(backend.HipError) as e:
+        backend.compile_kernel('#include "word_device.hpp"\n__global__ void k() { undefined_name(); }\n')
+    assert e.value.code == "E_HIP_RUNTIME" and "undefined_name" in str(e.value)
+
+
 def test_autotune_recommendation_lookup(tmp_path):
     """Exact and nearest matching like find_gpu_autotune_recommendation_with_mode
     (crates/rrtl-gpu-sim/src/lib.rs:108-172)."""
