@@ -49,7 +49,7 @@ typedef struct rrtl_hip_options {
   uint32_t struct_size;      /* sizeof(rrtl_hip_options) */
   uint32_t engine;           /* RRTL_HIP_AUTO picks GATE for all-1-bit bitwise designs */
   int32_t device;            /* HIP device ordinal */
-  uint32_t block_threads;    /* word engine workgroup size, 0 = default (256) */
+  uint32_t block_threads;    /* word engine workgroup size (64, 128, 192 or 256), 0 = automatic */
   uint32_t tile_words;       /* gate engine u32 lane-words per workgroup (1,2,4,8), 0 = auto */
   uint32_t ticks_per_launch; /* cycles per kernel launch in runs, 0 = whole run in one launch */
   /* Gate engine runs with more tiles than fit on the device at once: each
@@ -198,10 +198,13 @@ rrtl_status rrtl_hip_compile_kernel(const char* source, size_t* code_bytes);
  * stay valid while the batch exists.  Results land in each engine (digest,
  * trace, halt cycles) exactly as after rrtl_hip_run. */
 typedef struct rrtl_hip_batch rrtl_hip_batch;
-enum { RRTL_BATCH_PERSISTENT = 0, RRTL_BATCH_PER_DESIGN = 1 };
+enum { RRTL_BATCH_PERSISTENT = 0, RRTL_BATCH_PER_DESIGN = 1, RRTL_BATCH_CONCURRENT = 2, RRTL_BATCH_STREAMS = 3 };
 rrtl_status rrtl_hip_batch_create(rrtl_hip_engine* const* engines, const rrtl_hip_run_spec* specs, uint32_t n,
                                   rrtl_hip_batch** out);
-/* mode RRTL_BATCH_PERSISTENT: one graph launch; RRTL_BATCH_PER_DESIGN: one launch per design (the comparison) */
+/* mode RRTL_BATCH_PERSISTENT: one graph launch of the shared work-queue kernel (word designs run the record
+ * interpreter inside it); RRTL_BATCH_PER_DESIGN: one launch per design, back to back (the comparison);
+ * RRTL_BATCH_CONCURRENT: every design on its own kernels and stream, the fork/join captured into one graph
+ * and launched once; RRTL_BATCH_STREAMS: the same launches issued directly on the streams. */
 rrtl_status rrtl_hip_batch_run(rrtl_hip_batch* b, uint32_t mode);
 rrtl_status rrtl_hip_batch_stats(const rrtl_hip_batch* b, rrtl_hip_run_stats* out);
 const char* rrtl_hip_batch_last_error(const rrtl_hip_batch* b);

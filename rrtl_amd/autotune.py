@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import json
 from dataclasses import asdict, dataclass, field
-from typing import Dict, Iterable, List, Optional, Sequence
+from typing import Callable, Dict, Iterable, List, Optional, Sequence
 
 from .backend import HipError, HipSimulator
 from .program import Program
@@ -41,38 +41,42 @@ class HipAutotuneRecommendation:
                 "ticks_per_launch": self.ticks_per_launch}
 
 
-def _candidates(engine: str, steps: int) -> List[dict]:
-    tpl = [0] if steps <= 4096 else [0, 2048]
+def _candidates(engine: str, steps: int, probe: int) -> List[dict]:
+    # a cycles-per-launch candidate only differs from one launch when the probe
+    # spans at least two of its launches
+    tpl = [0] + [t for t in (2048,) if t < steps and 2 * t <= probe]
     if engine == "gate":
         return [{"tile_words": w, "ticks_per_launch": t} for w in (1, 2, 4, 8) for t in tpl]
-    return [{"block_threads": b, "ticks_per_launch": t} for b in (64, 128, 256, 512) for t in tpl]
+    return [{"block_threads": b, "ticks_per_launch": t} for b in (64, 128, 256) for t in tpl]
 
 
 def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = "auto",
              run_kwargs: Optional[dict] = None, probe_steps: Optional[int] = None, repeats: int = 2,
-             device: int = 0) -> HipAutotuneRecommendation:
-    """Measures every candidate on ``probe_steps`` cycles (default min(steps, 512))
-    of the same run and returns the fastest as a recommendation."""
+             device: int = 0, setup: Optional[Callable[[HipSimulator], None]] = None) -> HipAutotuneRecommendation:
+    """Measures every candidate on ``probe_steps`` cycles (default min(steps, 4096))
+    of the same run (``setup`` prepares each fresh engine, e.g. loads memories)
+    and returns the fastest per simulated lane-cycle as a recommendation."""
     if engine == "auto":
         engine = "gate" if program.is_gate_level() else "word"
-    probe = probe_steps or min(steps, 512)
+    probe = probe_steps or min(steps, 4096)
     run_kwargs = dict(run_kwargs or {})
     results = []
     best = None
-    for cand in _candidates(engine, steps):
+    for cand in _candidates(engine, steps, probe):
         try:
             sim = HipSimulator(program, lanes, engine=engine, device=device, **cand)
+            if setup:
+                setup(sim)
         except HipError as e:
             results.append({**cand, "error": e.code})
             continue
         try:
             sim.run(probe, **run_kwargs)  # warm
-            ms = min(_timed(sim, probe, run_kwargs) for _ in range(repeats))
+            ns = min(_timed(sim, probe, run_kwargs) for _ in range(repeats))
         except HipError as e:
             results.append({**cand, "error": e.code})
             sim.close()
             continue
-        ns = ms * 1e6 / (lanes * probe)
         row = {**cand, "ns_per_lane_cycle": ns}
         if engine == "gate":
             row.update(sim.tile_info())
@@ -90,8 +94,10 @@ def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = 
 
 
 def _timed(sim: HipSimulator, steps: int, run_kwargs: dict) -> float:
+    """Kernel ns per simulated lane-cycle of one run (halted lanes do not count)."""
     sim.run(steps, **run_kwargs)
-    return sim.stats()["kernel_ms"]
+    st = sim.stats()
+    return st["kernel_ms"] * 1e6 / max(1, st["lane_cycles"])
 
 
 def save_recommendations(path: str, items: Iterable[HipAutotuneRecommendation]) -> None:

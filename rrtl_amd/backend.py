@@ -45,7 +45,7 @@ EXPORTS = [
     "rrtl_hip_gate_tile_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
     "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy", "rrtl_hip_emit_kernel", "rrtl_hip_compile_kernel",
 ]
-BATCH_PERSISTENT, BATCH_PER_DESIGN = 0, 1
+BATCH_MODES = {"persistent": 0, "per_design": 1, "concurrent": 2, "streams": 3}
 
 
 class HipError(RuntimeError):
@@ -394,9 +394,13 @@ class HipSimulator:
 
 
 class Batch:
-    """Several engines (designs) run in one persistent, graph-captured launch
-    (``rrtl_hip_batch_*``).  ``runs`` pairs each simulator with its ``run`` keyword
-    arguments; results land in each simulator as after ``HipSimulator.run``."""
+    """Several engines (designs) run together (``rrtl_hip_batch_*``).  ``runs``
+    pairs each simulator with its ``run`` keyword arguments; results land in
+    each simulator as after ``HipSimulator.run``.  Modes: ``persistent`` (one
+    graph launch of the shared work-queue kernel), ``concurrent`` (each design
+    on its own kernels and stream, the fork/join captured into one graph),
+    ``streams`` (the same without the graph) and ``per_design`` (one design
+    after another, the comparison)."""
 
     def __init__(self, runs: Sequence[Tuple["HipSimulator", dict]]):
         self.L = load_library()
@@ -417,8 +421,8 @@ class Batch:
             raise HipError(st, self.L.rrtl_hip_create_error().decode())
         self.h = h
 
-    def run(self, persistent: bool = True) -> dict:
-        st = self.L.rrtl_hip_batch_run(self.h, BATCH_PERSISTENT if persistent else BATCH_PER_DESIGN)
+    def run(self, mode: str = "persistent") -> dict:
+        st = self.L.rrtl_hip_batch_run(self.h, BATCH_MODES[mode])
         if st != 0:
             raise HipError(st, self.L.rrtl_hip_batch_last_error(self.h).decode())
         s = RunStats()

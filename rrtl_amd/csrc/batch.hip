@@ -11,6 +11,12 @@
 // are captured once into a graph and replayed.  The alternative the batch is
 // measured against is one launch per design on one stream.
 //
+// The concurrent form keeps every design on its own kernels (the gate tile
+// kernel, the word engine's generated or interpreted kernel) and overlaps them
+// instead: each design's launches go on a stream of its own, forked from and
+// joined to the batch stream, and the whole fork/join is captured into one
+// graph that a single graph launch replays.
+//
 // Reference context: the reference runs each design on its own engine and the
 // runtime loops over shards (crates/rrtl-runtime/src/lib.rs:7446, :8063); it has
 // no cross-design launch.
@@ -92,6 +98,11 @@ class Batch {
   }
   ~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (cexec_) (void)hipGraphExecDestroy(cexec_);
+    if (cgraph_) (void)hipGraphDestroy(cgraph_);
+    for (auto e : join_) (void)hipEventDestroy(e);
+    if (fork_) (void)hipEventDestroy(fork_);
+    for (auto st : side_) (void)hipStreamDestroy(st);
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (jobs_) (void)hipFree(jobs_);
@@ -147,6 +158,58 @@ class Batch {
     for (size_t j = 0; j < E.size(); j++) stats_.lane_cycles += E[j]->last_stats().lane_cycles;
   }
 
+  // Every design on its own kernels and stream, overlapped: one graph launch
+  // of the captured fork/join (graph), or the same launches issued directly.
+  void run_concurrent(bool graph) {
+    for (size_t j = 0; j < E.size(); j++) E[j]->async_prepare(S[j]);
+    if (side_.empty()) {
+      side_.resize(E.size());
+      join_.resize(E.size());
+      for (size_t j = 0; j < E.size(); j++) {
+        RRTL_HIP_CHECK(hipStreamCreateWithFlags(&side_[j], hipStreamNonBlocking));
+        RRTL_HIP_CHECK(hipEventCreateWithFlags(&join_[j], hipEventDisableTiming));
+      }
+      RRTL_HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    }
+    uint64_t launches = 0;
+    auto issue = [&]() {
+      RRTL_HIP_CHECK(hipEventRecord(fork_, stream_));
+      for (size_t j = 0; j < E.size(); j++) {
+        RRTL_HIP_CHECK(hipStreamWaitEvent(side_[j], fork_, 0));
+        launches += E[j]->async_enqueue(side_[j]);
+        RRTL_HIP_CHECK(hipEventRecord(join_[j], side_[j]));
+        RRTL_HIP_CHECK(hipStreamWaitEvent(stream_, join_[j], 0));
+      }
+    };
+    if (graph) {
+      // the launches depend on the prepared runs; capture afresh each time
+      if (cexec_) { RRTL_HIP_CHECK(hipGraphExecDestroy(cexec_)); cexec_ = nullptr; }
+      if (cgraph_) { RRTL_HIP_CHECK(hipGraphDestroy(cgraph_)); cgraph_ = nullptr; }
+      RRTL_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      issue();
+      RRTL_HIP_CHECK(hipStreamEndCapture(stream_, &cgraph_));
+      RRTL_HIP_CHECK(hipGraphInstantiate(&cexec_, cgraph_, nullptr, nullptr, 0));
+    }
+    for (Engine* e : E) e->batch_reset(stream_);
+    RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
+    if (graph) {
+      RRTL_HIP_CHECK(hipGraphLaunch(cexec_, stream_));
+    } else {
+      issue();
+    }
+    RRTL_HIP_CHECK(hipEventRecord(ev1_, stream_));
+    RRTL_HIP_CHECK(hipEventSynchronize(ev1_));
+    float ms = 0;
+    RRTL_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    stats_ = RunStats{};
+    stats_.kernel_ms = ms;
+    stats_.launches = graph ? 1 : launches;
+    for (size_t j = 0; j < E.size(); j++) {
+      E[j]->async_finish(S[j]);
+      stats_.lane_cycles += E[j]->last_stats().lane_cycles;
+    }
+  }
+
   const RunStats& stats() const { return stats_; }
   uint32_t grid() const { return grid_; }
 
@@ -194,6 +257,11 @@ class Batch {
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
+  std::vector<hipStream_t> side_;       // concurrent form: one stream per design
+  std::vector<hipEvent_t> join_;
+  hipEvent_t fork_ = nullptr;
+  hipGraph_t cgraph_ = nullptr;
+  hipGraphExec_t cexec_ = nullptr;
   JobDev* jobs_ = nullptr;
   uint2* items_ = nullptr;
   uint32_t* counter_ = nullptr;
@@ -207,7 +275,13 @@ class Batch {
 Batch* make_batch(std::vector<Engine*> engines, std::vector<RunSpec> specs) {
   return new Batch(std::move(engines), std::move(specs));
 }
-void batch_run(Batch* b, bool persistent) { if (persistent) b->run_persistent(); else b->run_per_design(); }
+void batch_run(Batch* b, uint32_t mode) {
+  if (mode == RRTL_BATCH_PERSISTENT) b->run_persistent();
+  else if (mode == RRTL_BATCH_PER_DESIGN) b->run_per_design();
+  else if (mode == RRTL_BATCH_CONCURRENT) b->run_concurrent(true);
+  else if (mode == RRTL_BATCH_STREAMS) b->run_concurrent(false);
+  else throw Error("E_HIP_BATCH: unknown batch mode");
+}
 const RunStats& batch_stats(const Batch* b) { return b->stats(); }
 uint32_t batch_grid(const Batch* b) { return b->grid(); }
 void batch_destroy(Batch* b) { delete b; }

@@ -14,7 +14,7 @@ void gate_engine_get_words(Engine* e, uint32_t sig, uint32_t* out);
 void gate_engine_tile_info(Engine* e, uint32_t* info);
 class Batch;
 Batch* make_batch(std::vector<Engine*> engines, std::vector<RunSpec> specs);
-void batch_run(Batch* b, bool persistent);
+void batch_run(Batch* b, uint32_t mode);
 const RunStats& batch_stats(const Batch* b);
 void batch_destroy(Batch* b);
 std::string word_kernel_source(const Program& prog, int mode);
@@ -105,8 +105,8 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
       if (options->struct_size >= offsetof(rrtl_hip_options, codegen) + sizeof(uint32_t))
         opt.codegen = options->codegen;
     }
-    if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 1024))
-      throw rrtl::Error("E_HIP_OPTIONS: block_threads must be a multiple of 64 up to 1024");
+    if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 256))
+      throw rrtl::Error("E_HIP_OPTIONS: block_threads must be 64, 128, 192 or 256");
     if (opt.tile_words && opt.tile_words != 1 && opt.tile_words != 2 && opt.tile_words != 4 && opt.tile_words != 8)
       throw rrtl::Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
     if (opt.codegen > rrtl::CODEGEN_OFF) throw rrtl::Error("E_HIP_OPTIONS: codegen must be 0, 1 or 2");
@@ -275,8 +275,8 @@ rrtl_status rrtl_hip_batch_create(rrtl_hip_engine* const* engines, const rrtl_hi
 rrtl_status rrtl_hip_batch_run(rrtl_hip_batch* b, uint32_t mode) {
   if (!b || !b->impl) return RRTL_E_INVALID;
   try {
-    if (mode > RRTL_BATCH_PER_DESIGN) throw rrtl::Error("E_HIP_BATCH: unknown batch mode");
-    rrtl::batch_run(b->impl, mode == RRTL_BATCH_PERSISTENT);
+    if (mode > RRTL_BATCH_STREAMS) throw rrtl::Error("E_HIP_BATCH: unknown batch mode");
+    rrtl::batch_run(b->impl, mode);
     b->err.clear();
     return RRTL_OK;
   } catch (const std::exception& ex) {

@@ -79,6 +79,14 @@ class Engine {
   virtual BatchJob batch_prepare(const RunSpec& spec) = 0;
   virtual void batch_reset(hipStream_t s) = 0;
   virtual void batch_finish(const RunSpec& spec) = 0;
+  // Concurrent batches: prepare a run of the engine's own kernels (uploads;
+  // returns when they are done), enqueue those kernels on `s` without copies,
+  // fills or host waits (so the launches can be captured into a graph;
+  // returns the launch count), book-keep after they ran.  batch_reset clears
+  // the run's outputs before each launch.
+  virtual void async_prepare(const RunSpec& spec) = 0;
+  virtual uint64_t async_enqueue(hipStream_t s) = 0;
+  virtual void async_finish(const RunSpec& spec) = 0;
   const RunStats& last_stats() const { return stats_; }
   hipStream_t stream() const { return stream_; }
 

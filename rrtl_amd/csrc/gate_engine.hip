@@ -1194,50 +1194,88 @@ class GateEngine final : public Engine {
     return best;
   }
 
-  void run_impl(const RunSpec& spec, bool with_outputs) {
-    Prepared R = prepare(spec, with_outputs);
-    TileArgs A = R.A;
-    const uint32_t W = R.W, tiles = R.tiles;
-    const size_t lds = R.lds;
-    const uint64_t chunk = opt_.ticks_per_launch ? opt_.ticks_per_launch : std::max<uint64_t>(spec.cycles, 1);
-    const uint32_t resident = resident_tiles(W, R.byte_addr, R.M, lds);
-    const uint32_t chunks = opt_.tile_chunks ? opt_.tile_chunks : choose_chunks(tiles, resident);
-    if (chunks > 1) { qctl_.alloc(size_t(tiles) + 2); }
-    stats_ = RunStats{};
-    RRTL_HIP_CHECK(hipStreamSynchronize(stream_));
-    RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
+  // A prepared run and its launch schedule.
+  struct Launch {
+    Prepared R;
+    uint64_t cycles = 0;
+    uint32_t chunks = 1, resident = 0;
+  };
+  Launch plan_launch(const RunSpec& spec, bool with_outputs) {
+    Launch L;
+    L.R = prepare(spec, with_outputs);
+    L.cycles = spec.cycles;
+    L.resident = resident_tiles(L.R.W, L.R.byte_addr, L.R.M, L.R.lds);
+    L.chunks = opt_.tile_chunks ? opt_.tile_chunks : choose_chunks(L.R.tiles, L.resident);
+    if (L.chunks > 1) qctl_.alloc(size_t(L.R.tiles) + 2);
+    if (L.R.lds > 65536)
+      RRTL_HIP_CHECK(hipFuncSetAttribute(tile_fn(L.R.W, L.R.byte_addr, L.R.M), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         int(L.R.lds)));
+    return L;
+  }
+  // The run's kernels on stream s: kernels only (no copies, fills or host
+  // waits), so the sequence can be captured into a graph.  Returns the launches.
+  uint64_t enqueue(const Launch& L, hipStream_t s) {
+    TileArgs A = L.R.A;
+    const uint64_t chunk = opt_.ticks_per_launch ? opt_.ticks_per_launch : std::max<uint64_t>(L.cycles, 1);
+    uint64_t launches = 0;
     last_chunks_ = 1;
-    for (uint64_t done = 0; done < spec.cycles; done += chunk) {
+    for (uint64_t done = 0; done < L.cycles; done += chunk) {
       A.cycle_base = done;
-      A.cycles = std::min<uint64_t>(chunk, spec.cycles - done);
+      A.cycles = std::min<uint64_t>(chunk, L.cycles - done);
       if (A.trace) A.trace = trace_.p + done * A.n_obs * nwords_;
-      if (chunks > 1 && A.cycles > 1) {
+      if (L.chunks > 1 && A.cycles > 1) {
         QueueArgs Q{};
-        Q.ctl = qctl_.p; Q.n_tiles = tiles;
-        Q.chunk = (A.cycles + chunks - 1) / chunks;
+        Q.ctl = qctl_.p; Q.n_tiles = L.R.tiles;
+        Q.chunk = (A.cycles + L.chunks - 1) / L.chunks;
         Q.n_chunks = uint32_t((A.cycles + Q.chunk - 1) / Q.chunk);
         Q.wait_limit = wall_khz_ * 1000u * 120u;  // two minutes
-        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(256), 0, stream_, qctl_.p, tiles + 2);
-        const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(resident, 1), tiles * Q.n_chunks);
-        launch_queue(R, grid, A, Q);
+        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(256), 0, s, qctl_.p, L.R.tiles + 2);
+        const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(L.resident, 1), L.R.tiles * Q.n_chunks);
+        launch_queue(L.R, grid, A, Q, s);
         last_chunks_ = Q.n_chunks;
       } else {
-        launch_tile(R, A);
+        launch_tile(L.R, A, s);
       }
-      stats_.launches++;
+      launches++;
     }
-    RRTL_HIP_CHECK(hipEventRecord(ev1_, stream_));
-    RRTL_HIP_CHECK(hipEventSynchronize(ev1_));
-    if (chunks > 1) {
+    return launches;
+  }
+  // After the run's kernels finished: the work queue's error flag, stats.
+  void complete(const Launch& L) {
+    if (L.chunks > 1) {
       uint32_t err = 0;
       RRTL_HIP_CHECK(hipMemcpy(&err, qctl_.p + 1, sizeof(err), hipMemcpyDeviceToHost));
       if (err) throw Error("E_HIP_INTERNAL: tile work queue: a chunk waited past its limit for the previous chunk");
     }
+    stats_.lane_cycles = uint64_t(lanes_) * L.cycles;
+    wires_stale_ = wires_stale_ || L.cycles > 0;
+  }
+
+  void run_impl(const RunSpec& spec, bool with_outputs) {
+    const Launch L = plan_launch(spec, with_outputs);
+    stats_ = RunStats{};
+    RRTL_HIP_CHECK(hipStreamSynchronize(stream_));
+    RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
+    stats_.launches = enqueue(L, stream_);
+    RRTL_HIP_CHECK(hipEventRecord(ev1_, stream_));
+    RRTL_HIP_CHECK(hipEventSynchronize(ev1_));
     float ms = 0;
     RRTL_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
     stats_.kernel_ms = ms;
-    stats_.lane_cycles = uint64_t(lanes_) * spec.cycles;
-    wires_stale_ = wires_stale_ || spec.cycles > 0;
+    complete(L);
+  }
+
+  void async_prepare(const RunSpec& spec) override {
+    async_ = plan_launch(spec, true);
+    batch_digest_ = async_.R.A.digest != nullptr;
+    batch_detect_ = async_.R.A.detect != nullptr;
+    RRTL_HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+  uint64_t async_enqueue(hipStream_t s) override { return enqueue(async_, s); }
+  void async_finish(const RunSpec& spec) override {
+    (void)spec;
+    stats_ = RunStats{};
+    complete(async_);
   }
 
   BatchJob batch_prepare(const RunSpec& spec) override {
@@ -1289,19 +1327,17 @@ class GateEngine final : public Engine {
     RRTL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE_THREADS, lds));
     return uint32_t(std::max(per_cu, 0)) * cus_;
   }
-  void launch_queue(const Prepared& R, uint32_t grid, const TileArgs& A, const QueueArgs& Q) {
+  void launch_queue(const Prepared& R, uint32_t grid, const TileArgs& A, const QueueArgs& Q, hipStream_t s) {
     with_tile_variant(R.W, R.byte_addr, R.M, [&](auto w, auto ba, auto m) {
       hipLaunchKernelGGL((tile_queue_kernel<decltype(w)::value, decltype(ba)::value, decltype(m)::value>), dim3(grid),
-                         dim3(TILE_THREADS), R.lds, stream_, A, Q);
+                         dim3(TILE_THREADS), R.lds, s, A, Q);
     });
     RRTL_HIP_CHECK(hipGetLastError());
   }
-  void launch_tile(const Prepared& R, const TileArgs& A) {
-    const void* fn = tile_fn(R.W, R.byte_addr, R.M);
-    if (R.lds > 65536) RRTL_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(R.lds)));
+  void launch_tile(const Prepared& R, const TileArgs& A, hipStream_t s) {
     with_tile_variant(R.W, R.byte_addr, R.M, [&](auto w, auto ba, auto m) {
       hipLaunchKernelGGL((tile_kernel<decltype(w)::value, decltype(ba)::value, decltype(m)::value>), dim3(R.tiles),
-                         dim3(TILE_THREADS), R.lds, stream_, A);
+                         dim3(TILE_THREADS), R.lds, s, A);
     });
     RRTL_HIP_CHECK(hipGetLastError());
   }
@@ -1318,6 +1354,7 @@ class GateEngine final : public Engine {
   std::vector<Force> forces_;
   bool forces_dirty_ = true, wires_stale_ = false, has_digest_ = false;
   size_t trace_bytes_ = 0;
+  Launch async_;  // the run of a concurrent batch
   uint32_t W_ = 0, M_ = 2, last_slots_ = 0, last_levels_ = 0, last_gates_ = 0, last_chunks_ = 1, cus_ = 1;
   uint64_t wall_khz_ = 100000;
   DevBuf<uint32_t> qctl_;  // work-queue counter, error flag, per-tile chunk counts

@@ -647,14 +647,7 @@ class WordEngine final : public Engine {
   }
   void batch_finish(const RunSpec& spec) override {
     stats_ = RunStats{};
-    stats_.lane_cycles = uint64_t(lanes_) * spec.cycles;
-    if (batch_spec_halt_) {
-      std::vector<uint64_t> h(lanes_);
-      RRTL_HIP_CHECK(hipMemcpy(h.data(), halt_.p, size_t(lanes_) * 8, hipMemcpyDeviceToHost));
-      uint64_t total = 0;
-      for (auto x : h) total += x >> 1;
-      stats_.lane_cycles = total;
-    }
+    stats_.lane_cycles = batch_spec_halt_ ? halted_lane_cycles() : uint64_t(lanes_) * spec.cycles;
   }
   void snapshot(uint32_t* values, uint32_t* memories) override {
     RRTL_HIP_CHECK(hipMemcpyAsync(values, values_.p, size_t(P.total_signal_words) * lanes_ * 4, hipMemcpyDeviceToHost, stream_));
@@ -789,14 +782,13 @@ class WordEngine final : public Engine {
     return flags;
   }
 
-  void launch(uint32_t flags, uint64_t cycles, const RunSpec* spec, bool outputs = false) {
-    const WordArgs A = make_args(flags, cycles, spec, outputs);
+  // The run's kernels on stream s (kernels only: no copies, fills or host
+  // waits, so the sequence can be captured into a graph).  Returns the launches.
+  uint64_t enqueue(const WordArgs& A, uint64_t cycles, bool outputs, hipStream_t s) {
     const uint64_t chunk = (opt_.ticks_per_launch && outputs) ? opt_.ticks_per_launch : std::max<uint64_t>(cycles, 1);
     const uint32_t grid = (lanes_ + block_ - 1) / block_;
     const size_t lds = lds_bytes(mode_, block_, run_slots_);
-    stats_ = RunStats{};
-    RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
-    uint64_t done = 0;
+    uint64_t done = 0, launches = 0;
     do {
       WordArgs B = A;
       B.cycle_base = done;
@@ -807,30 +799,55 @@ class WordEngine final : public Engine {
       if (A.trace_in) B.trace_in = A.trace_in + done * A.trace_in_limbs * lanes_;
       if (jit_fn_) {
         void* params[] = {&B};
-        RRTL_HIP_CHECK(hipModuleLaunchKernel(jit_fn_, grid, 1, 1, block_, 1, 1, uint32_t(lds), stream_, params, nullptr));
-      } else if (mode_ == WM_ALL_LDS) hipLaunchKernelGGL(word_kernel<WM_ALL_LDS>, dim3(grid), dim3(block_), lds, stream_, B);
-      else if (mode_ == WM_SCRATCH_LDS) hipLaunchKernelGGL(word_kernel<WM_SCRATCH_LDS>, dim3(grid), dim3(block_), lds, stream_, B);
-      else hipLaunchKernelGGL(word_kernel<WM_HBM>, dim3(grid), dim3(block_), 0, stream_, B);
+        RRTL_HIP_CHECK(hipModuleLaunchKernel(jit_fn_, grid, 1, 1, block_, 1, 1, uint32_t(lds), s, params, nullptr));
+      } else if (mode_ == WM_ALL_LDS) hipLaunchKernelGGL(word_kernel<WM_ALL_LDS>, dim3(grid), dim3(block_), lds, s, B);
+      else if (mode_ == WM_SCRATCH_LDS) hipLaunchKernelGGL(word_kernel<WM_SCRATCH_LDS>, dim3(grid), dim3(block_), lds, s, B);
+      else hipLaunchKernelGGL(word_kernel<WM_HBM>, dim3(grid), dim3(block_), 0, s, B);
       RRTL_HIP_CHECK(hipGetLastError());
-      stats_.launches++;
+      launches++;
       done += B.cycles;
     } while (done < cycles);
+    return launches;
+  }
+  // Simulated lane-cycles of the last run: per-lane halt counts when halting.
+  uint64_t halted_lane_cycles() {
+    std::vector<uint64_t> h(lanes_);
+    RRTL_HIP_CHECK(hipMemcpy(h.data(), halt_.p, size_t(lanes_) * 8, hipMemcpyDeviceToHost));
+    uint64_t total = 0;
+    for (auto x : h) total += x >> 1;
+    return total;
+  }
+
+  void launch(uint32_t flags, uint64_t cycles, const RunSpec* spec, bool outputs = false) {
+    const WordArgs A = make_args(flags, cycles, spec, outputs);
+    stats_ = RunStats{};
+    RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
+    stats_.launches = enqueue(A, cycles, outputs, stream_);
     RRTL_HIP_CHECK(hipEventRecord(ev1_, stream_));
     RRTL_HIP_CHECK(hipEventSynchronize(ev1_));
     float ms = 0;
     RRTL_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
     stats_.kernel_ms = ms;
     stats_.lane_cycles = uint64_t(lanes_) * cycles;
-    if (has_halt_ && spec && outputs) {
-      std::vector<uint64_t> h(lanes_);
-      RRTL_HIP_CHECK(hipMemcpy(h.data(), halt_.p, size_t(lanes_) * 8, hipMemcpyDeviceToHost));
-      uint64_t total = 0;
-      for (auto x : h) total += x >> 1;
-      stats_.lane_cycles = total;
-    }
+    if (has_halt_ && spec && outputs) stats_.lane_cycles = halted_lane_cycles();
+  }
+
+  void async_prepare(const RunSpec& spec) override {
+    async_args_ = make_args(run_flags(spec), spec.cycles, &spec, true);
+    async_cycles_ = spec.cycles;
+    batch_spec_halt_ = has_halt_;
+    batch_digest_ = has_digest_;
+    RRTL_HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+  uint64_t async_enqueue(hipStream_t s) override { return enqueue(async_args_, async_cycles_, true, s); }
+  void async_finish(const RunSpec& spec) override {
+    stats_ = RunStats{};
+    stats_.lane_cycles = batch_spec_halt_ ? halted_lane_cycles() : uint64_t(lanes_) * spec.cycles;
   }
 
   WordProgram WP;
+  WordArgs async_args_{};               // the run of a concurrent batch
+  uint64_t async_cycles_ = 0;
   uint32_t run_slots_ = 0;              // scratch slots of the engine's own launches
   hipFunction_t jit_fn_ = nullptr;      // the design's generated kernel (null: record interpreter)
   size_t lds_max_ = 65536;

@@ -1,5 +1,7 @@
-"""Multi-design batches: one persistent graph-captured launch must give every
-design exactly what its own run gives (digests, traces, final state)."""
+"""Multi-design batches: one persistent graph-captured launch, or every
+design's own kernels overlapped on their own streams (captured into one graph
+or issued directly), must give every design exactly what its own run gives
+(digests, halt cycles, final state)."""
 import numpy as np
 import pytest
 
@@ -49,8 +51,9 @@ def make(job):
     return sim
 
 
-def test_batch_matches_individual_runs():
-    flush_print("case: mixed batch persistent vs individual")
+@pytest.mark.parametrize("mode", ["persistent", "concurrent", "streams"])
+def test_batch_matches_individual_runs(mode):
+    flush_print(f"case: mixed batch {mode} vs individual")
     jobs = mixed_jobs()
     solo = [make(j) for j in jobs]
     both = [make(j) for j in jobs]
@@ -58,8 +61,11 @@ def test_batch_matches_individual_runs():
     for rep in range(2):  # the second launch replays the captured graph
         for s, j in zip(solo, jobs):
             s.run(**j[3])
-        st = batch.run(persistent=True)
-        assert st["launches"] == 1
+        st = batch.run(mode)
+        if mode == "streams":
+            assert st["launches"] >= len(jobs)
+        else:
+            assert st["launches"] == 1
         assert st["lane_cycles"] == sum(s.stats()["lane_cycles"] for s in solo)
         for a, b, j in zip(solo, both, jobs):
             assert np.array_equal(a.digest(), b.digest()), (rep, a.kind)
@@ -75,7 +81,7 @@ def test_batch_per_design_mode_matches():
     sims = [make(j) for j in jobs]
     ref = [make(j) for j in jobs]
     batch = Batch([(s, j[3]) for s, j in zip(sims, jobs)])
-    st = batch.run(persistent=False)
+    st = batch.run("per_design")
     assert st["launches"] == 2
     for s, j in zip(ref, jobs):
         s.run(**j[3])
