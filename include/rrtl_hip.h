@@ -70,6 +70,13 @@ typedef struct rrtl_hip_options {
    * interprets larger ones.  Results are identical.  Read only when
    * struct_size covers it. */
   uint32_t codegen;
+  /* Word engine: active lanes per 64-thread wave (4, 8, 16, 32 or 64).  Fewer
+   * lanes per wave put more waves on each SIMD, which hides memory latency for
+   * a batch too small to do so at 64 lanes per wave, at the price of issuing
+   * each instruction for fewer lanes.  0 = automatic (the largest of 64, 32,
+   * 16 that still gives four waves per CU, else 16).  Read only when
+   * struct_size covers it. */
+  uint32_t wave_lanes;
 } rrtl_hip_options;
 
 enum { RRTL_CODEGEN_AUTO = 0, RRTL_CODEGEN_ON = 1, RRTL_CODEGEN_OFF = 2 };
@@ -178,11 +185,13 @@ rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4);
  * crates/rrtl-sim-ir/src/gpu_codegen.rs:503, which returns a design's CUDA /
  * OpenCL tick kernel as source text).  The word engine's generated HIP kernel
  * "rrtl_word_jit" for `storage` (0: scratch and signals in HBM, 1: scratch in
- * LDS, 2: scratch and signals in LDS).  Writes at most `cap` bytes including
- * the terminating NUL and sets *len to the full length (without NUL); call
- * with cap 0 to size the buffer.  Needs no device. */
-rrtl_status rrtl_hip_emit_kernel(const void* program_blob, size_t blob_bytes, uint32_t storage, char* out,
-                                 size_t cap, size_t* len);
+ * LDS, 2: scratch and signals in LDS).  With storage 2 and a nonzero
+ * `block_threads`, small memories are staged in LDS as the engine does for
+ * workgroups of that size.  Writes at most `cap` bytes including the
+ * terminating NUL and sets *len to the full length (without NUL); call with cap
+ * 0 to size the buffer.  Needs no device. */
+rrtl_status rrtl_hip_emit_kernel(const void* program_blob, size_t blob_bytes, uint32_t storage, uint32_t block_threads,
+                                 char* out, size_t cap, size_t* len);
 /* Compiles HIP source for gfx950 at run time (hiprtc) against the library's
  * device headers, as the word engine does for its generated kernels; sets
  * *code_bytes to the code object's size.  Needs no device.  On failure the

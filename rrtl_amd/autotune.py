@@ -11,7 +11,7 @@ metric.
 Knobs: the gate engine's tile width (u32 lane-words per workgroup, i.e. how
 many lanes share one LDS-resident copy of the live gate values, which sets
 LDS per workgroup and so workgroups per CU) and the cycles per launch; the
-word engine's workgroup size and cycles per launch.
+word engine's workgroup size, active lanes per wave and cycles per launch.
 """
 from __future__ import annotations
 
@@ -32,13 +32,14 @@ class HipAutotuneRecommendation:
     tile_words: int = 0
     block_threads: int = 0
     ticks_per_launch: int = 0
+    wave_lanes: int = 0
     autotune_metric: str = "kernel_ns_per_lane_cycle"
     autotune_metric_ns: Optional[float] = None
     candidates: List[dict] = field(default_factory=list)
 
     def options(self) -> dict:
         return {"engine": self.engine, "tile_words": self.tile_words, "block_threads": self.block_threads,
-                "ticks_per_launch": self.ticks_per_launch}
+                "ticks_per_launch": self.ticks_per_launch, "wave_lanes": self.wave_lanes}
 
 
 def _candidates(engine: str, steps: int, probe: int) -> List[dict]:
@@ -47,7 +48,8 @@ def _candidates(engine: str, steps: int, probe: int) -> List[dict]:
     tpl = [0] + [t for t in (2048,) if t < steps and 2 * t <= probe]
     if engine == "gate":
         return [{"tile_words": w, "ticks_per_launch": t} for w in (1, 2, 4, 8) for t in tpl]
-    return [{"block_threads": b, "ticks_per_launch": t} for b in (64, 128, 256) for t in tpl]
+    return ([{"block_threads": b, "wave_lanes": 64, "ticks_per_launch": t} for b in (64, 128, 256) for t in tpl] +
+            [{"block_threads": 64, "wave_lanes": w, "ticks_per_launch": t} for w in (8, 16, 32) for t in tpl])
 
 
 def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = "auto",
@@ -90,6 +92,7 @@ def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = 
                                      tile_words=best[0].get("tile_words", 0),
                                      block_threads=best[0].get("block_threads", 0),
                                      ticks_per_launch=best[0].get("ticks_per_launch", 0),
+                                     wave_lanes=best[0].get("wave_lanes", 0),
                                      autotune_metric_ns=best[1], candidates=results)
 
 

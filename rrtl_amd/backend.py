@@ -59,7 +59,7 @@ class Options(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("engine", ctypes.c_uint32), ("device", ctypes.c_int32),
                 ("block_threads", ctypes.c_uint32), ("tile_words", ctypes.c_uint32),
                 ("ticks_per_launch", ctypes.c_uint32), ("tile_chunks", ctypes.c_uint32),
-                ("stage_gates", ctypes.c_uint32), ("codegen", ctypes.c_uint32)]
+                ("stage_gates", ctypes.c_uint32), ("codegen", ctypes.c_uint32), ("wave_lanes", ctypes.c_uint32)]
 
 
 class RunSpec(ctypes.Structure):
@@ -123,7 +123,7 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_batch_last_error.restype = ctypes.c_char_p
     L.rrtl_hip_batch_last_error.argtypes = [vp]
     L.rrtl_hip_batch_destroy.argtypes = [vp]
-    L.rrtl_hip_emit_kernel.argtypes = [ctypes.c_char_p, sz, u32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+    L.rrtl_hip_emit_kernel.argtypes = [ctypes.c_char_p, sz, u32, u32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
     L.rrtl_hip_compile_kernel.argtypes = [ctypes.c_char_p, ctypes.POINTER(sz)]
     for fn in EXPORTS:
         if fn != "rrtl_hip_destroy" and getattr(L, fn).restype is ctypes.c_int:
@@ -142,18 +142,19 @@ SignalRef = Union[int, str]
 STORAGE_HBM, STORAGE_SCRATCH_LDS, STORAGE_ALL_LDS = 0, 1, 2
 
 
-def emit_kernel(program: Program, storage: int = STORAGE_ALL_LDS) -> str:
+def emit_kernel(program: Program, storage: int = STORAGE_ALL_LDS, block_threads: int = 0) -> str:
     """The word engine's generated HIP kernel for ``program`` (the counterpart
-    of gpu_codegen::emit_kernel, crates/rrtl-sim-ir/src/gpu_codegen.rs:503).
-    Needs no GPU."""
+    of gpu_codegen::emit_kernel, crates/rrtl-sim-ir/src/gpu_codegen.rs:503);
+    with ``block_threads``, small memories staged in LDS as the engine stages
+    them for that workgroup size.  Needs no GPU."""
     L = load_library()
     blob = program.to_blob()
     n = ctypes.c_size_t()
-    st = L.rrtl_hip_emit_kernel(blob, len(blob), storage, None, 0, ctypes.byref(n))
+    st = L.rrtl_hip_emit_kernel(blob, len(blob), storage, block_threads, None, 0, ctypes.byref(n))
     if st != 0:
         raise HipError(st, L.rrtl_hip_create_error().decode())
     buf = ctypes.create_string_buffer(n.value + 1)
-    st = L.rrtl_hip_emit_kernel(blob, len(blob), storage, buf, n.value + 1, ctypes.byref(n))
+    st = L.rrtl_hip_emit_kernel(blob, len(blob), storage, block_threads, buf, n.value + 1, ctypes.byref(n))
     if st != 0:
         raise HipError(st, L.rrtl_hip_create_error().decode())
     return buf.value.decode()
@@ -175,12 +176,12 @@ class HipSimulator:
 
     def __init__(self, program: Program, lanes: int, engine: str = "auto", device: int = 0,
                  tile_words: int = 0, ticks_per_launch: int = 0, block_threads: int = 0, tile_chunks: int = 0,
-                 stage_gates: int = 0, codegen: str = "auto"):
+                 stage_gates: int = 0, codegen: str = "auto", wave_lanes: int = 0):
         self.L = load_library()
         self.program = program
         blob = program.to_blob()
         opt = Options(ctypes.sizeof(Options), ENGINE_NAMES[engine], device, block_threads, tile_words,
-                      ticks_per_launch, tile_chunks, stage_gates, CODEGEN_NAMES[codegen])
+                      ticks_per_launch, tile_chunks, stage_gates, CODEGEN_NAMES[codegen], wave_lanes)
         h = ctypes.c_void_p()
         st = self.L.rrtl_hip_create(blob, len(blob), lanes, ctypes.byref(opt), ctypes.byref(h))
         if st != 0:

@@ -17,7 +17,7 @@ Batch* make_batch(std::vector<Engine*> engines, std::vector<RunSpec> specs);
 void batch_run(Batch* b, uint32_t mode);
 const RunStats& batch_stats(const Batch* b);
 void batch_destroy(Batch* b);
-std::string word_kernel_source(const Program& prog, int mode);
+std::string word_kernel_source(const Program& prog, int mode, uint32_t block);
 }
 
 struct rrtl_hip_batch {
@@ -104,12 +104,16 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
         opt.stage_gates = options->stage_gates;
       if (options->struct_size >= offsetof(rrtl_hip_options, codegen) + sizeof(uint32_t))
         opt.codegen = options->codegen;
+      if (options->struct_size >= offsetof(rrtl_hip_options, wave_lanes) + sizeof(uint32_t))
+        opt.wave_lanes = options->wave_lanes;
     }
     if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 256))
       throw rrtl::Error("E_HIP_OPTIONS: block_threads must be 64, 128, 192 or 256");
     if (opt.tile_words && opt.tile_words != 1 && opt.tile_words != 2 && opt.tile_words != 4 && opt.tile_words != 8)
       throw rrtl::Error("E_HIP_OPTIONS: tile_words must be 1, 2, 4 or 8");
     if (opt.codegen > rrtl::CODEGEN_OFF) throw rrtl::Error("E_HIP_OPTIONS: codegen must be 0, 1 or 2");
+    if (opt.wave_lanes && (opt.wave_lanes < 4 || opt.wave_lanes > 64 || (opt.wave_lanes & (opt.wave_lanes - 1))))
+      throw rrtl::Error("E_HIP_OPTIONS: wave_lanes must be 4, 8, 16, 32 or 64");
     if (opt.stage_gates && opt.stage_gates != 2 && opt.stage_gates != 3)
       throw rrtl::Error("E_HIP_OPTIONS: stage_gates must be 2 or 3");
     rrtl::Program prog = rrtl::Program::parse(blob, bytes);
@@ -132,11 +136,12 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
   }
 }
 
-rrtl_status rrtl_hip_emit_kernel(const void* blob, size_t bytes, uint32_t storage, char* out, size_t cap,
-                                 size_t* len) {
+rrtl_status rrtl_hip_emit_kernel(const void* blob, size_t bytes, uint32_t storage, uint32_t block_threads, char* out,
+                                 size_t cap, size_t* len) {
   try {
     if (!blob || !len || (cap && !out)) throw rrtl::Error("E_HIP_NULL: null argument");
-    const std::string src = rrtl::word_kernel_source(rrtl::Program::parse(blob, bytes), int(storage));
+    const std::string src =
+        rrtl::word_kernel_source(rrtl::Program::parse(blob, bytes), int(storage), block_threads);
     *len = src.size();
     if (cap) {
       const size_t n = std::min(cap - 1, src.size());

@@ -32,6 +32,8 @@ struct WordArgs {
   uint64_t* digest; uint32_t* trace;                      // [lanes] / [cycles][n_obs_limbs][lanes]
   uint32_t halt_offset; uint64_t* halt_cycles;            // NONE = no halting
   uint32_t n_scratch, n_sig_words;                        // scratch slots; signal words (LDS signal mode)
+  uint32_t block_lanes;                                   // lanes per workgroup (0: one per thread); a
+                                                          // smaller count leaves the higher threads idle
 };
 
 // Where a block keeps its per-lane scratch and signal values during a launch:
@@ -62,7 +64,7 @@ struct Ctx {
   uint32_t* values; uint32_t* mems; const uint32_t* cpool; const uint32_t* mem_table;
   uint32_t lanes, lane;
   uint32_t* scr; uint32_t sstride;   // HBM scratch (WM_HBM)
-  uint32_t col, stride, sig0;        // LDS: thread column, block stride, first signal row offset
+  uint32_t col, stride, sig0, mem0;  // LDS: thread column, block stride, first signal / memory row offsets
   __device__ __forceinline__ uint32_t ld(uint32_t code, uint32_t limb) const {
     const uint32_t idx = code & ~SP_MASK;
     switch (code & SP_MASK) {
@@ -84,6 +86,11 @@ struct Ctx {
     else values[size_t(off + limb) * lanes + lane] = v;
   }
   __device__ __forceinline__ uint32_t& mem(uint32_t base, uint64_t word) const { return mems[(size_t(base) + word) * lanes + lane]; }
+  // A memory staged in LDS for the launch (generated kernels, WM_ALL_LDS):
+  // word `word` of the memory whose first LDS row is `row`.
+  __device__ __forceinline__ uint32_t& lmem(uint32_t row, uint32_t word) const {
+    return lds_words()[mem0 + (row + word) * stride + col];
+  }
 };
 
 __device__ __forceinline__ uint32_t fmask(uint32_t w) { uint32_t r = w & 31; return r ? ((1u << r) - 1) : 0xFFFFFFFFu; }
@@ -408,6 +415,11 @@ struct InterpExec {
   __device__ __forceinline__ static void run(const WordArgs& A, uint32_t id, const Cx& C) {
     run_records(A.blk[id], A.nrec[id], C);
   }
+  // staging of LDS-resident memories at the start and end of a launch (none here)
+  template <class Cx>
+  __device__ __forceinline__ static void enter(const Cx&) {}
+  template <class Cx>
+  __device__ __forceinline__ static void leave(const Cx&) {}
 };
 
 template <class Cx>
@@ -463,15 +475,18 @@ constexpr uint32_t STEP_OBS = 15;  // sequence entry: observe (else a block id)
 
 template <int kMode, class Exec = InterpExec>
 __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
-  const uint32_t lane = block * blockDim.x + threadIdx.x;
-  if (lane >= A.lanes) return;
+  const uint32_t per_block = A.block_lanes ? A.block_lanes : blockDim.x;
+  const uint32_t lane = block * per_block + threadIdx.x;
+  if (threadIdx.x >= per_block || lane >= A.lanes) return;
   Ctx<kMode> C;
   C.values = A.values; C.mems = A.mems; C.cpool = A.cpool; C.mem_table = A.mem_table;
   C.lanes = A.lanes; C.lane = lane;
   C.scr = A.scratch + lane; C.sstride = A.lanes;
   C.col = threadIdx.x; C.stride = blockDim.x; C.sig0 = A.n_scratch * blockDim.x;
+  C.mem0 = C.sig0 + A.n_sig_words * blockDim.x;
   if constexpr (kMode == WM_ALL_LDS)
     for (uint32_t w = 0; w < A.n_sig_words; w++) lds_words()[C.sig0 + w * C.stride + C.col] = A.values[size_t(w) * A.lanes + lane];
+  Exec::enter(C);
   uint64_t d = A.digest ? A.digest[lane] : 0;
   uint64_t done = 0;
   bool halted = false;
@@ -510,6 +525,7 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
       if (A.halt_offset != NONE && (C.ld(SP_SIGNAL | A.halt_offset, 0) & 1u)) halted = true;
     }
   }
+  Exec::leave(C);
   if constexpr (kMode == WM_ALL_LDS)
     for (uint32_t w = 0; w < A.n_sig_words; w++) A.values[size_t(w) * A.lanes + lane] = lds_words()[C.sig0 + w * C.stride + C.col];
   if (A.digest) A.digest[lane] = d;

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <memory>
+#include <numeric>
 #include <cstring>
 #include <map>
 #include <unordered_map>
@@ -232,12 +233,26 @@ class WordCompiler {
 // store; captures stay in registers until the commit.  Records with results
 // wider than one limb keep the interpreter's code: they are emitted as
 // exec_rec calls on a literal record, which the compiler specialises, with
-// their values in scratch slots.  The executor relies on word_body running
+// their values in scratch slots.  Memories given an LDS row (mem_row) are
+// staged in LDS for the launch -- copied in by enter(), back by leave() -- and
+// accessed there.  The executor relies on word_body running
 // B_COMB always followed by B_ASYNC and B_NEXT always followed by B_COMMIT: it
 // runs each pair as one function (jit_eval, jit_edge).
+// Where a generated kernel finds its state in LDS.  With a nonzero stride (the
+// launch's workgroup size, WM_ALL_LDS) signal and staged-memory words are
+// addressed as constant offsets from the thread's LDS column: no per-access
+// address arithmetic.  mem_row: per memory, its first row in the staged-memory
+// area, or -1 (HBM).
+struct JitLayout {
+  std::vector<int32_t> mem_row;
+  uint32_t stride = 0, slots = 0, sig_words = 0;
+};
+
 class WordJit {
  public:
-  explicit WordJit(const WordProgram& WP) : WP(WP) {
+  explicit WordJit(const WordProgram& WP, JitLayout lay = {}) : WP(WP), lay_(std::move(lay)) {
+    mem_row_ = lay_.mem_row;
+    mem_row_.resize(WP.mem_table.size() / 4, -1);
     std::string eval, edge;
     begin(eval); emit_block(B_COMB); scr_.clear(); emit_block(B_ASYNC);
     begin(edge); emit_block(B_NEXT); scr_.clear(); emit_block(B_COMMIT);
@@ -249,7 +264,7 @@ class WordJit {
             "  __device__ __forceinline__ static void run(const WordArgs&, uint32_t id, const Cx& C) {\n"
             "    if (id == B_COMB) jit_eval(C);\n"
             "    else if (id == B_COMMIT) jit_edge(C);\n"
-            "  }\n"
+            "  }\n" + staging() +
             "};\n\n}  // namespace jit\n}  // namespace word\n}  // namespace rrtl\n";
   }
   // Scratch slots the generated code needs (only records kept in interpreted form use them).
@@ -269,10 +284,44 @@ class WordJit {
     return b;
   }
   static std::string num(uint32_t v) { return std::to_string(v) + "u"; }
+  // enter()/leave(): copy the LDS-staged memories in from HBM and back out.
+  std::string staging() const {
+    std::string in, out;
+    for (size_t m = 0; m < mem_row_.size(); m++) {
+      if (mem_row_[m] < 0) continue;
+      const uint32_t* mt = &WP.mem_table[4 * m];
+      const std::string n = num(mt[1] * mt[2]), row = num(uint32_t(mem_row_[m])), base = num(mt[0]);
+      in += "    for (uint32_t w = 0; w < " + n + "; w++) C.lmem(" + row + ", w) = C.mem(" + base + ", w);\n";
+      out += "    for (uint32_t w = 0; w < " + n + "; w++) C.mem(" + base + ", w) = C.lmem(" + row + ", w);\n";
+    }
+    return "  template <class Cx>\n  __device__ __forceinline__ static void enter(const Cx& C) {\n" + in + "  }\n"
+           "  template <class Cx>\n  __device__ __forceinline__ static void leave(const Cx& C) {\n" + out + "  }\n";
+  }
+  // Word `word` (an expression) of memory m, in LDS or HBM.
+  std::string mem_ref(uint32_t m, const std::string& word) const {
+    const uint32_t* mt = &WP.mem_table[4 * size_t(m)];
+    if (mem_row_[m] >= 0 && lay_.stride)
+      return "L[" + num((lay_.slots + lay_.sig_words + uint32_t(mem_row_[m])) * lay_.stride) + " + (" + word + ") * " +
+             num(lay_.stride) + "]";
+    if (mem_row_[m] >= 0) return "C.lmem(" + num(uint32_t(mem_row_[m])) + ", " + word + ")";
+    return "C.mem(" + num(mt[0]) + ", " + word + ")";
+  }
   static std::string masked(const std::string& e, uint32_t width) {
     return width % 32 ? "(" + e + " & " + hex(final_mask(width)) + ")" : e;
   }
-  void begin(std::string& out) { out_ = &out; sig_.clear(); scr_.clear(); stage_.clear(); }
+  void begin(std::string& out) {
+    out_ = &out; sig_.clear(); scr_.clear(); stage_.clear();
+    if (lay_.stride) line("uint32_t* const L = lds_words() + threadIdx.x;");
+  }
+  // Signal word w as an lvalue expression.
+  std::string sig_ref(uint32_t w) const {
+    if (lay_.stride) return "L[" + num((lay_.slots + w) * lay_.stride) + "]";
+    return "";
+  }
+  void store_sig(uint32_t off, uint32_t l, const std::string& v) {
+    if (lay_.stride) line(sig_ref(off + l) + " = " + v + ";");
+    else line("C.stsig(" + num(off) + ", " + num(l) + ", " + v + ");");
+  }
   void line(const std::string& s) { *out_ += "  " + s + "\n"; }
   std::string fresh(char p) { return p + std::to_string(tmp_++); }
   std::string def(char p, const std::string& e) {
@@ -298,7 +347,7 @@ class WordJit {
         const uint32_t w = idx + limb;
         auto it = sig_.find(w);
         if (it != sig_.end()) return it->second;
-        return sig_[w] = def('s', "C.ld(" + hex(SP_SIGNAL | w) + ", 0)");
+        return sig_[w] = def('s', lay_.stride ? sig_ref(w) : "C.ld(" + hex(SP_SIGNAL | w) + ", 0)");
       }
       default: {
         auto it = scr_.find(idx + limb);
@@ -394,7 +443,7 @@ class WordJit {
         const std::string o = fresh('o');
         line("const bool " + o + " = " + ok + ";");
         // out-of-range lanes read word 0 (in bounds) and take 0
-        const std::string m = def('m', "C.mem(" + num(mt[0]) + ", uint64_t(" + o + " ? " + lo + " : 0u) * " + num(mt[2]) + ")");
+        const std::string m = def('m', mem_ref(R.w[6], "uint32_t(" + o + " ? " + lo + " : 0u) * " + num(mt[2])));
         return "(" + o + " ? " + (mt[2] == 1 ? masked(m, mt[3]) : m) + " : 0u)";
       }
       default: return "";
@@ -437,7 +486,7 @@ class WordJit {
           std::string v = l < R.w[7] ? opnd(R.w[2], l) : "0u";
           if (l + 1 == L) v = masked(v, R.w[5]);
           const std::string n = def('w', v);
-          line("C.stsig(" + num(R.w[1]) + ", " + num(l) + ", " + n + ");");
+          store_sig(R.w[1], l, n);
           sig_[R.w[1] + l] = n;
         }
         break;
@@ -449,7 +498,7 @@ class WordJit {
           if (l + 1 == L) v[l] = masked(v[l], R.w[5]);
         }
         line("if (" + r + ") {");
-        for (uint32_t l = 0; l < L; l++) line("  C.stsig(" + num(R.w[1]) + ", " + num(l) + ", " + v[l] + ");");
+        for (uint32_t l = 0; l < L; l++) store_sig(R.w[1], l, v[l]);
         line("}");
         for (uint32_t l = 0; l < L; l++) sig_.erase(R.w[1] + l);
         break;
@@ -468,7 +517,7 @@ class WordJit {
         for (uint32_t l = 0; l < L; l++) {
           auto it = stage_.find((R.w[2] & ~SP_MASK) + l);
           if (it == stage_.end()) throw Error("E_PROGRAM: register commit without a capture");
-          line("C.stsig(" + num(R.w[1]) + ", " + num(l) + ", " + it->second + ");");
+          store_sig(R.w[1], l, it->second);
           sig_[R.w[1] + l] = it->second;
         }
         break;
@@ -486,7 +535,7 @@ class WordJit {
         }
         line("if ((" + en + " & 1u) && " + ok + ") {");
         for (uint32_t l = 0; l < L; l++)
-          line("  C.mem(" + num(mt[0]) + ", uint64_t(" + lo + ") * " + num(mt[2]) + " + " + num(l) + ") = " + v[l] + ";");
+          line("  " + mem_ref(R.w[1], lo + " * " + num(mt[2]) + " + " + num(l)) + " = " + v[l] + ";");
         line("}");
         break;
       }
@@ -505,12 +554,35 @@ class WordJit {
   }
 
   const WordProgram& WP;
+  JitLayout lay_;
+  std::vector<int32_t> mem_row_;  // per memory: first LDS row, or -1 (HBM)
   std::string body_;
   std::string* out_ = nullptr;
   std::unordered_map<uint32_t, std::string> sig_, scr_, stage_;  // word -> register holding it
   uint32_t tmp_ = 0, n_generic_ = 0;
   bool generic_ = false;
 };
+
+// Memories a generated kernel stages in LDS (WM_ALL_LDS): small ones (a
+// register file, a FIFO's storage), smallest first, while a block of `block`
+// lanes keeps its scratch, signals and staged memories within 64 KiB.  Returns
+// each memory's first LDS row (-1: stays in HBM); *used = staged words per lane.
+std::vector<int32_t> lds_memory_rows(const Program& P, uint32_t slots, uint32_t block, uint32_t* used) {
+  std::vector<uint32_t> order(P.memories.size());
+  std::iota(order.begin(), order.end(), 0u);
+  auto words = [&](uint32_t m) { return uint64_t(P.memories[m].depth) * P.memories[m].limbs; };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return words(a) < words(b); });
+  std::vector<int32_t> row(P.memories.size(), -1);
+  const uint64_t fixed = uint64_t(slots) + P.total_signal_words;
+  *used = 0;
+  for (uint32_t m : order) {
+    const uint64_t w = words(m);
+    if (w == 0 || w > 1024 || (fixed + *used + w) * block * 4 > 64 * 1024) continue;
+    row[m] = int32_t(*used);
+    *used += uint32_t(w);
+  }
+  return row;
+}
 
 // ------------------------------------------------------------------ engine
 class WordEngine final : public Engine {
@@ -535,16 +607,37 @@ class WordEngine final : public Engine {
     values_.alloc(size_t(P.total_signal_words) * lanes); values_.zero(stream_);
     mems_.alloc(size_t(P.total_memory_words) * lanes); mems_.zero(stream_);
     // One lane per thread: blocks of 256 lanes, smaller (down to one wave)
-    // when that is what it takes to give every CU a block.
-    if (opt.block_threads) {
+    // when that is what it takes to give every CU a block.  A batch too small
+    // to give every SIMD (four per CU) a wave runs 32 or 16 lanes per wave
+    // instead: a wave's time per cycle is a dependent chain that barely
+    // depends on how many of its lanes are active, so more, thinner waves
+    // finish the batch sooner.
+    const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
+    if (opt.wave_lanes) {
+      wave_lanes_ = opt.wave_lanes;  // 4, 8, 16, 32 or 64
+    } else {
+      wave_lanes_ = 64;
+      while (wave_lanes_ > 16 && (uint64_t(lanes) + wave_lanes_ - 1) / wave_lanes_ < 4ull * cus) wave_lanes_ /= 2;
+    }
+    if (wave_lanes_ < 64) {
+      block_ = 64;
+    } else if (opt.block_threads) {
       block_ = opt.block_threads;
     } else {
-      const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
       const uint32_t per_cu = (lanes + cus - 1) / cus;
       block_ = std::min<uint32_t>(256, std::max<uint32_t>(64, (per_cu + 63) / 64 * 64));
     }
     mode_ = lds_mode(block_, run_slots_);
     use_lds_ = mode_ != WM_HBM;
+    if (jit) {
+      // the final layout: LDS-staged memories and constant LDS offsets
+      JitLayout lay;
+      if (mode_ == WM_ALL_LDS) {
+        lay.mem_row = lds_memory_rows(P, run_slots_, block_, &lds_mem_words_);
+        lay.stride = block_; lay.slots = run_slots_; lay.sig_words = P.total_signal_words;
+      }
+      jit.reset(new WordJit(WP, std::move(lay)));
+    }
     if (jit) jit_fn_ = rtc_kernel(jit->source(mode_), "rrtl_word_jit", opt.device);
     if (!use_lds_) { scratch_.alloc(size_t(WP.scratch_slots) * lanes); scratch_.zero(stream_); }
     for (int b = 0; b < 4; b++) {
@@ -626,6 +719,7 @@ class WordEngine final : public Engine {
     if (mode == WM_HBM && !scratch_.p) { scratch_.alloc(size_t(WP.scratch_slots) * lanes_); scratch_.zero(stream_); }
     WordArgs A = make_args(run_flags(spec), spec.cycles, &spec, true);
     A.n_scratch = WP.scratch_slots;
+    A.block_lanes = 0;  // 256 lanes per batch item
     A.scratch = mode != WM_HBM ? nullptr : scratch_.p;
     BatchJob J;
     J.kind = ENGINE_WORD; J.variant = uint32_t(mode);
@@ -710,10 +804,11 @@ class WordEngine final : public Engine {
     return reinterpret_cast<T*>(b.p);
   }
 
-  // LDS bytes a block of `block` lanes with `slots` scratch slots needs in `mode`.
-  size_t lds_bytes(int mode, uint32_t block, uint32_t slots) const {
+  // LDS bytes a block of `block` lanes with `slots` scratch slots (and, in
+  // WM_ALL_LDS, `mem_words` words of LDS-staged memories) needs in `mode`.
+  size_t lds_bytes(int mode, uint32_t block, uint32_t slots, uint32_t mem_words = 0) const {
     if (mode == WM_HBM) return 0;
-    const size_t words = slots + (mode == WM_ALL_LDS ? P.total_signal_words : 0);
+    const size_t words = slots + (mode == WM_ALL_LDS ? size_t(P.total_signal_words) + mem_words : 0);
     return words * block * 4;
   }
   // Signals join scratch in LDS when both fit in 64 KiB per block.
@@ -726,6 +821,7 @@ class WordEngine final : public Engine {
   WordArgs make_args(uint32_t flags, uint64_t cycles, const RunSpec* spec, bool outputs) {
     WordArgs A{};
     A.n_scratch = run_slots_; A.n_sig_words = P.total_signal_words;
+    A.block_lanes = wave_lanes_ < 64 ? wave_lanes_ : 0;
     A.values = values_.p; A.mems = mems_.p; A.scratch = use_lds_ ? nullptr : scratch_.p;
     for (int b = 0; b < 4; b++) { A.blk[b] = reinterpret_cast<const Rec*>(blk_[b].p); A.nrec[b] = uint32_t(WP.blk[b].size()); }
     A.cpool = cpool_.p; A.mem_table = mtab_.p; A.lanes = lanes_; A.flags = flags; A.cycles = cycles;
@@ -786,8 +882,9 @@ class WordEngine final : public Engine {
   // waits, so the sequence can be captured into a graph).  Returns the launches.
   uint64_t enqueue(const WordArgs& A, uint64_t cycles, bool outputs, hipStream_t s) {
     const uint64_t chunk = (opt_.ticks_per_launch && outputs) ? opt_.ticks_per_launch : std::max<uint64_t>(cycles, 1);
-    const uint32_t grid = (lanes_ + block_ - 1) / block_;
-    const size_t lds = lds_bytes(mode_, block_, run_slots_);
+    const uint32_t per_block = wave_lanes_ < 64 ? wave_lanes_ : block_;
+    const uint32_t grid = (lanes_ + per_block - 1) / per_block;
+    const size_t lds = lds_bytes(mode_, block_, run_slots_, lds_mem_words_);
     uint64_t done = 0, launches = 0;
     do {
       WordArgs B = A;
@@ -849,9 +946,11 @@ class WordEngine final : public Engine {
   WordArgs async_args_{};               // the run of a concurrent batch
   uint64_t async_cycles_ = 0;
   uint32_t run_slots_ = 0;              // scratch slots of the engine's own launches
+  uint32_t lds_mem_words_ = 0;          // words per lane of memories staged in LDS by the generated kernel
   hipFunction_t jit_fn_ = nullptr;      // the design's generated kernel (null: record interpreter)
   size_t lds_max_ = 65536;
   uint32_t block_ = 256;
+  uint32_t wave_lanes_ = 64;            // active lanes per 64-thread wave (16, 32 or 64)
   int mode_ = WM_SCRATCH_LDS;
   bool use_lds_ = true, has_digest_ = false, has_halt_ = false, batch_spec_halt_ = false, batch_digest_ = false;
   size_t trace_bytes_ = 0;
@@ -862,9 +961,18 @@ class WordEngine final : public Engine {
 
 }  // namespace
 
-std::string word_kernel_source(const Program& prog, int mode) {
+std::string word_kernel_source(const Program& prog, int mode, uint32_t block) {
   if (mode < WM_HBM || mode > WM_ALL_LDS) throw Error("E_HIP_OPTIONS: storage mode must be 0, 1 or 2");
+  if (block % 64 || block > 256) throw Error("E_HIP_OPTIONS: block_threads must be 0, 64, 128, 192 or 256");
   const WordProgram wp = WordCompiler(prog).compile();
+  if (mode == WM_ALL_LDS && block) {
+    JitLayout lay;
+    lay.slots = WordJit(wp).uses_scratch() ? wp.scratch_slots : 0;
+    uint32_t used = 0;
+    lay.mem_row = lds_memory_rows(prog, lay.slots, block, &used);
+    lay.stride = block; lay.sig_words = prog.total_signal_words;
+    return WordJit(wp, std::move(lay)).source(mode);
+  }
   return WordJit(wp).source(mode);
 }
 

@@ -37,18 +37,18 @@ def test_sum_program_closed_form():
     assert all(t == 1 for t in sim.get_signal("trap"))
 
 
-@pytest.mark.parametrize("codegen", ["on", "off"])
+@pytest.mark.parametrize("codegen,wave_lanes", [("on", 0), ("off", 0), ("on", 4), ("on", 8), ("on", 32), ("off", 16)])
 @pytest.mark.parametrize("lanes,n,cap,seed", [(300, 64, 1500, 1), (1024, 512, 4000, 2)])
-def test_random_programs_match_iss(lanes, n, cap, seed, codegen):
+def test_random_programs_match_iss(lanes, n, cap, seed, codegen, wave_lanes):
     """Uneven halting: each lane stops after the cycle whose edge raised trap
     (its ecall, so the cycle count equals the instructions the ISS retired,
     the ecall included); lanes that loop run to the cap.  Halt cycles,
     registers and memory equal the ISS's."""
-    flush_print(f"case: rv32i random programs lanes={lanes} n={n} cap={cap} codegen={codegen}")
+    flush_print(f"case: rv32i random programs lanes={lanes} n={n} cap={cap} codegen={codegen} wave_lanes={wave_lanes}")
     p = lower.lower_module(riscv.rv32i_core())
     rng = random.Random(seed)
     progs = [riscv.random_program(rng, n=n) for _ in range(lanes)]
-    sim = HipSimulator(p, lanes, engine="word", codegen=codegen)
+    sim = HipSimulator(p, lanes, engine="word", codegen=codegen, wave_lanes=wave_lanes)
     load_programs(sim, progs)
     sim.run(cap, inputs=core_inputs(p), halt="trap")
     halt = sim.halt_cycles()

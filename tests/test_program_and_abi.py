@@ -122,9 +122,19 @@ def test_generated_kernels_compile_for_gfx950():
     (straight-line code, no interpreted records); a 100-bit counter keeps its
     wide records in interpreted form; every storage mode compiles."""
     from rrtl_amd import riscv
-    rv = backend.emit_kernel(lower.lower_module(riscv.rv32i_core()))
-    assert "exec_rec" not in rv and "rrtl_word_jit" in rv
+    core = lower.lower_module(riscv.rv32i_core())
+    rv = backend.emit_kernel(core)
+    assert "exec_rec" not in rv and "rrtl_word_jit" in rv and "C.lmem" not in rv
     assert backend.compile_kernel(rv) > 0
+    # at 64 lanes per workgroup the 32-word register file is staged in LDS
+    # (after the core's signal words) and addressed at constant offsets from the
+    # thread's column; the 1024-word memory stays in HBM
+    staged = backend.emit_kernel(core, backend.STORAGE_ALL_LDS, 64)
+    sig_words = sum(s.limbs for s in core.signals)
+    assert "C.lmem(0u, w) = C.mem(1024u, w)" in staged
+    assert f"L[{sig_words * 64}u + (" in staged.split("jit_eval")[1]
+    assert "C.mem(0u, uint32_t(" in staged
+    assert backend.compile_kernel(staged) > 0
     wide = lower.lower_module(designs.counter(100))
     for storage in (backend.STORAGE_HBM, backend.STORAGE_SCRATCH_LDS, backend.STORAGE_ALL_LDS):
         src = backend.emit_kernel(wide, storage)
@@ -139,12 +149,12 @@ def test_generated_kernel_abi_errors():
     p = lower.lower_module(designs.counter())
     blob = p.to_blob()
     n = ctypes.c_size_t()
-    assert lib.rrtl_hip_emit_kernel(blob, len(blob), 3, None, 0, ctypes.byref(n)) != 0
+    assert lib.rrtl_hip_emit_kernel(blob, len(blob), 3, 0, None, 0, ctypes.byref(n)) != 0
     assert lib.rrtl_hip_create_error().decode().startswith("E_HIP_OPTIONS")
     # a short buffer gets a NUL-terminated prefix and the full length
     full = backend.emit_kernel(p)
     buf = ctypes.create_string_buffer(17)
-    assert lib.rrtl_hip_emit_kernel(blob, len(blob), 2, buf, 17, ctypes.byref(n)) == 0
+    assert lib.rrtl_hip_emit_kernel(blob, len(blob), 2, 0, buf, 17, ctypes.byref(n)) == 0
     assert n.value == len(full) and buf.value.decode() == full[:16]
     with pytest.raises(backend.HipError) as e:
         backend.compile_kernel('#include "word_device.hpp"\n__global__ void k() { undefined_name(); }\n')
