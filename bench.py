@@ -244,16 +244,68 @@ def bench_rv32i(args) -> int:
     wall = time.perf_counter() - t0
     st = sim.stats()
     halt = sim.halt_cycles()
+    value = st["lane_cycles"] / (st["kernel_ms"] / 1e3)
+    # issue-bound path: the lowered program's operations per simulated cycle
+    # (every machine instruction and effect of the four blocks) against the
+    # int32 VALU peak (256 CUs x 64 lanes x 2.4 GHz)
+    from rrtl_amd.program import OP_LIT, OP_SIGNAL
+    ops = sum(sum(1 for i in pk.instrs if i.op not in (OP_LIT, OP_SIGNAL)) + len(pk.effects)
+              for b in job.program.blocks for pk in b.packets)
+    valu_peak = 256 * 64 * 2.4e9 / 1e12
+    achieved = value * ops / 1e12
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = rv32i_cpu_baseline(job, args)
     line = {"metric": "RV32I per-lane programs: simulated lane-cycles/s (halted lanes stop)",
-            "value": st["lane_cycles"] / (st["kernel_ms"] / 1e3), "unit": "lane-cycles/s", "n_gpus": 1,
-            "higher_is_better": True, "data": "synthetic (random RV32I programs)",
+            "value": value, "unit": "lane-cycles/s", "n_gpus": 1,
+            "higher_is_better": True, "data": "synthetic (random RV32I programs)", "dtype": "u32",
             "wall_ms": wall * 1e3, "kernel_ms": st["kernel_ms"],
             "config": {"workload": f"rv32i core, 16384 lanes, 512-instruction random programs, cap {args.cycles}",
                        "codegen": args.codegen,
                        "halt_cycles_min": int(halt.min()), "halt_cycles_max": int(halt.max()),
-                       "lanes_at_cap": int((halt >= args.cycles).sum())}}
+                       "lanes_at_cap": int((halt >= args.cycles).sum())},
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": valu_peak, "unit": "Tops/s",
+                         "frac": achieved / valu_peak, "traffic": None, "ops_per_lane_cycle": ops},
+            "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     return 0
+
+
+def rv32i_cpu_baseline(job, args) -> dict:
+    """The host batch-SIMD engine on all 16,384 lanes for a bounded number of
+    cycles (no halting: the reference's engine steps every lane), and the GPU
+    on the same lanes and cycles for a bit-exactness check of registers,
+    memory, pc and trap."""
+    import oracle
+    p = job.program
+    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
+    ins = job.run["inputs"]
+
+    class Images:   # captures the job's memory images
+        lanes = job.lanes
+
+        def set_memory_limbs(self, name, v):
+            self.mem = v
+    img = Images()
+    job.setup(img)
+
+    def cpu_run(cycles):
+        eng = oracle.SimdCpu(p, job.lanes, threads=threads)
+        eng.set_memory_limbs(p.memory_index("mem"), img.mem)
+        _, secs = eng.run(job.run["seed"], cycles, ins, [])
+        return eng, secs
+    probe = 10
+    _, secs = cpu_run(probe)
+    cyc = max(probe, min(args.cycles, int(probe * args.cpu_seconds / max(secs, 1e-6))))
+    eng, secs = cpu_run(cyc)
+    gpu = job.make(codegen=args.codegen)
+    gpu.run(cyc, seed=job.run["seed"], inputs=ins)
+    same = all(np.array_equal(gpu.get_memory_limbs(m), eng.get_memory_limbs(p.memory_index(m))) for m in ("regs", "mem"))
+    for name in ("pc", "trap"):
+        same &= bool(np.array_equal(gpu.get_signal_limbs(name), eng.get_signal_limbs(p.signal_index(name))))
+    gpu.close()
+    return {"value": job.lanes * cyc / secs, "unit": "lane-cycles/s", "cores": threads, "kind": "port",
+            "sample": f"{job.lanes} lanes x {cyc} cycles, every lane stepped ({secs:.1f} s)", "bit_exact_vs_gpu": same}
 
 
 def bench_faultsim(args) -> int:

@@ -316,6 +316,8 @@ def _simd_lib():
         L.simd_tick_many.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.simd_get_signal_limbs.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
         L.simd_set_signal_limbs.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.simd_set_memory_limbs.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.simd_get_memory_limbs.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
         _simd = L
     return _simd
 
@@ -358,3 +360,15 @@ class SimdCpu:
     def set_signal_limbs(self, sig, vals) -> None:
         v = np.ascontiguousarray(vals, dtype=np.uint32).reshape(self.program.signals[sig].limbs, self.lanes)
         self.L.simd_set_signal_limbs(self.h, sig, _ptr(v))
+
+    def set_memory_limbs(self, mem, vals) -> None:
+        """vals: [lanes][depth][limbs]; re-settles comb."""
+        m = self.program.memories[mem]
+        v = np.ascontiguousarray(vals, dtype=np.uint32).reshape(self.lanes, m.depth, m.limbs)
+        self.L.simd_set_memory_limbs(self.h, mem, _ptr(v))
+
+    def get_memory_limbs(self, mem) -> np.ndarray:
+        m = self.program.memories[mem]
+        out = np.zeros((self.lanes, m.depth, m.limbs), dtype=np.uint32)
+        self.L.simd_get_memory_limbs(self.h, mem, _ptr(out))
+        return out

@@ -426,3 +426,30 @@ void simd_set_signal_limbs(simd_engine *e, uint32_t sig, const uint32_t *vals) {
         eval_comb(s);
     }
 }
+
+/* memory access by lane: [lanes][depth][limbs] (set_memory_limbs,
+ * crates/rrtl-sim-ir/src/lib.rs:10136); a write re-settles comb like
+ * set_packed_signal_limbs */
+void simd_set_memory_limbs(simd_engine *e, uint32_t mem, const uint32_t *vals) {
+    const p_memory *m = &e->P.mem[mem];
+    const size_t per_lane = (size_t)m->depth * m->limbs;
+    for (uint32_t i = 0; i < e->nshards; i++) {
+        simd_shard *s = &e->shards[i];
+        for (uint32_t lane = 0; lane < s->lanes; lane++)
+            for (size_t k = 0; k < per_lane; k++) {
+                uint32_t v = vals[(size_t)(s->lane_base + lane) * per_lane + k];
+                if (k % m->limbs == m->limbs - 1) v &= final_mask(m->width);
+                s->memories[(size_t)lane * e->P.tmw + m->offset + k] = v;
+            }
+        eval_comb(s);
+    }
+}
+void simd_get_memory_limbs(const simd_engine *e, uint32_t mem, uint32_t *out) {
+    const p_memory *m = &e->P.mem[mem];
+    const size_t per_lane = (size_t)m->depth * m->limbs;
+    for (uint32_t i = 0; i < e->nshards; i++) {
+        const simd_shard *s = &e->shards[i];
+        for (uint32_t lane = 0; lane < s->lanes; lane++)
+            memcpy(&out[(size_t)(s->lane_base + lane) * per_lane], &s->memories[(size_t)lane * e->P.tmw + m->offset], 4 * per_lane);
+    }
+}

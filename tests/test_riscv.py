@@ -77,3 +77,26 @@ def test_core_matches_iss_on_packed_oracle():
     for l, s in enumerate(iss):
         assert [int(x) for x in regs[l, :, 0]] == s.regs, l
         assert [int(x) for x in mem[l, :, 0]] == s.mem, l
+
+
+def test_host_simd_engine_matches_iss():
+    """The host batch-SIMD restatement (the RV32I CPU baseline) on random
+    programs, lane-sharded over threads: registers and memory equal the ISS's
+    after the same number of cycles (the core holds after trap)."""
+    from rrtl_amd.backend import IN_CONST
+    p = lower.lower_module(riscv.rv32i_core())
+    lanes, cycles = 37, 150
+    rng = random.Random(9)
+    progs = [riscv.random_program(rng, n=64) for _ in range(lanes)]
+    cpu = oracle.SimdCpu(p, lanes, threads=3, shard_lanes=8)
+    img = np.array([load_image(pr) for pr in progs], dtype=np.uint32)
+    cpu.set_memory_limbs(p.memory_index("mem"), img.reshape(lanes, -1, 1))
+    ins = [(p.signal_index("clk"), IN_CONST, 1), (p.signal_index("rst"), IN_CONST, 0)]
+    cpu.run(1, cycles, ins, [])
+    regs = cpu.get_memory_limbs(p.memory_index("regs"))
+    mem = cpu.get_memory_limbs(p.memory_index("mem"))
+    for l, pr in enumerate(progs):
+        s = Iss(load_image(pr))
+        s.run(cycles)
+        assert [int(x) for x in regs[l, :, 0]] == s.regs, l
+        assert [int(x) for x in mem[l, :, 0]] == s.mem, l
