@@ -31,8 +31,7 @@ __host__ __device__ inline uint32_t gate_eval(uint32_t op, uint32_t a, uint32_t 
 struct TileArgs {
   uint32_t* state; uint64_t nwords;             // HBM rows [node][nwords]
   const uint2* desc;                            // [stage][position][thread], STAGE_AHEAD padding stages
-  const uint32_t* stage_fix;                    // per stage: bit k = position k applies output inversions
-  uint32_t n_cycle_stages, n_post_stages, any_fix;
+  uint32_t n_cycle_stages, n_post_stages, any_fix;  // any_fix: some gate applies its output inversion
   const uint32_t* load_slot; const uint32_t* load_node; uint32_t n_load;
   const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
   const uint4* commit; uint32_t n_commit;
@@ -123,14 +122,16 @@ __device__ __forceinline__ void load_stage(StageDesc<M>& D, const uint2* row) {
 
 // One stage of this thread: M gate positions over every lane-word of the tile,
 //   t = x ? A ^ B : A & B,  A = a ^ ia, B = b ^ ib  (per-thread masks),
-// stored as is (its output inversion is carried by the readers) unless the
-// stage's fix bit for the position says some gate there applies it (t ^ io).
+// stored as is: a gate's output inversion is carried by its readers, except
+// for the gates whose io bit is set (values that must hold their true
+// polarity), which store t ^ io.  kFix: some gate of the program sets io (else
+// the XOR is compiled out).
 // Position 0 reads a and b; a later position takes a from the previous
 // position's t (registers) and reads b.  All LDS reads of the stage are issued
 // first (their operands come from earlier stages), then the positions run in
 // order, each storing its result.
-template <int W, bool kBA, int M>
-__device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D, uint32_t fix) {
+template <int W, bool kBA, int M, bool kFix>
+__device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D) {
   Words<W> va, vb[M];
   va = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[0].x & 0xFFFFu));
 #pragma unroll
@@ -143,7 +144,7 @@ __device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D,
     Words<W> r;
 #pragma unroll
     for (int w = 0; w < W; w++) r.v[w] = op3_xor_or_and((k == 0 ? va.v[w] : prev.v[w]) ^ ia, vb[k].v[w] ^ ib, x);
-    if (fix >> k & 1u) {
+    if constexpr (kFix) {
       const uint32_t io = 0u - ((y >> 18) & 1u);
 #pragma unroll
       for (int w = 0; w < W; w++) r.v[w] ^= io;
@@ -287,7 +288,7 @@ __device__ __forceinline__ Words<W> stimulus(const TileArgs& A, uint32_t s, uint
 template <int W, bool kBA, int M, bool kClamp, bool kFix>
 __device__ __forceinline__ void stage_step(const TileArgs& A, uint32_t* lds, uint32_t S, StageDesc<M>& D, uint32_t fb,
                                            uint32_t fe) {
-  eval_stage<W, kBA, M>(lds, D, kFix ? A.stage_fix[S] : 0u);
+  eval_stage<W, kBA, M, kFix>(lds, D);
   load_stage<M>(D, A.desc + size_t(S + STAGE_AHEAD) * (uint32_t(M) * TILE_THREADS));
   __syncthreads();
   if (kClamp) { apply_clamps<W>(A, lds, S + 1, fb, fe); __syncthreads(); }

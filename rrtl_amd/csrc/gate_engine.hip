@@ -1102,9 +1102,6 @@ class GateEngine final : public Engine {
       }
     }
     A.desc = upload_prog(0, desc);
-    std::vector<uint32_t> fix(T.stage_fix);
-    fix.resize(KS + STAGE_AHEAD, 0u);
-    A.stage_fix = upload_prog(1, fix);
     A.any_fix = 0;
     for (uint32_t f : T.stage_fix) A.any_fix |= f;
     A.n_cycle_stages = T.n_cycle_stages; A.n_post_stages = T.n_post_stages;
