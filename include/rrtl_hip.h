@@ -132,7 +132,9 @@ rrtl_status rrtl_hip_synchronize(rrtl_hip_engine* e);         /* InterpGpuSimula
  * (POST) or nothing; the state after the run equals the reference's after its
  * trailing settle. */
 enum { RRTL_IN_HOLD = 0, RRTL_IN_RANDOM = 1, RRTL_IN_CONST = 2, RRTL_IN_PULSE = 3, RRTL_IN_TRACE = 4,
-       RRTL_IN_GROUP_RANDOM = 5 };
+       RRTL_IN_GROUP_RANDOM = 5,
+       /* word engine: values shared by every lane, [cycles][uniform-trace limbs] (rrtl_hip_replay) */
+       RRTL_IN_TRACE_UNIFORM = 6 };
 enum { RRTL_OBSERVE_PRE = 0, RRTL_OBSERVE_POST = 1 };
 typedef struct rrtl_hip_run_spec {
   uint32_t struct_size;
@@ -180,6 +182,45 @@ rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out);
 rrtl_status rrtl_hip_get_detect(rrtl_hip_engine* e, uint32_t* out_words);
 /* Gate engine: [tile_words, live_slots, levels, gates] of the last compiled run. */
 rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4);
+
+/* ---- trace replay (SimBackend::replay_trace, crates/rrtl-sim-ir/src/lib.rs:5270-5345,
+ * with ReplayOptions / ReplayReport :3463-3491, :4328-4407) moved on chip: per
+ * step the inputs are set (raw), comb is evaluated, every active check compares
+ * its signal with the expected value, and the edge runs with no settle after it
+ * (tick_from_evaluated_no_post_eval) -- one launch for the whole plan.  Word
+ * engine; checked signals up to 128 bits.  Inputs and checks are fixed lists;
+ * an input that a plan step leaves unset repeats its previous value, a check
+ * absent from a step is cleared in check_active. */
+enum { RRTL_REPLAY_REPLICATED = 0, RRTL_REPLAY_INDEPENDENT = 1 };  /* ReplayLaneMode */
+enum { RRTL_REPLAY_LANE0 = 0, RRTL_REPLAY_ALL_LANES = 1 };         /* ReplayCheckMode (Lane0Fast, AllLanes) */
+typedef struct rrtl_hip_replay_spec {
+  uint32_t struct_size;
+  uint64_t steps;
+  uint32_t lane_mode;            /* REPLICATED: one value per step for every lane; INDEPENDENT: one per lane */
+  uint32_t check_mode;           /* REPLICATED + LANE0 checks lane 0 only; otherwise every lane */
+  uint32_t n_inputs;
+  const uint32_t* input_signal;
+  const uint32_t* input_values;  /* REPLICATED [steps][input limbs]; INDEPENDENT [steps][input limbs][lanes] */
+  uint32_t n_checks;
+  const uint32_t* check_signal;
+  const uint32_t* check_index;   /* reported check_index of each check, or NULL: its position */
+  const uint32_t* expected;      /* REPLICATED [steps][check limbs]; INDEPENDENT [steps][check limbs][lanes] */
+  const uint32_t* check_active;  /* [steps][ceil(n_checks / 32)] bit j = check j applies, or NULL: all */
+  uint32_t max_mismatches;       /* ReplayOptions::max_mismatches */
+} rrtl_hip_replay_spec;
+/* ReplayMismatch: step, check_index, lane, expected and actual (u128 as 4 limbs) */
+typedef struct rrtl_hip_replay_mismatch {
+  uint64_t step;
+  uint32_t check_index;
+  uint32_t lane;
+  uint32_t expected[4];
+  uint32_t actual[4];
+} rrtl_hip_replay_mismatch;
+/* *mismatch_count = every mismatch (ReplayReport::mismatch_count); the first
+ * max_mismatches in the reference's order (step, check, lane) go to
+ * `mismatches` (capacity max_mismatches), their number to *n_mismatches. */
+rrtl_status rrtl_hip_replay(rrtl_hip_engine* e, const rrtl_hip_replay_spec* spec, uint64_t* mismatch_count,
+                            rrtl_hip_replay_mismatch* mismatches, uint32_t* n_mismatches);
 
 /* ---- per-design code generation (gpu_codegen::emit_kernel,
  * crates/rrtl-sim-ir/src/gpu_codegen.rs:503, which returns a design's CUDA /

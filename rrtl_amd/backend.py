@@ -25,7 +25,9 @@ from .program import Program
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librrtl_hip.so")
 
 RRTL_HIP_AUTO, RRTL_HIP_WORD, RRTL_HIP_GATE = 0, 1, 2
-IN_HOLD, IN_RANDOM, IN_CONST, IN_PULSE, IN_TRACE, IN_GROUP_RANDOM = 0, 1, 2, 3, 4, 5
+IN_HOLD, IN_RANDOM, IN_CONST, IN_PULSE, IN_TRACE, IN_GROUP_RANDOM, IN_TRACE_UNIFORM = 0, 1, 2, 3, 4, 5, 6
+REPLAY_REPLICATED, REPLAY_INDEPENDENT = 0, 1   # ReplayLaneMode
+REPLAY_LANE0, REPLAY_ALL_LANES = 0, 1          # ReplayCheckMode (Lane0Fast, AllLanes)
 OBSERVE_PRE, OBSERVE_POST = 0, 1
 NO_HALT = 0xFFFFFFFF
 ENGINE_NAMES = {"auto": RRTL_HIP_AUTO, "word": RRTL_HIP_WORD, "gate": RRTL_HIP_GATE}
@@ -44,8 +46,22 @@ EXPORTS = [
     "rrtl_hip_get_detect",
     "rrtl_hip_gate_tile_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
     "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy", "rrtl_hip_emit_kernel", "rrtl_hip_compile_kernel",
+    "rrtl_hip_replay",
 ]
 BATCH_MODES = {"persistent": 0, "per_design": 1, "concurrent": 2, "streams": 3}
+
+
+class ReplaySpec(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("steps", ctypes.c_uint64), ("lane_mode", ctypes.c_uint32),
+                ("check_mode", ctypes.c_uint32), ("n_inputs", ctypes.c_uint32), ("input_signal", ctypes.c_void_p),
+                ("input_values", ctypes.c_void_p), ("n_checks", ctypes.c_uint32), ("check_signal", ctypes.c_void_p),
+                ("check_index", ctypes.c_void_p), ("expected", ctypes.c_void_p), ("check_active", ctypes.c_void_p),
+                ("max_mismatches", ctypes.c_uint32)]
+
+
+class ReplayMismatch(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_uint64), ("check_index", ctypes.c_uint32), ("lane", ctypes.c_uint32),
+                ("expected", ctypes.c_uint32 * 4), ("actual", ctypes.c_uint32 * 4)]
 
 
 class HipError(RuntimeError):
@@ -125,6 +141,8 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_batch_destroy.argtypes = [vp]
     L.rrtl_hip_emit_kernel.argtypes = [ctypes.c_char_p, sz, u32, u32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
     L.rrtl_hip_compile_kernel.argtypes = [ctypes.c_char_p, ctypes.POINTER(sz)]
+    L.rrtl_hip_replay.argtypes = [vp, ctypes.POINTER(ReplaySpec), ctypes.POINTER(u64), ctypes.POINTER(ReplayMismatch),
+                                  ctypes.POINTER(u32)]
     for fn in EXPORTS:
         if fn != "rrtl_hip_destroy" and getattr(L, fn).restype is ctypes.c_int:
             getattr(L, fn).restype = ctypes.c_int
@@ -375,6 +393,37 @@ class HipSimulator:
         out = np.zeros(n // 4, dtype=np.uint32)
         self._check(self.L.rrtl_hip_get_trace(self.h, _ptr(out)))
         return out
+
+    def replay(self, inputs: Sequence[SignalRef], input_values: np.ndarray, checks: Sequence[SignalRef],
+               expected: np.ndarray, lane_mode: int = REPLAY_REPLICATED, check_mode: int = REPLAY_LANE0,
+               check_active: Optional[np.ndarray] = None, check_index: Optional[Sequence[int]] = None,
+               max_mismatches: int = 16) -> dict:
+        """Trace replay on the device (``SimBackend::replay_trace``,
+        crates/rrtl-sim-ir/src/lib.rs:5270).  ``input_values``: replicated
+        [steps][input limbs] or independent [steps][input limbs][lanes] u32;
+        ``expected`` likewise over the checked signals' limbs; ``check_active``
+        [steps][ceil(checks/32)] u32 bits or None.  Returns the mismatch count
+        and the first ``max_mismatches`` as (step, check_index, lane, expected,
+        actual) with values as Python ints."""
+        isig = np.array([self.sig(x) for x in inputs] or [0], dtype=np.uint32)
+        csig = np.array([self.sig(x) for x in checks] or [0], dtype=np.uint32)
+        iv = np.ascontiguousarray(input_values, dtype=np.uint32)
+        ev = np.ascontiguousarray(expected, dtype=np.uint32)
+        steps = int(iv.shape[0]) if len(inputs) else int(ev.shape[0])
+        ca = None if check_active is None else np.ascontiguousarray(check_active, dtype=np.uint32)
+        ci = None if check_index is None else np.ascontiguousarray(check_index, dtype=np.uint32)
+        sp = ReplaySpec(ctypes.sizeof(ReplaySpec), steps, lane_mode, check_mode, len(inputs), isig.ctypes.data,
+                        iv.ctypes.data, len(checks), csig.ctypes.data, None if ci is None else ci.ctypes.data,
+                        ev.ctypes.data, None if ca is None else ca.ctypes.data, max_mismatches)
+        out = (ReplayMismatch * max(1, max_mismatches))()
+        count, n = ctypes.c_uint64(), ctypes.c_uint32()
+        self._check(self.L.rrtl_hip_replay(self.h, ctypes.byref(sp), ctypes.byref(count), out, ctypes.byref(n)))
+
+        def u128(v):
+            return sum(int(v[k]) << (32 * k) for k in range(4))
+        mism = [(int(m.step), int(m.check_index), int(m.lane), u128(m.expected), u128(m.actual))
+                for m in out[: n.value]]
+        return {"mismatch_count": int(count.value), "mismatches": mism}
 
     def detect_flags(self) -> np.ndarray:
         """Per-lane detection flags of the last detect run (bool array)."""

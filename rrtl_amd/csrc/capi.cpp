@@ -246,6 +246,62 @@ rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out) {
 rrtl_status rrtl_hip_get_detect(rrtl_hip_engine* e, uint32_t* out) {
   return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_detect(out); });
 }
+rrtl_status rrtl_hip_replay(rrtl_hip_engine* e, const rrtl_hip_replay_spec* sp, uint64_t* mismatch_count,
+                            rrtl_hip_replay_mismatch* mismatches, uint32_t* n_mismatches) {
+  return guard(e, [&](rrtl::Engine& x) {
+    if (!sp || sp->struct_size < sizeof(rrtl_hip_replay_spec)) throw rrtl::Error("E_HIP_RUN_SPEC: replay spec is null or too small");
+    if (!mismatch_count || !n_mismatches || (sp->max_mismatches && !mismatches))
+      throw rrtl::Error("E_HIP_NULL: null report buffer");
+    if (sp->lane_mode > RRTL_REPLAY_INDEPENDENT || sp->check_mode > RRTL_REPLAY_ALL_LANES)
+      throw rrtl::Error("E_HIP_RUN_SPEC: unknown replay lane or check mode");
+    if ((sp->n_inputs && (!sp->input_signal || !sp->input_values)) || (sp->n_checks && (!sp->check_signal || !sp->expected)))
+      throw rrtl::Error("E_HIP_NULL: null input or check array");
+    const rrtl::Program& P = x.program();
+    const bool indep = sp->lane_mode == RRTL_REPLAY_INDEPENDENT;
+    rrtl::RunSpec r;
+    r.cycles = sp->steps;
+    for (uint32_t i = 0; i < sp->n_inputs; i++) {
+      if (sp->input_signal[i] >= P.signals.size()) throw rrtl::Error("E_SIM_IR_REPLAY_SIGNAL: input signal out of range");
+      r.in_sig.push_back(sp->input_signal[i]);
+      r.in_mode.push_back(indep ? rrtl::IN_TRACE : rrtl::IN_TRACE_UNIFORM);
+      r.in_arg.push_back(0);
+    }
+    if (indep) r.trace_inputs = sp->input_values; else r.trace_uniform = sp->input_values;
+    std::vector<uint32_t> limb_at;  // first limb of each check in a step's expected row
+    uint32_t check_limbs = 0;
+    for (uint32_t j = 0; j < sp->n_checks; j++) {
+      if (sp->check_signal[j] >= P.signals.size()) throw rrtl::Error("E_SIM_IR_REPLAY_SIGNAL: check signal out of range");
+      r.obs_sig.push_back(sp->check_signal[j]);
+      limb_at.push_back(check_limbs);
+      check_limbs += P.signals[sp->check_signal[j]].limbs;
+    }
+    static const uint32_t kNoChecks = 0;
+    r.expect = sp->n_checks ? sp->expected : &kNoChecks;
+    r.expect_per_lane = indep;
+    r.check_active = sp->check_active;
+    r.check_lane0 = !indep && sp->check_mode == RRTL_REPLAY_LANE0;
+    r.max_mismatches = sp->max_mismatches;
+    r.final_eval = false;
+    x.run(r);
+    std::vector<rrtl::Mismatch> mm;
+    *mismatch_count = x.get_mismatches(sp->max_mismatches, mm);
+    const uint64_t lanes = x.lanes();
+    for (size_t k = 0; k < mm.size(); k++) {
+      rrtl_hip_replay_mismatch& o = mismatches[k];
+      const rrtl::Mismatch& m = mm[k];
+      o.step = m.step;
+      o.check_index = sp->check_index ? sp->check_index[m.check] : m.check;
+      o.lane = m.lane;
+      const uint32_t limbs = P.signals[sp->check_signal[m.check]].limbs;
+      for (uint32_t l = 0; l < 4; l++) {
+        const uint64_t at = m.step * check_limbs + limb_at[m.check] + l;
+        o.expected[l] = l >= limbs ? 0u : indep ? sp->expected[at * lanes + m.lane] : sp->expected[at];
+        o.actual[l] = m.actual[l];
+      }
+    }
+    *n_mismatches = uint32_t(mm.size());
+  });
+}
 rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4) {
   return guard(e, [&](rrtl::Engine& x) {
     if (!out4) throw rrtl::Error("E_HIP_NULL: null buffer");

@@ -28,6 +28,23 @@ struct RunSpec {
   const uint32_t* trace_inputs = nullptr;  // IN_TRACE values, host [cycles][n_trace_limbs][lanes]
   bool detect = false;                     // flag lanes whose observed values leave lane 0 of their 64-lane group
   std::vector<uint64_t> group_key;         // IN_GROUP_RANDOM key per 64-lane group (empty: the group index)
+  // Trace replay (word engine): values shared by all lanes, checks of the
+  // observed signals, and the mismatch records kept per lane.
+  const uint32_t* trace_uniform = nullptr; // IN_TRACE_UNIFORM values, host [cycles][n_uniform_limbs]
+  const uint32_t* expect = nullptr;        // host [cycles][observed limbs] (or [...][lanes] when expect_per_lane)
+  bool expect_per_lane = false;
+  const uint32_t* check_active = nullptr;  // host [cycles][ceil(observed / 32)] bits, or null: every check
+  bool check_lane0 = false;                // only lane 0 compares
+  uint32_t max_mismatches = 0;             // records kept per lane
+  bool final_eval = true;                  // false: no settle after the last cycle (replay ends on the edge)
+};
+
+// One mismatch of a replay: the cycle (step), the check's position among the
+// observed signals, the lane and the actual value's first four limbs.
+struct Mismatch {
+  uint64_t step = 0;
+  uint32_t check = 0, lane = 0;
+  uint32_t actual[4] = {0, 0, 0, 0};
 };
 
 struct RunStats {

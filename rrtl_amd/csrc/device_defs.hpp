@@ -44,6 +44,7 @@ __host__ __device__ inline uint64_t lane_digest(uint64_t d, uint32_t limb, uint3
 }
 
 // Stimulus modes (rrtl_hip_input_mode in the C header).
-enum : uint32_t { IN_HOLD = 0, IN_RANDOM = 1, IN_CONST = 2, IN_PULSE = 3, IN_TRACE = 4, IN_GROUP_RANDOM = 5 };
+enum : uint32_t { IN_HOLD = 0, IN_RANDOM = 1, IN_CONST = 2, IN_PULSE = 3, IN_TRACE = 4, IN_GROUP_RANDOM = 5,
+                  IN_TRACE_UNIFORM = 6 };
 
 }  // namespace rrtl

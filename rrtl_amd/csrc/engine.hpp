@@ -74,6 +74,12 @@ class Engine {
   virtual void get_halt_cycles(uint64_t* out) { (void)out; throw Error("E_HIP_UNSUPPORTED: engine has no halting"); }
   // Runs with detect set: lane flags packed 32 per u32 word (ceil(lanes/32) words).
   virtual void get_detect(uint32_t* out) { (void)out; throw Error("E_HIP_UNSUPPORTED: engine has no fault detection"); }
+  // Runs with expected values: the total mismatch count and the first `max`
+  // mismatches in (step, check, lane) order.
+  virtual uint64_t get_mismatches(uint32_t max, std::vector<Mismatch>& out) {
+    (void)max; (void)out;
+    throw Error("E_HIP_UNSUPPORTED: trace replay with checks runs on the word engine");
+  }
   virtual void synchronize() { RRTL_HIP_CHECK(hipStreamSynchronize(stream_)); }
   // Multi-design batches: compile and upload a run, return its kernel
   // arguments; reset its outputs before every launch; book-keep after it.

@@ -661,7 +661,8 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
   // stimulus
   for (size_t i = 0; i < spec.in_sig.size(); i++) {
     if (spec.in_mode[i] == IN_HOLD) continue;
-    if (spec.in_mode[i] == IN_TRACE) throw Error("E_HIP_UNSUPPORTED: trace-driven inputs are not supported by the gate engine run");
+    if (spec.in_mode[i] == IN_TRACE || spec.in_mode[i] == IN_TRACE_UNIFORM)
+      throw Error("E_HIP_UNSUPPORTED: trace-driven inputs are not supported by the gate engine run");
     if (spec.in_mode[i] > IN_GROUP_RANDOM) throw Error("E_HIP_RUN_SPEC: unknown input mode");
     T.stim_slot.push_back(T.persist.at(spec.in_sig[i]));
     T.stim_mode.push_back(spec.in_mode[i]);
@@ -999,6 +1000,7 @@ class GateEngine final : public Engine {
   }
 
   Prepared prepare(const RunSpec& spec, bool with_outputs) {
+    if (spec.expect) throw Error("E_HIP_UNSUPPORTED: trace replay with checks runs on the word engine");
     for (uint32_t s : spec.obs_sig) check_sig(s);
     for (uint32_t s : spec.in_sig) check_sig(s);
     if (spec.halt_sig != NONE) throw Error("E_HIP_UNSUPPORTED: the gate engine does not halt lanes");

@@ -34,6 +34,14 @@ struct WordArgs {
   uint32_t n_scratch, n_sig_words;                        // scratch slots; signal words (LDS signal mode)
   uint32_t block_lanes;                                   // lanes per workgroup (0: one per thread); a
                                                           // smaller count leaves the higher threads idle
+  // Trace replay (SimBackend::replay_trace moved on chip): inputs shared by all
+  // lanes, and checks of the observed signals against expected values.
+  const uint32_t* trace_uni; uint32_t trace_uni_limbs;    // IN_TRACE_UNIFORM values [cycles][trace_uni_limbs]
+  const uint32_t* expect; uint32_t expect_per_lane;       // [cycles][n_obs_limbs] or [cycles][n_obs_limbs][lanes]
+  const uint32_t* check_active; uint32_t check_words;     // [cycles][check_words]: bit j checks observed j; null: all
+  uint32_t check_lanes;                                   // lanes below this compare (UINT32_MAX: all)
+  uint32_t* mm_count; uint32_t* mm_rec; uint32_t mm_max;  // per lane: mismatch count; first mm_max records
+                                                          // [lanes][mm_max][6] = (cycle, check, actual limbs 0..3)
 };
 
 // Where a block keeps its per-lane scratch and signal values during a launch:
@@ -424,7 +432,7 @@ struct InterpExec {
 
 template <class Cx>
 __device__ __forceinline__ void apply_stimulus(const WordArgs& A, const Cx& C, uint64_t c, uint64_t cyc) {
-  uint32_t trace_limb = 0;
+  uint32_t trace_limb = 0, uni_limb = 0;
   for (uint32_t i = 0; i < A.n_in; i++) {
     const uint32_t off = A.in_desc[4 * i], limbs = A.in_desc[4 * i + 1], width = A.in_desc[4 * i + 2], mode = A.in_desc[4 * i + 3];
     const uint64_t arg = A.in_arg[i];
@@ -444,6 +452,8 @@ __device__ __forceinline__ void apply_stimulus(const WordArgs& A, const Cx& C, u
         v = l == 0 ? uint32_t(cyc < (arg & 0xFFFFFFFFull) ? (arg >> 32) & 1 : (arg >> 33) & 1) : 0u;
       } else if (mode == IN_TRACE) {
         v = A.trace_in[(size_t(c) * A.trace_in_limbs + trace_limb + l) * A.lanes + C.lane];
+      } else if (mode == IN_TRACE_UNIFORM) {
+        v = A.trace_uni[size_t(c) * A.trace_uni_limbs + uni_limb + l];
       } else {
         continue;
       }
@@ -451,18 +461,42 @@ __device__ __forceinline__ void apply_stimulus(const WordArgs& A, const Cx& C, u
       C.stsig(off, l, v);
     }
     if (mode == IN_TRACE) trace_limb += limbs;
+    if (mode == IN_TRACE_UNIFORM) uni_limb += limbs;
   }
 }
 
+// Observation point: trace, digest and, in a replay, the checks: an active
+// check whose value differs from its expected value in any limb counts one
+// mismatch for the lane and, among the lane's first mm_max, is recorded with
+// its cycle, check number and actual value.
 template <class Cx>
-__device__ __forceinline__ void observe(const WordArgs& A, const Cx& C, uint64_t c, uint64_t& d) {
+__device__ __forceinline__ void observe(const WordArgs& A, const Cx& C, uint64_t c, uint64_t& d, uint32_t& mm) {
+  const bool check = A.expect && C.lane < A.check_lanes;
   uint32_t k = 0;
   for (uint32_t j = 0; j < A.n_obs; j++) {
     const uint32_t off = A.obs_desc[2 * j], limbs = A.obs_desc[2 * j + 1];
+    uint32_t diff = 0, act[4] = {0, 0, 0, 0};
     for (uint32_t l = 0; l < limbs; l++, k++) {
       const uint32_t v = C.ld(SP_SIGNAL | off, l);
       if (A.trace) A.trace[(size_t(c) * A.n_obs_limbs + k) * A.lanes + C.lane] = v;
       if (A.digest) d = lane_digest(d, v, j, l);
+      if (check) {
+        const uint32_t e = A.expect_per_lane ? A.expect[(size_t(c) * A.n_obs_limbs + k) * A.lanes + C.lane]
+                                             : A.expect[size_t(c) * A.n_obs_limbs + k];
+        diff |= v ^ e;
+        if (l == 0) act[0] = v;
+        else if (l == 1) act[1] = v;
+        else if (l == 2) act[2] = v;
+        else if (l == 3) act[3] = v;
+      }
+    }
+    if (check && diff && (!A.check_active || ((A.check_active[size_t(c) * A.check_words + j / 32] >> (j % 32)) & 1u))) {
+      if (mm < A.mm_max) {
+        uint32_t* r = A.mm_rec + (size_t(C.lane) * A.mm_max + mm) * 6;
+        r[0] = uint32_t(A.cycle_base + c); r[1] = j;
+        r[2] = act[0]; r[3] = act[1]; r[4] = act[2]; r[5] = act[3];
+      }
+      mm++;
     }
   }
 }
@@ -488,6 +522,7 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
     for (uint32_t w = 0; w < A.n_sig_words; w++) lds_words()[C.sig0 + w * C.stride + C.col] = A.values[size_t(w) * A.lanes + lane];
   Exec::enter(C);
   uint64_t d = A.digest ? A.digest[lane] : 0;
+  uint32_t mm = A.mm_count ? A.mm_count[lane] : 0;
   uint64_t done = 0;
   bool halted = false;
   if (A.halt_cycles) { done = A.halt_cycles[lane] >> 1; halted = A.halt_cycles[lane] & 1; }
@@ -517,7 +552,7 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
     const uint32_t seq_u = __builtin_amdgcn_readfirstlane(seq), n_u = __builtin_amdgcn_readfirstlane(n);
     for (uint32_t k = 0; k < n_u; k++) {
       const uint32_t id = (seq_u >> (4 * k)) & 15u;
-      if (id == STEP_OBS) observe(A, C, c, d);
+      if (id == STEP_OBS) observe(A, C, c, d, mm);
       else Exec::run(A, id, C);
     }
     if (!first && !last) {
@@ -529,6 +564,7 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
   if constexpr (kMode == WM_ALL_LDS)
     for (uint32_t w = 0; w < A.n_sig_words; w++) A.values[size_t(w) * A.lanes + lane] = lds_words()[C.sig0 + w * C.stride + C.col];
   if (A.digest) A.digest[lane] = d;
+  if (A.mm_count) A.mm_count[lane] = mm;
   if (A.halt_cycles) A.halt_cycles[lane] = (done << 1) | (halted ? 1u : 0u);
 }
 

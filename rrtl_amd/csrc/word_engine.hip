@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <memory>
 #include <numeric>
+#include <tuple>
 #include <cstring>
 #include <map>
 #include <unordered_map>
@@ -658,7 +659,7 @@ class WordEngine final : public Engine {
     values_.release(); mems_.release(); scratch_.release(); cpool_.release(); mtab_.release();
     for (auto& b : blk_) b.release();
     for (auto& b : run_bufs_) b.release();
-    digest_.release(); trace_.release(); halt_.release();
+    digest_.release(); trace_.release(); halt_.release(); mm_count_.release(); mm_rec_.release();
     if (stream_) (void)hipStreamDestroy(stream_);
   }
   uint32_t kind() const override { return ENGINE_WORD; }
@@ -787,6 +788,32 @@ class WordEngine final : public Engine {
     synchronize();
     for (uint32_t l = 0; l < lanes_; l++) out[l] >>= 1;
   }
+  uint64_t get_mismatches(uint32_t max, std::vector<Mismatch>& out) override {
+    if (!mm_count_.p || !mm_lanes_) throw Error("E_HIP_NO_RUN: no run has checked expected values");
+    std::vector<uint32_t> cnt(lanes_);
+    RRTL_HIP_CHECK(hipMemcpy(cnt.data(), mm_count_.p, size_t(lanes_) * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> rec(size_t(lanes_) * mm_max_ * 6);
+    if (!rec.empty()) RRTL_HIP_CHECK(hipMemcpy(rec.data(), mm_rec_.p, rec.size() * 4, hipMemcpyDeviceToHost));
+    uint64_t total = 0;
+    out.clear();
+    for (uint32_t lane = 0; lane < mm_lanes_; lane++) {
+      total += cnt[lane];
+      for (uint32_t r = 0; r < std::min(cnt[lane], mm_max_); r++) {
+        const uint32_t* x = &rec[(size_t(lane) * mm_max_ + r) * 6];
+        Mismatch m;
+        m.step = x[0]; m.check = x[1]; m.lane = lane;
+        std::copy(x + 2, x + 6, m.actual);
+        out.push_back(m);
+      }
+    }
+    // the reference's report order: step, then check, then lane; each lane's
+    // first records contain the overall first ones
+    std::sort(out.begin(), out.end(), [](const Mismatch& a, const Mismatch& b) {
+      return std::make_tuple(a.step, a.check, a.lane) < std::make_tuple(b.step, b.check, b.lane);
+    });
+    if (out.size() > max) out.resize(max);
+    return total;
+  }
   uint32_t scratch_slots() const { return run_slots_; }
   bool compiled() const { return jit_fn_ != nullptr; }
   bool scratch_in_lds() const { return use_lds_; }
@@ -832,7 +859,7 @@ class WordEngine final : public Engine {
         if (m == IN_GROUP_RANDOM) throw Error("E_HIP_UNSUPPORTED: group-shared random inputs run on the bit-sliced gate engine");
       A.seed = spec->seed;
       std::vector<uint32_t> in_desc; std::vector<uint64_t> in_arg;
-      uint32_t trace_limbs = 0;
+      uint32_t trace_limbs = 0, uni_limbs = 0;
       for (size_t i = 0; i < spec->in_sig.size(); i++) {
         const Signal& s = sig_at(spec->in_sig[i]);
         uint32_t mode = spec->in_mode[i];
@@ -842,6 +869,8 @@ class WordEngine final : public Engine {
           arg = uint64_t(spec->in_sig[i]) << 40;
         }
         if (mode == IN_TRACE) trace_limbs += s.limbs;
+        if (mode == IN_TRACE_UNIFORM) uni_limbs += s.limbs;
+        if (mode > IN_TRACE_UNIFORM) throw Error("E_HIP_RUN_SPEC: unknown input mode");
         in_desc.insert(in_desc.end(), {s.offset, s.limbs, s.width, mode});
         in_arg.push_back(arg);
       }
@@ -851,6 +880,11 @@ class WordEngine final : public Engine {
         std::vector<uint32_t> tin(spec->trace_inputs, spec->trace_inputs + size_t(cycles) * trace_limbs * lanes_);
         A.trace_in = upload_run(2, tin); A.trace_in_limbs = trace_limbs;
       }
+      if (uni_limbs) {
+        if (!spec->trace_uniform) throw Error("E_HIP_RUN_SPEC: uniform trace inputs requested without their values");
+        std::vector<uint32_t> tu(spec->trace_uniform, spec->trace_uniform + size_t(cycles) * uni_limbs);
+        A.trace_uni = upload_run(4, tu); A.trace_uni_limbs = uni_limbs;
+      }
       std::vector<uint32_t> obs;
       uint32_t obs_limbs = 0;
       for (uint32_t s : spec->obs_sig) { const Signal& g = sig_at(s); obs.insert(obs.end(), {g.offset, g.limbs}); obs_limbs += g.limbs; }
@@ -859,6 +893,25 @@ class WordEngine final : public Engine {
       if (has_digest_) { digest_.alloc(lanes_); digest_.zero(stream_); A.digest = digest_.p; }
       trace_bytes_ = 0;
       if (spec->trace && A.n_obs) { trace_bytes_ = size_t(cycles) * obs_limbs * lanes_ * 4; trace_.alloc(trace_bytes_ / 4); A.trace = trace_.p; }
+      mm_lanes_ = 0;
+      if (spec->expect) {
+        for (uint32_t s : spec->obs_sig)
+          if (sig_at(s).limbs > 4) throw Error("E_HIP_WIDE: checked signals are limited to 128 bits");
+        const size_t per = size_t(obs_limbs) * (spec->expect_per_lane ? lanes_ : 1u);
+        std::vector<uint32_t> ex(spec->expect, spec->expect + size_t(cycles) * per);
+        A.expect = upload_run(5, ex); A.expect_per_lane = spec->expect_per_lane ? 1u : 0u;
+        if (spec->check_active) {
+          A.check_words = (A.n_obs + 31) / 32;
+          std::vector<uint32_t> ca(spec->check_active, spec->check_active + size_t(cycles) * A.check_words);
+          A.check_active = upload_run(6, ca);
+        }
+        A.check_lanes = spec->check_lane0 ? 1u : 0xFFFFFFFFu;
+        mm_lanes_ = spec->check_lane0 ? 1u : lanes_;
+        mm_max_ = spec->max_mismatches;
+        mm_count_.alloc(lanes_); mm_count_.zero(stream_);
+        mm_rec_.alloc(std::max<size_t>(1, size_t(lanes_) * mm_max_ * 6));
+        A.mm_count = mm_count_.p; A.mm_rec = mm_rec_.p; A.mm_max = mm_max_;
+      }
       has_halt_ = spec->halt_sig != NONE;
       if (has_halt_) {
         const Signal& h = sig_at(spec->halt_sig);
@@ -873,7 +926,7 @@ class WordEngine final : public Engine {
   uint32_t run_flags(const RunSpec& spec) const {
     uint32_t flags = F_EVAL_PRE | F_EDGE;
     bool need_post = spec.observe_post || (spec.halt_sig != NONE && P.signals.at(spec.halt_sig).kind != K_REG);
-    if (need_post) flags |= F_EVAL_POST; else flags |= F_FINAL_EVAL;
+    if (need_post) flags |= F_EVAL_POST; else if (spec.final_eval) flags |= F_FINAL_EVAL;
     if (spec.observe_post) flags |= F_OBS_POST;
     return flags;
   }
@@ -894,6 +947,9 @@ class WordEngine final : public Engine {
       if (done + B.cycles < cycles) B.flags &= ~uint32_t(F_FINAL_EVAL);
       if (A.trace) B.trace = A.trace + done * A.n_obs_limbs * lanes_;
       if (A.trace_in) B.trace_in = A.trace_in + done * A.trace_in_limbs * lanes_;
+      if (A.trace_uni) B.trace_uni = A.trace_uni + done * A.trace_uni_limbs;
+      if (A.expect) B.expect = A.expect + done * A.n_obs_limbs * (A.expect_per_lane ? lanes_ : 1u);
+      if (A.check_active) B.check_active = A.check_active + done * A.check_words;
       if (jit_fn_) {
         void* params[] = {&B};
         RRTL_HIP_CHECK(hipModuleLaunchKernel(jit_fn_, grid, 1, 1, block_, 1, 1, uint32_t(lds), s, params, nullptr));
@@ -954,7 +1010,8 @@ class WordEngine final : public Engine {
   int mode_ = WM_SCRATCH_LDS;
   bool use_lds_ = true, has_digest_ = false, has_halt_ = false, batch_spec_halt_ = false, batch_digest_ = false;
   size_t trace_bytes_ = 0;
-  DevBuf<uint32_t> values_, mems_, scratch_, cpool_, mtab_, blk_[4], run_bufs_[4], trace_;
+  DevBuf<uint32_t> values_, mems_, scratch_, cpool_, mtab_, blk_[4], run_bufs_[8], trace_, mm_count_, mm_rec_;
+  uint32_t mm_lanes_ = 0, mm_max_ = 0;  // lanes that checked in the last replay, records kept per lane
   DevBuf<uint64_t> digest_, halt_;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };
