@@ -54,9 +54,14 @@ hipFunction_t rtc_kernel(const std::string& source, const char* name, int device
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<char> code = rtc_compile(source);
     g_compile_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // load into `device`'s context, leaving the caller's current device as it was
+    int prev = 0;
+    RRTL_HIP_CHECK(hipGetDevice(&prev));
     RRTL_HIP_CHECK(hipSetDevice(device));
     hipModule_t mod;
-    RRTL_HIP_CHECK(hipModuleLoadData(&mod, code.data()));
+    const hipError_t st = hipModuleLoadData(&mod, code.data());
+    RRTL_HIP_CHECK(hipSetDevice(prev));
+    RRTL_HIP_CHECK(st);
     it = g_modules.emplace(std::move(key), mod).first;
   }
   hipFunction_t fn;

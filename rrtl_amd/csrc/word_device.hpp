@@ -25,8 +25,10 @@ struct WordArgs {
   const uint32_t* cpool; const uint32_t* mem_table;
   uint32_t lanes, flags;
   uint64_t cycles, cycle_base, seed;
-  // stimulus: per input (offset, limbs, width, mode), arg, stream base
-  const uint32_t* in_desc; const uint64_t* in_arg; uint32_t n_in;
+  // stimulus: per input (offset, limbs, width, mode | trace limb base << 8), arg;
+  // the first n_in_once inputs hold one value for the whole run (IN_CONST) and
+  // are written once per launch, the rest every cycle
+  const uint32_t* in_desc; const uint64_t* in_arg; uint32_t n_in, n_in_once;
   const uint32_t* trace_in; uint32_t trace_in_limbs;      // [cycles][trace_in_limbs][lanes]
   const uint32_t* obs_desc; uint32_t n_obs, n_obs_limbs;  // per observed: (offset, limbs)
   uint64_t* digest; uint32_t* trace;                      // [lanes] / [cycles][n_obs_limbs][lanes]
@@ -431,10 +433,11 @@ struct InterpExec {
 };
 
 template <class Cx>
-__device__ __forceinline__ void apply_stimulus(const WordArgs& A, const Cx& C, uint64_t c, uint64_t cyc) {
-  uint32_t trace_limb = 0, uni_limb = 0;
-  for (uint32_t i = 0; i < A.n_in; i++) {
-    const uint32_t off = A.in_desc[4 * i], limbs = A.in_desc[4 * i + 1], width = A.in_desc[4 * i + 2], mode = A.in_desc[4 * i + 3];
+__device__ __forceinline__ void apply_stimulus(const WordArgs& A, const Cx& C, uint64_t c, uint64_t cyc, uint32_t i0,
+                                               uint32_t i1) {
+  for (uint32_t i = i0; i < i1; i++) {
+    const uint32_t off = A.in_desc[4 * i], limbs = A.in_desc[4 * i + 1], width = A.in_desc[4 * i + 2];
+    const uint32_t mode = A.in_desc[4 * i + 3] & 0xFFu, tl = A.in_desc[4 * i + 3] >> 8;
     const uint64_t arg = A.in_arg[i];
     const uint32_t sig_index = uint32_t(arg >> 40);  // stream id for IN_RANDOM
     for (uint32_t l = 0; l < limbs; l++) {
@@ -451,17 +454,15 @@ __device__ __forceinline__ void apply_stimulus(const WordArgs& A, const Cx& C, u
       } else if (mode == IN_PULSE) {
         v = l == 0 ? uint32_t(cyc < (arg & 0xFFFFFFFFull) ? (arg >> 32) & 1 : (arg >> 33) & 1) : 0u;
       } else if (mode == IN_TRACE) {
-        v = A.trace_in[(size_t(c) * A.trace_in_limbs + trace_limb + l) * A.lanes + C.lane];
+        v = A.trace_in[(size_t(c) * A.trace_in_limbs + tl + l) * A.lanes + C.lane];
       } else if (mode == IN_TRACE_UNIFORM) {
-        v = A.trace_uni[size_t(c) * A.trace_uni_limbs + uni_limb + l];
+        v = A.trace_uni[size_t(c) * A.trace_uni_limbs + tl + l];
       } else {
         continue;
       }
       if (l == limbs - 1) v &= fmask(width);
       C.stsig(off, l, v);
     }
-    if (mode == IN_TRACE) trace_limb += limbs;
-    if (mode == IN_TRACE_UNIFORM) uni_limb += limbs;
   }
 }
 
@@ -521,6 +522,9 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
   if constexpr (kMode == WM_ALL_LDS)
     for (uint32_t w = 0; w < A.n_sig_words; w++) lds_words()[C.sig0 + w * C.stride + C.col] = A.values[size_t(w) * A.lanes + lane];
   Exec::enter(C);
+  // inputs that hold one value all run: the design never writes an input, so
+  // once per launch is the same as every cycle
+  if (A.n_in_once) apply_stimulus(A, C, 0, A.cycle_base, 0, A.n_in_once);
   uint64_t d = A.digest ? A.digest[lane] : 0;
   uint32_t mm = A.mm_count ? A.mm_count[lane] : 0;
   uint64_t done = 0;
@@ -545,7 +549,7 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
     else {
       if (halted) continue;
       c = s - 1;
-      if (A.n_in) apply_stimulus(A, C, c, A.cycle_base + c);
+      if (A.n_in > A.n_in_once) apply_stimulus(A, C, c, A.cycle_base + c, A.n_in_once, A.n_in);
       seq = cyc_seq; n = cyc_n;
     }
     // the sequence is the same on every active lane; make that visible

@@ -792,7 +792,7 @@ class WordEngine final : public Engine {
     if (!mm_count_.p || !mm_lanes_) throw Error("E_HIP_NO_RUN: no run has checked expected values");
     std::vector<uint32_t> cnt(lanes_);
     RRTL_HIP_CHECK(hipMemcpy(cnt.data(), mm_count_.p, size_t(lanes_) * 4, hipMemcpyDeviceToHost));
-    std::vector<uint32_t> rec(size_t(lanes_) * mm_max_ * 6);
+    std::vector<uint32_t> rec(size_t(mm_lanes_) * mm_max_ * 6);
     if (!rec.empty()) RRTL_HIP_CHECK(hipMemcpy(rec.data(), mm_rec_.p, rec.size() * 4, hipMemcpyDeviceToHost));
     uint64_t total = 0;
     out.clear();
@@ -858,22 +858,35 @@ class WordEngine final : public Engine {
       for (uint32_t m : spec->in_mode)
         if (m == IN_GROUP_RANDOM) throw Error("E_HIP_UNSUPPORTED: group-shared random inputs run on the bit-sliced gate engine");
       A.seed = spec->seed;
-      std::vector<uint32_t> in_desc; std::vector<uint64_t> in_arg;
+      // Input descriptors: (offset, limbs, width, mode | trace limb base << 8).
+      // Constant inputs driven by one entry go first (written once per launch),
+      // the others keep their order; held inputs write nothing.
+      std::vector<uint32_t> in_desc, var_desc; std::vector<uint64_t> in_arg, var_arg;
       uint32_t trace_limbs = 0, uni_limbs = 0;
+      std::unordered_map<uint32_t, uint32_t> entries;
+      for (uint32_t sg : spec->in_sig) entries[sg]++;
       for (size_t i = 0; i < spec->in_sig.size(); i++) {
         const Signal& s = sig_at(spec->in_sig[i]);
         uint32_t mode = spec->in_mode[i];
         uint64_t arg = spec->in_arg.empty() ? 0 : spec->in_arg[i];
+        if (mode > IN_TRACE_UNIFORM) throw Error("E_HIP_RUN_SPEC: unknown input mode");
+        uint32_t base = 0;
         if (mode == IN_RANDOM) {
           if (s.width > 256) throw Error("E_HIP_WIDE: random stimulus is limited to 256-bit inputs");
           arg = uint64_t(spec->in_sig[i]) << 40;
         }
-        if (mode == IN_TRACE) trace_limbs += s.limbs;
-        if (mode == IN_TRACE_UNIFORM) uni_limbs += s.limbs;
-        if (mode > IN_TRACE_UNIFORM) throw Error("E_HIP_RUN_SPEC: unknown input mode");
-        in_desc.insert(in_desc.end(), {s.offset, s.limbs, s.width, mode});
-        in_arg.push_back(arg);
+        if (mode == IN_TRACE) { base = trace_limbs; trace_limbs += s.limbs; }
+        if (mode == IN_TRACE_UNIFORM) { base = uni_limbs; uni_limbs += s.limbs; }
+        if (mode == IN_HOLD) continue;
+        if (base >= (1u << 24)) throw Error("E_HIP_RUN_SPEC: too many trace input limbs");
+        const bool once = mode == IN_CONST && entries[spec->in_sig[i]] == 1;
+        auto& d = once ? in_desc : var_desc;
+        d.insert(d.end(), {s.offset, s.limbs, s.width, mode | (base << 8)});
+        (once ? in_arg : var_arg).push_back(arg);
       }
+      A.n_in_once = uint32_t(in_desc.size() / 4);
+      in_desc.insert(in_desc.end(), var_desc.begin(), var_desc.end());
+      in_arg.insert(in_arg.end(), var_arg.begin(), var_arg.end());
       A.in_desc = upload_run(0, in_desc); A.in_arg = upload_run(1, in_arg); A.n_in = uint32_t(in_desc.size() / 4);
       if (trace_limbs) {
         if (!spec->trace_inputs) throw Error("E_HIP_RUN_SPEC: trace inputs requested without a trace buffer");
@@ -907,9 +920,13 @@ class WordEngine final : public Engine {
         }
         A.check_lanes = spec->check_lane0 ? 1u : 0xFFFFFFFFu;
         mm_lanes_ = spec->check_lane0 ? 1u : lanes_;
-        mm_max_ = spec->max_mismatches;
+        if (cycles >= (uint64_t(1) << 32)) throw Error("E_HIP_RUN_SPEC: a checked run is limited to 2^32 - 1 steps");
+        // a lane cannot mismatch more often than steps x checks
+        mm_max_ = uint32_t(std::min<uint64_t>(spec->max_mismatches, cycles * A.n_obs));
+        if (uint64_t(mm_lanes_) * mm_max_ * 24 > (uint64_t(1) << 31))
+          throw Error("E_HIP_TOO_LARGE: mismatch records for every checked lane exceed 2 GiB; lower max_mismatches");
         mm_count_.alloc(lanes_); mm_count_.zero(stream_);
-        mm_rec_.alloc(std::max<size_t>(1, size_t(lanes_) * mm_max_ * 6));
+        mm_rec_.alloc(std::max<size_t>(1, size_t(mm_lanes_) * mm_max_ * 6));
         A.mm_count = mm_count_.p; A.mm_rec = mm_rec_.p; A.mm_max = mm_max_;
       }
       has_halt_ = spec->halt_sig != NONE;
