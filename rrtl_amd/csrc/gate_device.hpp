@@ -35,6 +35,12 @@ struct TileArgs {
   const uint32_t* load_slot; const uint32_t* load_node; uint32_t n_load;
   const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
   const uint4* commit; uint32_t n_commit;
+  // commits without a reset, at byte offsets: reg | (val | stored inverted) << 16;
+  // at most FAST_COMMITS_PER_THREAD x TILE_THREADS, held in registers for the run
+  const uint32_t* commit_fast; uint32_t n_commit_fast;
+  // 1: the PRE observation, the commit and the next cycle's inputs touch
+  // disjoint slots and share one barrier (no post program)
+  uint32_t merge_tail;
   const uint32_t* obs_slot; uint32_t n_obs;     // slot | stored value inverted << 31
   // per tile: 5-word entries (pos, slot, word, and, or); tile t owns
   // [force_start[t], force_start[t+1]), of which [force_mid[t], ...) clamp at the commit
@@ -184,8 +190,8 @@ __device__ __forceinline__ void apply_clamps(const TileArgs& A, uint32_t* lds, u
 // flags accumulate in the thread's registers (thread j owns observed signals
 // j, j + 256, ...) and reach HBM once per tile run.
 template <int W>
-__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, Words<W>& sreg, uint64_t c, uint64_t cyc,
-                                        uint64_t w0, Words<W>& det) {
+__device__ __forceinline__ void observe_values(const TileArgs& A, uint32_t* lds, Words<W>& sreg, uint64_t c,
+                                               uint64_t cyc, uint64_t w0, Words<W>& det) {
   const uint32_t tid = threadIdx.x;
   if (A.detect)
     for (uint32_t j = tid; j < A.n_obs; j += TILE_THREADS) {
@@ -215,6 +221,12 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, Words<
       sreg.v[k] ^= a & ~b;
     }
   }
+}
+
+template <int W>
+__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, Words<W>& sreg, uint64_t c, uint64_t cyc,
+                                        uint64_t w0, Words<W>& det) {
+  observe_values<W>(A, lds, sreg, c, cyc, w0, det);
   // the commit (or the next cycle's inputs) may overwrite an observed slot
   __syncthreads();
 }
@@ -225,8 +237,56 @@ constexpr uint32_t SINK_SLOTS = 64;     // per-tile slots written by lanes with 
 constexpr uint32_t GATE_VARIANT_BYTE_ADDR = 0x100;
 constexpr uint32_t GATE_VARIANT_M_SHIFT = 12;
 
+constexpr int FAST_COMMITS_PER_THREAD = 16;
+constexpr int FAST_COMMIT_BATCH = 8;
+
+// This thread's reset-free commits (entries tid, tid + 256, ...), loaded once per run.
+struct FastCommits { uint32_t e[FAST_COMMITS_PER_THREAD]; uint32_t n; };
+
+__device__ __forceinline__ FastCommits load_fast_commits(const TileArgs& A) {
+  FastCommits F;
+  const uint32_t tid = threadIdx.x;
+  F.n = A.n_commit_fast > tid ? (A.n_commit_fast - tid + TILE_THREADS - 1) / TILE_THREADS : 0u;
+#pragma unroll
+  for (int j = 0; j < FAST_COMMITS_PER_THREAD; j++) F.e[j] = uint32_t(j) < F.n ? A.commit_fast[tid + j * TILE_THREADS] : 0u;
+  return F;
+}
+
+// Reset-free commits: reg = val (^ ~0 when the value's slot holds its
+// complement).  No value slot is a committed register's slot (such values are
+// staged through copies before the commit), so every read may go ahead of
+// every write: the reads of a batch are issued together and the thread waits
+// for LDS once per batch instead of once per register.
 template <int W>
-__device__ __forceinline__ void commit_regs(const TileArgs& A, uint32_t* lds, uint32_t fb, uint32_t fe) {
+__device__ __forceinline__ void commit_fast(uint32_t* lds, const FastCommits& F) {
+#pragma unroll
+  for (int j0 = 0; j0 < FAST_COMMITS_PER_THREAD; j0 += FAST_COMMIT_BATCH) {
+    if (uint32_t(j0) >= F.n) break;
+    Words<W> v[FAST_COMMIT_BATCH];
+#pragma unroll
+    for (int j = 0; j < FAST_COMMIT_BATCH; j++)
+      if (uint32_t(j0 + j) < F.n) v[j] = ld_words_p<W>(lds_at<W, true>(lds, (F.e[j0 + j] >> 16) & ~1u));
+#pragma unroll
+    for (int j = 0; j < FAST_COMMIT_BATCH; j++)
+      if (uint32_t(j0 + j) < F.n) {
+        const uint32_t e = F.e[j0 + j], inv = 0u - ((e >> 16) & 1u);
+#pragma unroll
+        for (int q = 0; q < W; q++) v[j].v[q] ^= inv;
+        uint32_t* o = lds_at<W, true>(lds, e & 0xFFFFu);
+        if constexpr (W == 1) { o[0] = v[j].v[0]; }
+        else if constexpr (W == 2) { *reinterpret_cast<uint2*>(o) = make_uint2(v[j].v[0], v[j].v[1]); }
+        else {
+#pragma unroll
+          for (int q = 0; q < W; q += 4) *reinterpret_cast<uint4*>(o + q) = make_uint4(v[j].v[q], v[j].v[q + 1], v[j].v[q + 2], v[j].v[q + 3]);
+        }
+      }
+  }
+}
+
+// Commits with a reset (or any commit when the tile addresses slots by index):
+// the general form, descriptors read from the table.
+template <int W>
+__device__ __forceinline__ void commit_general(const TileArgs& A, uint32_t* lds) {
   for (uint32_t k = threadIdx.x; k < A.n_commit; k += TILE_THREADS) {
     const uint4 cm = A.commit[k];
     Words<W> v = ld_words<W>(lds, cm.y);
@@ -241,8 +301,6 @@ __device__ __forceinline__ void commit_regs(const TileArgs& A, uint32_t* lds, ui
     }
     st_words<W>(lds, cm.x, v);
   }
-  __syncthreads();
-  if (fe > fb) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fb, fe); __syncthreads(); }
 }
 
 // Input words of one tile for one cycle, for the thread that drives input s.
@@ -357,17 +415,39 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
 #pragma unroll
   for (int k = 0; k < W; k++) sreg.v[k] = 0u;
   if (A.digest && tid < 64) sreg = ld_words<W>(sig, tid);
+  FastCommits F;
+  if constexpr (kBA) F = load_fast_commits(A);
+  else F.n = 0;
   for (uint64_t c = 0; c < A.cycles; c++) {
     const uint64_t cyc = A.cycle_base + c;
-    // primary inputs for this cycle (thread per input, all words of the tile)
-    for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS) st_words<W>(lds, A.stim_slot[s], stimulus<W>(A, s, cyc, w0, gkey));
-    __syncthreads();
+    // primary inputs for this cycle (thread per input, all words of the tile);
+    // with a merged tail they were written by the previous cycle's tail
+    if (c == 0 || !A.merge_tail) {
+      for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS) st_words<W>(lds, A.stim_slot[s], stimulus<W>(A, s, cyc, w0, gkey));
+      __syncthreads();
+    }
     // cycle program, observe before the edge, commit, post program
     run_program<W, kBA, M>(A, lds, 0, A.n_cycle_stages, fb, fm);
-    if (!A.observe_post && A.n_obs) observe<W>(A, lds, sreg, c, cyc, w0, det);
-    commit_regs<W>(A, lds, fm, fe);
-    if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fm);
-    if (A.observe_post && A.n_obs) observe<W>(A, lds, sreg, c, cyc, w0, det);
+    if (A.merge_tail) {
+      // observed slots, committed registers and input slots are disjoint, and
+      // no committed value is read from an input slot: one barrier phase
+      if (A.n_obs) observe_values<W>(A, lds, sreg, c, cyc, w0, det);
+      if constexpr (kBA) commit_fast<W>(lds, F);
+      commit_general<W>(A, lds);
+      if (c + 1 < A.cycles)
+        for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS)
+          st_words<W>(lds, A.stim_slot[s], stimulus<W>(A, s, cyc + 1, w0, gkey));
+      __syncthreads();
+      if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
+    } else {
+      if (!A.observe_post && A.n_obs) observe<W>(A, lds, sreg, c, cyc, w0, det);
+      if constexpr (kBA) commit_fast<W>(lds, F);
+      commit_general<W>(A, lds);
+      __syncthreads();
+      if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
+      if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fm);
+      if (A.observe_post && A.n_obs) observe<W>(A, lds, sreg, c, cyc, w0, det);
+    }
   }
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
     uint32_t s = i / W, w = i % W;
