@@ -57,7 +57,7 @@ typedef struct rrtl_hip_options {
    * the last wave of tiles does not run alone.  0 = automatic, 1 = off.  Read
    * only when struct_size covers it. */
   uint32_t tile_chunks;
-  /* Gate engine: gate positions per thread between two barriers (2 or 3); a
+  /* Gate engine: gate positions per thread between two barriers (2, 3 or 4); a
    * position may take the previous position's result through registers.
    * 0 = automatic.  Read only when struct_size covers it. */
   uint32_t stage_gates;

@@ -114,8 +114,8 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
     if (opt.codegen > rrtl::CODEGEN_OFF) throw rrtl::Error("E_HIP_OPTIONS: codegen must be 0, 1 or 2");
     if (opt.wave_lanes && (opt.wave_lanes < 4 || opt.wave_lanes > 64 || (opt.wave_lanes & (opt.wave_lanes - 1))))
       throw rrtl::Error("E_HIP_OPTIONS: wave_lanes must be 4, 8, 16, 32 or 64");
-    if (opt.stage_gates && opt.stage_gates != 2 && opt.stage_gates != 3)
-      throw rrtl::Error("E_HIP_OPTIONS: stage_gates must be 2 or 3");
+    if (opt.stage_gates && (opt.stage_gates < 2 || opt.stage_gates > 4))
+      throw rrtl::Error("E_HIP_OPTIONS: stage_gates must be 2, 3 or 4");
     rrtl::Program prog = rrtl::Program::parse(blob, bytes);
     if (kind == RRTL_HIP_AUTO) kind = prog.is_gate_level() ? RRTL_HIP_GATE : RRTL_HIP_WORD;
     auto* h = new rrtl_hip_engine;

@@ -18,7 +18,7 @@ struct EngineOptions {
   uint32_t tile_words = 0;         // gate engine: u32 lane-words per workgroup tile (0: autotune)
   uint32_t ticks_per_launch = 0;   // 0: whole run in one launch
   uint32_t tile_chunks = 0;        // gate engine: work-queue chunks per tile (0: automatic, 1: off)
-  uint32_t stage_gates = 0;        // gate engine: gate positions per thread per stage (0: automatic, 2 or 3)
+  uint32_t stage_gates = 0;        // gate engine: gate positions per thread per stage (0: automatic, 2, 3 or 4)
   uint32_t codegen = 0;            // word engine: CODEGEN_AUTO / CODEGEN_ON (generated kernel) or CODEGEN_OFF (interpreter)
   uint32_t wave_lanes = 0;         // word engine: active lanes per 64-thread wave (16, 32, 64; 0: automatic)
 };

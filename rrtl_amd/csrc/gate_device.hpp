@@ -17,6 +17,10 @@ constexpr uint32_t G_COPY = 0;    // b = ones
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t CLAMP_COMMIT = 0xFFFFFFFEu;
 constexpr uint32_t TILE_THREADS = 256;
+// Waves per SIMD the tile kernels are register-budgeted for (at most 168
+// VGPRs each): a tile of the 20k-gate netlist takes 49 KB of LDS, so three
+// workgroups share a CU, one wave of each on every SIMD.
+constexpr int TILE_WAVES_PER_SIMD = 3;
 
 struct CGate { uint32_t op, a, b, dst; };
 struct CCommit { uint32_t reg, val, rsig, pol, rbit, has_reset; };
@@ -189,11 +193,43 @@ __device__ __forceinline__ void apply_clamps(const TileArgs& A, uint32_t* lds, u
 // flips both sides of the comparison, so the inversion flag drops out.  The
 // flags accumulate in the thread's registers (thread j owns observed signals
 // j, j + 256, ...) and reach HBM once per tile run.
-template <int W>
-__device__ __forceinline__ void observe_values(const TileArgs& A, uint32_t* lds, Words<W>& sreg, uint64_t c,
-                                               uint64_t cyc, uint64_t w0, Words<W>& det) {
+constexpr int OBS_REGS = 4;
+
+// Observed slots a thread reads every cycle, held in registers for the run:
+// for detection signals tid, tid + 256, ... when there are at most 1024 of
+// them; else for the signature (wave 0) signals tid, tid + 64, ... when there
+// are at most 256.  Otherwise the table is read every cycle.
+struct ObsRegs { uint32_t o[OBS_REGS]; bool det_on, dig_on; };
+
+__device__ __forceinline__ ObsRegs load_obs(const TileArgs& A) {
+  ObsRegs R;
   const uint32_t tid = threadIdx.x;
-  if (A.detect)
+  R.det_on = A.detect && A.n_obs <= TILE_THREADS * OBS_REGS;
+  R.dig_on = !R.det_on && A.digest && A.n_obs <= 64 * OBS_REGS;
+  const uint32_t stride = R.det_on ? TILE_THREADS : 64u;
+#pragma unroll
+  for (int k = 0; k < OBS_REGS; k++) {
+    const uint32_t j = tid + stride * k;
+    R.o[k] = (R.det_on || (R.dig_on && tid < 64)) && j < A.n_obs ? A.obs_slot[j] : NONE;
+  }
+  return R;
+}
+
+template <int W>
+__device__ __forceinline__ void observe_values(const TileArgs& A, uint32_t* lds, const ObsRegs& R, Words<W>& sreg,
+                                               uint64_t c, uint64_t cyc, uint64_t w0, Words<W>& det) {
+  const uint32_t tid = threadIdx.x;
+  if (R.det_on) {
+    Words<W> v[OBS_REGS];
+#pragma unroll
+    for (int k = 0; k < OBS_REGS; k++)
+      if (R.o[k] != NONE) v[k] = ld_words<W>(lds, R.o[k] & 0x7FFFFFFFu);
+#pragma unroll
+    for (int k = 0; k < OBS_REGS; k++)
+      if (R.o[k] != NONE)
+#pragma unroll
+        for (int q = 0; q < W; q++) det.v[q] |= v[k].v[q] ^ (0u - (v[k].v[q & ~1] & 1u));
+  } else if (A.detect)
     for (uint32_t j = tid; j < A.n_obs; j += TILE_THREADS) {
       const Words<W> v = ld_words<W>(lds, A.obs_slot[j] & 0x7FFFFFFFu);
 #pragma unroll
@@ -207,13 +243,30 @@ __device__ __forceinline__ void observe_values(const TileArgs& A, uint32_t* lds,
       if (gw < A.nwords) A.trace[(c * A.n_obs + j) * A.nwords + gw] = lds[(o & 0x7FFFFFFFu) * W + w] ^ (0u - (o >> 31));
     }
   if (A.digest && tid < 64) {
-    for (uint32_t j = (tid - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) {
-      const uint32_t o = A.obs_slot[j];
-      const Words<W> v = ld_words<W>(lds, o & 0x7FFFFFFFu);
-      const uint32_t inv = 0u - (o >> 31);
+    if (R.dig_on) {
+      // lane q folds signals j = q (mod 64); S[p] takes them from lane (p - cyc) mod 64
+      Words<W> v[OBS_REGS], x;
 #pragma unroll
-      for (int k = 0; k < W; k++) sreg.v[k] ^= v.v[k] ^ inv;
-    }
+      for (int k = 0; k < OBS_REGS; k++)
+        if (R.o[k] != NONE) v[k] = ld_words<W>(lds, R.o[k] & 0x7FFFFFFFu);
+#pragma unroll
+      for (int q = 0; q < W; q++) x.v[q] = 0u;
+#pragma unroll
+      for (int k = 0; k < OBS_REGS; k++)
+        if (R.o[k] != NONE)
+#pragma unroll
+          for (int q = 0; q < W; q++) x.v[q] ^= v[k].v[q] ^ (0u - (R.o[k] >> 31));
+      const int src = int((tid - uint32_t(cyc & 63)) & 63);
+#pragma unroll
+      for (int q = 0; q < W; q++) sreg.v[q] ^= uint32_t(__shfl(int(x.v[q]), src, 64));
+    } else
+      for (uint32_t j = (tid - uint32_t(cyc & 63)) & 63; j < A.n_obs; j += 64) {
+        const uint32_t o = A.obs_slot[j];
+        const Words<W> v = ld_words<W>(lds, o & 0x7FFFFFFFu);
+        const uint32_t inv = 0u - (o >> 31);
+#pragma unroll
+        for (int k = 0; k < W; k++) sreg.v[k] ^= v.v[k] ^ inv;
+      }
     const int pa = int((tid + 17) & 63), pb = int((tid + 41) & 63);
 #pragma unroll
     for (int k = 0; k < W; k++) {
@@ -224,9 +277,9 @@ __device__ __forceinline__ void observe_values(const TileArgs& A, uint32_t* lds,
 }
 
 template <int W>
-__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, Words<W>& sreg, uint64_t c, uint64_t cyc,
-                                        uint64_t w0, Words<W>& det) {
-  observe_values<W>(A, lds, sreg, c, cyc, w0, det);
+__device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, const ObsRegs& R, Words<W>& sreg, uint64_t c,
+                                        uint64_t cyc, uint64_t w0, Words<W>& det) {
+  observe_values<W>(A, lds, R, sreg, c, cyc, w0, det);
   // the commit (or the next cycle's inputs) may overwrite an observed slot
   __syncthreads();
 }
@@ -237,7 +290,7 @@ constexpr uint32_t SINK_SLOTS = 64;     // per-tile slots written by lanes with 
 constexpr uint32_t GATE_VARIANT_BYTE_ADDR = 0x100;
 constexpr uint32_t GATE_VARIANT_M_SHIFT = 12;
 
-constexpr int FAST_COMMITS_PER_THREAD = 16;
+constexpr int FAST_COMMITS_PER_THREAD = 8;
 constexpr int FAST_COMMIT_BATCH = 8;
 
 // This thread's reset-free commits (entries tid, tid + 256, ...), loaded once per run.
@@ -303,14 +356,14 @@ __device__ __forceinline__ void commit_general(const TileArgs& A, uint32_t* lds)
   }
 }
 
-// Input words of one tile for one cycle, for the thread that drives input s.
+// Input words of one tile for one cycle, for the thread that drives an input
+// (mode, stream and argument of its table entry).
 template <int W>
-__device__ __forceinline__ Words<W> stimulus(const TileArgs& A, uint32_t s, uint64_t cyc, uint64_t w0, const uint64_t* gkey) {
-  const uint32_t mode = A.stim_mode[s];
-  const uint64_t arg = A.stim_arg[s];
+__device__ __forceinline__ Words<W> stimulus_of(const TileArgs& A, uint32_t mode, uint32_t stream, uint64_t arg,
+                                                uint64_t cyc, uint64_t w0, const uint64_t* gkey) {
   Words<W> v;
   if (mode == IN_RANDOM) {
-    const uint64_t key = stim_key(A.seed, A.stim_stream[s], cyc);
+    const uint64_t key = stim_key(A.seed, stream, cyc);
 #pragma unroll
     for (int k = 0; k < W; k += 2) {
       const uint64_t gw = w0 + k;
@@ -320,7 +373,7 @@ __device__ __forceinline__ Words<W> stimulus(const TileArgs& A, uint32_t s, uint
     }
   } else if (mode == IN_GROUP_RANDOM) {
     // one bit per 64-lane group, the same in all of its lanes
-    const uint64_t key = stim_key(A.seed, A.stim_stream[s], cyc);
+    const uint64_t key = stim_key(A.seed, stream, cyc);
 #pragma unroll
     for (int k = 0; k < W; k += 2) {
       const uint32_t b = 0u - uint32_t(stim_from_key(key, gkey[k / 2]) & 1u);
@@ -334,6 +387,31 @@ __device__ __forceinline__ Words<W> stimulus(const TileArgs& A, uint32_t s, uint
     for (int k = 0; k < W; k++) v.v[k] = bit ? ~0u : 0u;
   }
   return v;
+}
+
+// The input table entry of input tid (the first input a thread drives), held
+// in registers for the run; inputs tid + 256, ... are read from the table.
+// stream_mode: the stream (a multiple of 256) | the mode; slot NONE: no input.
+struct StimReg { uint32_t slot, stream_mode; uint64_t arg; };
+
+__device__ __forceinline__ StimReg load_stim(const TileArgs& A) {
+  StimReg r{NONE, 0u, 0ull};
+  if (threadIdx.x < A.n_stim) {
+    r.slot = A.stim_slot[threadIdx.x];
+    r.stream_mode = A.stim_stream[threadIdx.x] | A.stim_mode[threadIdx.x];
+    r.arg = A.stim_arg[threadIdx.x];
+  }
+  return r;
+}
+
+// Writes every input of the tile for cycle cyc (the caller's barrier publishes them).
+template <int W>
+__device__ __forceinline__ void drive_inputs(const TileArgs& A, uint32_t* lds, const StimReg& R, uint64_t cyc,
+                                             uint64_t w0, const uint64_t* gkey) {
+  if (R.slot != NONE)
+    st_words<W>(lds, R.slot, stimulus_of<W>(A, R.stream_mode & 0xFFu, R.stream_mode & ~0xFFu, R.arg, cyc, w0, gkey));
+  for (uint32_t s = threadIdx.x + TILE_THREADS; s < A.n_stim; s += TILE_THREADS)
+    st_words<W>(lds, A.stim_slot[s], stimulus_of<W>(A, A.stim_mode[s], A.stim_stream[s], A.stim_arg[s], cyc, w0, gkey));
 }
 
 // One stage: this thread's positions (descriptors fetched STAGE_AHEAD stages
@@ -418,12 +496,14 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
   FastCommits F;
   if constexpr (kBA) F = load_fast_commits(A);
   else F.n = 0;
+  const StimReg SR = load_stim(A);
+  const ObsRegs OR = load_obs(A);
   for (uint64_t c = 0; c < A.cycles; c++) {
     const uint64_t cyc = A.cycle_base + c;
     // primary inputs for this cycle (thread per input, all words of the tile);
     // with a merged tail they were written by the previous cycle's tail
     if (c == 0 || !A.merge_tail) {
-      for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS) st_words<W>(lds, A.stim_slot[s], stimulus<W>(A, s, cyc, w0, gkey));
+      drive_inputs<W>(A, lds, SR, cyc, w0, gkey);
       __syncthreads();
     }
     // cycle program, observe before the edge, commit, post program
@@ -431,22 +511,20 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
     if (A.merge_tail) {
       // observed slots, committed registers and input slots are disjoint, and
       // no committed value is read from an input slot: one barrier phase
-      if (A.n_obs) observe_values<W>(A, lds, sreg, c, cyc, w0, det);
+      if (A.n_obs) observe_values<W>(A, lds, OR, sreg, c, cyc, w0, det);
       if constexpr (kBA) commit_fast<W>(lds, F);
       commit_general<W>(A, lds);
-      if (c + 1 < A.cycles)
-        for (uint32_t s = tid; s < A.n_stim; s += TILE_THREADS)
-          st_words<W>(lds, A.stim_slot[s], stimulus<W>(A, s, cyc + 1, w0, gkey));
+      if (c + 1 < A.cycles) drive_inputs<W>(A, lds, SR, cyc + 1, w0, gkey);
       __syncthreads();
       if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
     } else {
-      if (!A.observe_post && A.n_obs) observe<W>(A, lds, sreg, c, cyc, w0, det);
+      if (!A.observe_post && A.n_obs) observe<W>(A, lds, OR, sreg, c, cyc, w0, det);
       if constexpr (kBA) commit_fast<W>(lds, F);
       commit_general<W>(A, lds);
       __syncthreads();
       if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
       if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fm);
-      if (A.observe_post && A.n_obs) observe<W>(A, lds, sreg, c, cyc, w0, det);
+      if (A.observe_post && A.n_obs) observe<W>(A, lds, OR, sreg, c, cyc, w0, det);
     }
   }
   for (uint32_t i = tid; i < A.n_load * W; i += TILE_THREADS) {
@@ -470,17 +548,18 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
 }
 
 template <int W, bool kBA, int M>
-__global__ void __launch_bounds__(TILE_THREADS) tile_kernel(TileArgs A) {
+__global__ void __launch_bounds__(TILE_THREADS, TILE_WAVES_PER_SIMD) tile_kernel(TileArgs A) {
   extern __shared__ __align__(16) uint32_t lds[];
   tile_body<W, kBA, M>(A, blockIdx.x, lds);
 }
 
 // Calls f(W, byte_addr, M) with each as a std::integral_constant, for the
-// compiled variants: W in {1, 2, 4, 8}, M in {2, 3}.
+// compiled variants: W in {1, 2, 4, 8}, M in {2, 3, 4}.
 template <class F>
 __host__ __device__ inline void with_tile_variant(uint32_t W, bool byte_addr, uint32_t M, F&& f) {
   auto m_ = [&](auto w, auto ba) {
-    if (M == 3) f(w, ba, std::integral_constant<int, 3>{});
+    if (M == 4) f(w, ba, std::integral_constant<int, 4>{});
+    else if (M == 3) f(w, ba, std::integral_constant<int, 3>{});
     else f(w, ba, std::integral_constant<int, 2>{});
   };
   auto b_ = [&](auto w) {
@@ -517,7 +596,7 @@ struct QueueArgs {
 // compiler may build an inner exec-mask loop that runs the other lanes again on
 // the old item while thread 0 waits at its exit.)
 template <int W, bool kBA, int M>
-__global__ void __launch_bounds__(TILE_THREADS) tile_queue_kernel(TileArgs A, QueueArgs Q) {
+__global__ void __launch_bounds__(TILE_THREADS, TILE_WAVES_PER_SIMD) tile_queue_kernel(TileArgs A, QueueArgs Q) {
   extern __shared__ __align__(16) uint32_t lds[];
   __shared__ uint32_t s_item, s_err;
   const uint32_t total = Q.n_tiles * Q.n_chunks;

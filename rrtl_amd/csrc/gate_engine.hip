@@ -1065,7 +1065,7 @@ class GateEngine final : public Engine {
       T = compile_tile(N, P, spec, forced, opt_.stage_gates);
     } else {
       double best = 0;
-      for (uint32_t m : {2u, 3u}) {
+      for (uint32_t m : {2u, 3u, 4u}) {
         TileProgram C = compile_tile(N, P, spec, forced, m);
         const double cost = double(C.n_cycle_stages + C.n_post_stages) * (2.0 + m);
         if (m == 2 || cost < best) { best = cost; T = std::move(C); }

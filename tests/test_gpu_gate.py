@@ -129,7 +129,7 @@ def netlist_outputs(p):
     return [i for i, s in enumerate(p.signals) if s.kind == KIND_OUTPUT]
 
 
-@pytest.mark.parametrize("tile_words,stage_gates", [(1, 2), (2, 2), (4, 2), (8, 2), (1, 3), (2, 3), (4, 3), (8, 3)])
+@pytest.mark.parametrize("tile_words,stage_gates", [(1, 2), (2, 2), (4, 2), (8, 2), (1, 3), (2, 3), (4, 3), (8, 3), (1, 4), (2, 4), (4, 4), (8, 4)])
 def test_synthetic_netlist_trace_matches_oracle(tile_words, stage_gates):
     flush_print(f"case: small synthetic netlist tile_words={tile_words} stage_gates={stage_gates}")
     p = small_netlist_program(gates=3000, levels=24, dffs=300, inputs=48, outputs=32, seed=4)
