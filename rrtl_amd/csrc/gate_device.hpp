@@ -17,10 +17,14 @@ constexpr uint32_t G_COPY = 0;    // b = ones
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t CLAMP_COMMIT = 0xFFFFFFFEu;
 constexpr uint32_t TILE_THREADS = 256;
-// Waves per SIMD the tile kernels are register-budgeted for (at most 168
-// VGPRs each): a tile of the 20k-gate netlist takes 49 KB of LDS, so three
-// workgroups share a CU, one wave of each on every SIMD.
-constexpr int TILE_WAVES_PER_SIMD = 3;
+// Waves per SIMD the tile kernels are register-budgeted for.  The one-pass
+// kernel runs when every tile is resident at once, which for the 20k-gate
+// netlist (39.5 KB of LDS per 64-lane tile) means four workgroups per CU, one
+// wave of each on every SIMD: at most 128 VGPRs.  The work-queue kernel runs
+// when tiles outnumber the resident slots, typically at three tiles per CU: at
+// most 168 VGPRs.
+constexpr int TILE_WAVES_PER_SIMD = 4;
+constexpr int QUEUE_WAVES_PER_SIMD = 3;
 
 struct CGate { uint32_t op, a, b, dst; };
 struct CCommit { uint32_t reg, val, rsig, pol, rbit, has_reset; };
@@ -596,7 +600,7 @@ struct QueueArgs {
 // compiler may build an inner exec-mask loop that runs the other lanes again on
 // the old item while thread 0 waits at its exit.)
 template <int W, bool kBA, int M>
-__global__ void __launch_bounds__(TILE_THREADS, TILE_WAVES_PER_SIMD) tile_queue_kernel(TileArgs A, QueueArgs Q) {
+__global__ void __launch_bounds__(TILE_THREADS, QUEUE_WAVES_PER_SIMD) tile_queue_kernel(TileArgs A, QueueArgs Q) {
   extern __shared__ __align__(16) uint32_t lds[];
   __shared__ uint32_t s_item, s_err;
   const uint32_t total = Q.n_tiles * Q.n_chunks;

@@ -431,6 +431,30 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
   for (uint32_t r = 0; r < RES; r++) fresh[r] = first_free + ((r + RES - first_free % RES) % RES);
   uint32_t slot_end = first_free;
   std::multimap<uint32_t, uint32_t> releases;  // last-read pos -> slot
+  // Borrowed register slots.  A register's slot is rewritten by the commit at
+  // the end of every cycle, so between the last read of its value in the
+  // cycle program and the commit it can hold a cycle-program value that dies
+  // before the commit (last read at a cycle stage, not at the observation or
+  // the commit).  Not for registers read by the commit itself (staged), by the
+  // observation, forced, or written by a gate.
+  std::vector<uint8_t> reg_borrow(N.n_nodes, 0);
+  for (auto& c : N.commits) if (P.signals[c.reg].kind == K_REG && !forced[c.reg]) reg_borrow[c.reg] = 1;
+  for (auto& c : N.commits) { reg_borrow[c.val] = 0; if (c.rsig != NONE) reg_borrow[c.rsig] = 0; }
+  for (uint32_t s : spec.obs_sig) reg_borrow[s] = 0;
+  for (size_t i = 0; i < ng; i++) reg_borrow[gl[i]->dst] = 0;
+  std::vector<uint32_t> reg_last(N.n_nodes, 0);
+  for (auto& in : insts) {
+    if (in.post) continue;
+    for (int op = 0; op < 2; op++) {
+      const uint32_t node = op == 0 ? gl[in.gate]->a : gl[in.gate]->b;
+      if (node < nsig && pers[node]) reg_last[node] = std::max(reg_last[node], in.pos);
+    }
+  }
+  std::vector<std::vector<uint32_t>> borrow_res(RES);
+  std::vector<uint8_t> slot_borrowed(first_free, 0);
+  std::multimap<uint32_t, uint32_t> borrow_releases;  // free after this pos -> slot
+  for (auto& kv : T.persist)
+    if (reg_borrow[kv.first]) { slot_borrowed[kv.second] = 1; borrow_releases.emplace(reg_last[kv.first], kv.second); }
   std::vector<uint32_t> lane_of(insts.size(), NONE);
   std::vector<uint32_t> write_mask;            // per (position, depth, half): residues written
   std::vector<uint32_t> mask_base(size_t(max_pos) + 2, 0);
@@ -442,8 +466,13 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
   std::vector<uint32_t> half_fill(HALVES);
   for (uint32_t pos = 1; pos <= max_pos; pos++) {
     while (!releases.empty() && releases.begin()->first < pos) {
-      free_res[releases.begin()->second % RES].push_back(releases.begin()->second);
+      const uint32_t sl = releases.begin()->second;
+      (sl < first_free && slot_borrowed[sl] ? borrow_res : free_res)[sl % RES].push_back(sl);
       releases.erase(releases.begin());
+    }
+    while (!borrow_releases.empty() && borrow_releases.begin()->first < pos) {
+      borrow_res[borrow_releases.begin()->second % RES].push_back(borrow_releases.begin()->second);
+      borrow_releases.erase(borrow_releases.begin());
     }
     const std::vector<uint32_t>& v = at_pos[pos];
     if (v.empty()) continue;
@@ -498,16 +527,19 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
         const uint32_t d = gl[in.gate]->dst;
         if (d < nsig && pers[d]) { in.slot = T.persist.at(d); taken |= 1u << (in.slot % RES); continue; }
         if (last[ii] == 0) { in.slot = NONE - 1; continue; }  // read only through the register chain
+        const bool may_borrow = !in.post && last[ii] <= L;
+        auto avail = [&](uint32_t c) { return free_res[c].size() + (may_borrow ? borrow_res[c].size() : 0); };
         uint32_t r = RES;
         for (uint32_t c = 0; c < RES; c++) {
           if (taken >> c & 1u) continue;
           if (r == RES) { r = c; continue; }
-          const size_t fc = free_res[c].size(), fr = free_res[r].size();
+          const size_t fc = avail(c), fr = avail(r);
           if (fc > fr || (fc == 0 && fr == 0 && fresh[c] < fresh[r])) r = c;
         }
         if (r == RES) r = 0;  // every residue taken: accept a conflict
         taken |= 1u << r;
-        if (!free_res[r].empty()) { in.slot = free_res[r].back(); free_res[r].pop_back(); }
+        if (may_borrow && !borrow_res[r].empty()) { in.slot = borrow_res[r].back(); borrow_res[r].pop_back(); }
+        else if (!free_res[r].empty()) { in.slot = free_res[r].back(); free_res[r].pop_back(); }
         else { in.slot = fresh[r]; fresh[r] += RES; }
         slot_end = std::max(slot_end, in.slot + 1);
         releases.emplace(std::max(last[ii], in.pos), in.slot);
@@ -966,10 +998,18 @@ class GateEngine final : public Engine {
 
   uint32_t choose_tile_words(uint32_t n_slots) const {
     if (opt_.tile_words) return opt_.tile_words;
-    // largest tile that still leaves room for two workgroups per CU
-    for (uint32_t w : {8u, 4u, 2u, 1u})
-      if (tile_lds_bytes(n_slots, w) * 2 <= lds_max_) return w;
-    return 1;
+    // The W >= 2 that keeps the most workgroups resident: tiles of the run,
+    // capped by the CUs times the workgroups whose LDS fits a CU (at most
+    // four: the one-pass kernel's register budget); ties go to the wider tile,
+    // which spends fewer descriptor fetches and address operations per lane.
+    uint32_t best = 0;
+    uint64_t best_res = 0;
+    for (uint32_t w : {2u, 4u, 8u}) {
+      const uint64_t per_cu = std::min<uint64_t>(lds_max_ / tile_lds_bytes(n_slots, w), TILE_WAVES_PER_SIMD);
+      const uint64_t res = std::min<uint64_t>((nwords_ + w - 1) / w, per_cu * cus_);
+      if (per_cu > 0 && res >= best_res) { best = w; best_res = res; }
+    }
+    return best ? best : 1;
   }
 
   // W lane-words per tile; byte_addr: descriptors hold LDS byte offsets.
@@ -1057,9 +1097,10 @@ class GateEngine final : public Engine {
     for (auto& f : forces_) forced[f.sig] = 1;
     // Gate positions per thread per stage: the M whose stage count x (fixed
     // stage cost + per-position cost) is lowest, unless the options fix it.
-    // A stage's fixed part (the barrier and an LDS round trip before it) costs
-    // about two positions: on the 20k-gate netlist 26 stages of 3 positions
-    // run 7 % faster than 35 stages of 2.
+    // With four workgroups per CU the barrier waits are hidden and a stage
+    // costs what its positions cost: on the 20k-gate netlist 35 stages of 2
+    // positions (70 per cycle) run 12 % faster than 26 of 3 (78) and 25 %
+    // faster than 22 of 4 (88).
     TileProgram T;
     if (opt_.stage_gates) {
       T = compile_tile(N, P, spec, forced, opt_.stage_gates);
@@ -1067,7 +1108,7 @@ class GateEngine final : public Engine {
       double best = 0;
       for (uint32_t m : {2u, 3u, 4u}) {
         TileProgram C = compile_tile(N, P, spec, forced, m);
-        const double cost = double(C.n_cycle_stages + C.n_post_stages) * (2.0 + m);
+        const double cost = double(C.n_cycle_stages + C.n_post_stages) * (0.25 + m);
         if (m == 2 || cost < best) { best = cost; T = std::move(C); }
       }
     }
@@ -1229,8 +1270,11 @@ class GateEngine final : public Engine {
     Launch L;
     L.R = prepare(spec, with_outputs);
     L.cycles = spec.cycles;
-    L.resident = resident_tiles(L.R.W, L.R.byte_addr, L.R.M, L.R.lds);
-    L.chunks = opt_.tile_chunks ? opt_.tile_chunks : choose_chunks(L.R.tiles, L.resident);
+    // one pass when the one-pass kernel holds every tile at once, else the
+    // work queue with its own residency
+    const uint32_t one_pass = resident_tiles(tile_fn(L.R.W, L.R.byte_addr, L.R.M), L.R.lds);
+    L.resident = L.R.tiles <= one_pass ? one_pass : resident_tiles(queue_fn(L.R.W, L.R.byte_addr, L.R.M), L.R.lds);
+    L.chunks = opt_.tile_chunks ? opt_.tile_chunks : (L.R.tiles <= one_pass ? 1u : choose_chunks(L.R.tiles, L.resident));
     if (L.chunks > 1) qctl_.alloc(size_t(L.R.tiles) + 2);
     if (L.R.lds > 65536)
       RRTL_HIP_CHECK(hipFuncSetAttribute(tile_fn(L.R.W, L.R.byte_addr, L.R.M), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1345,8 +1389,9 @@ class GateEngine final : public Engine {
     return fn;
   }
   // Workgroups of the queue kernel the device holds at once.
-  uint32_t resident_tiles(uint32_t W, bool byte_addr, uint32_t M, size_t lds) {
-    const void* fn = queue_fn(W, byte_addr, M);
+  // Tiles the device holds at once for a kernel (the one-pass and the
+  // work-queue kernels have different register budgets).
+  uint32_t resident_tiles(const void* fn, size_t lds) {
     if (lds > 65536) RRTL_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     int per_cu = 0;
     RRTL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE_THREADS, lds));
