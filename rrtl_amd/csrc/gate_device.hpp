@@ -5,6 +5,10 @@
 #include "common.hpp"
 #include <type_traits>
 
+#ifndef RRTL_STAGE_AHEAD
+#define RRTL_STAGE_AHEAD 2
+#endif
+
 namespace rrtl {
 namespace gate {
 
@@ -91,10 +95,17 @@ __device__ __forceinline__ void st_words(uint32_t* lds, uint32_t slot, const Wor
   }
 }
 
-// Gate descriptor, one per thread, stage and position (8 bytes): x = a | b << 16,
-// y = dst | op << 16 | chained << 20.  a, b and dst are LDS byte offsets when
-// the tile fits in 64 KiB (kBA) and slot indices otherwise.  XOR-family ops
-// carry no input inversions (they cancel into the output inversion).  A
+// Gate descriptor, one per thread, stage and position (8 bytes).  When the tile
+// fits in 64 KiB (kBA) a, b and dst are LDS byte offsets and
+//   x = a | b << 16 at position 0; at position 1 operand a comes from
+//       registers, so its low half holds the mode bits of every position k
+//       of the thread: xor << 2k | fix << (2k + 1); later positions: b << 16,
+//   y = dst | (ia ? 0xFF : 0) << 16 | (ib ? 0xFF : 0) << 24,
+// so the input inversions are sign-extended bytes that the XORs read directly
+// (no mask is built for them).  Otherwise a, b and dst are slot indices:
+// x = a | b << 16, y = dst | op << 16.
+// XOR-family ops carry no input inversions (they cancel into the output
+// inversion).  A
 // position without a gate reads an operand some other lane of its half-wave
 // reads (or constant 0) and writes one of 64 sink slots, so every thread runs
 // the same straight-line code.
@@ -147,6 +158,7 @@ __device__ __forceinline__ void load_stage(StageDesc<M>& D, const uint2* row) {
 template <int W, bool kBA, int M, bool kFix>
 __device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D) {
   Words<W> va, vb[M];
+  static_assert(!kBA || M >= 2, "the mode bits live in position 1's descriptor");
   va = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[0].x & 0xFFFFu));
 #pragma unroll
   for (int k = 0; k < M; k++) vb[k] = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[k].x >> 16));
@@ -154,12 +166,18 @@ __device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D)
 #pragma unroll
   for (int k = 0; k < M; k++) {
     const uint32_t y = D.d[k].y;
-    const uint32_t ia = 0u - ((y >> 16) & 1u), ib = 0u - ((y >> 17) & 1u), x = 0u - ((y >> 19) & 1u);
+    uint32_t ia, ib, x, io;
+    if constexpr (kBA) {
+      const uint32_t modes = D.d[M > 1 ? 1 : 0].x;
+      ia = uint32_t(int32_t(int8_t(y >> 16))); ib = uint32_t(int32_t(int8_t(y >> 24)));
+      x = 0u - ((modes >> (2 * k)) & 1u); io = 0u - ((modes >> (2 * k + 1)) & 1u);
+    } else {
+      ia = 0u - ((y >> 16) & 1u); ib = 0u - ((y >> 17) & 1u); x = 0u - ((y >> 19) & 1u); io = 0u - ((y >> 18) & 1u);
+    }
     Words<W> r;
 #pragma unroll
     for (int w = 0; w < W; w++) r.v[w] = op3_xor_or_and((k == 0 ? va.v[w] : prev.v[w]) ^ ia, vb[k].v[w] ^ ib, x);
     if constexpr (kFix) {
-      const uint32_t io = 0u - ((y >> 18) & 1u);
 #pragma unroll
       for (int w = 0; w < W; w++) r.v[w] ^= io;
     }
@@ -288,7 +306,7 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, const 
   __syncthreads();
 }
 
-constexpr uint32_t STAGE_AHEAD = 2;     // stages of descriptor look-ahead
+constexpr uint32_t STAGE_AHEAD = RRTL_STAGE_AHEAD;     // stages of descriptor look-ahead
 constexpr uint32_t SINK_SLOTS = 64;     // per-tile slots written by lanes with no output to keep
 // batch job variant: tile words W | descriptors hold byte offsets | positions per stage M << 12
 constexpr uint32_t GATE_VARIANT_BYTE_ADDR = 0x100;
@@ -439,16 +457,17 @@ __device__ __forceinline__ void stage_step(const TileArgs& A, uint32_t* lds, uin
 template <int W, bool kBA, int M, bool kClamp, bool kFix>
 __device__ __forceinline__ void run_stages(const TileArgs& A, uint32_t* lds, uint32_t S0, uint32_t S1, uint32_t fb,
                                            uint32_t fe) {
-  static_assert(STAGE_AHEAD == 2, "run_stages is unrolled for two descriptor register sets");
-  StageDesc<M> D0, D1;
-  load_stage<M>(D0, A.desc + size_t(S0) * (uint32_t(M) * TILE_THREADS));
-  load_stage<M>(D1, A.desc + size_t(S0 + 1) * (uint32_t(M) * TILE_THREADS));
+  StageDesc<M> D[STAGE_AHEAD];
+#pragma unroll
+  for (uint32_t q = 0; q < STAGE_AHEAD; q++) load_stage<M>(D[q], A.desc + size_t(S0 + q) * (uint32_t(M) * TILE_THREADS));
   uint32_t S = S0;
-  for (; S + 2 <= S1; S += 2) {
-    stage_step<W, kBA, M, kClamp, kFix>(A, lds, S, D0, fb, fe);
-    stage_step<W, kBA, M, kClamp, kFix>(A, lds, S + 1, D1, fb, fe);
+  for (; S + STAGE_AHEAD <= S1; S += STAGE_AHEAD) {
+#pragma unroll
+    for (uint32_t q = 0; q < STAGE_AHEAD; q++) stage_step<W, kBA, M, kClamp, kFix>(A, lds, S + q, D[q], fb, fe);
   }
-  if (S < S1) stage_step<W, kBA, M, kClamp, kFix>(A, lds, S, D0, fb, fe);
+#pragma unroll
+  for (uint32_t q = 0; q < STAGE_AHEAD; q++)
+    if (S + q < S1) stage_step<W, kBA, M, kClamp, kFix>(A, lds, S + q, D[q], fb, fe);
 }
 
 template <int W, bool kBA, int M>

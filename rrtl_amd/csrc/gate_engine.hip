@@ -460,6 +460,11 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
   std::vector<uint32_t> mask_base(size_t(max_pos) + 2, 0);
   for (uint32_t pos = 1; pos <= max_pos + 1; pos++) mask_base[pos] = mask_base[pos - 1] + M * HALVES;
   write_mask.assign(mask_base[max_pos + 1] + M * HALVES, 0);
+  // LDS writes are served per quarter-wave over 32 banks: at W = 2 the 16
+  // lanes of a quarter write without conflict when their slots differ mod 16.
+  // Per (position, depth, half, quarter): residues mod 16 written.
+  constexpr uint32_t QRES = 16;
+  std::vector<uint32_t> write_mask_q(write_mask.size() * 2, 0);
   std::vector<uint32_t> cnt_a(size_t(M) * HALVES * RES), cnt_b(size_t(M) * HALVES * RES);
   std::vector<std::vector<uint32_t>> half_slots(size_t(M) * HALVES);
   std::vector<std::vector<uint32_t>> by_thread(TILE_THREADS);
@@ -523,21 +528,31 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
       for (uint32_t ii : v) {
         Inst& in = insts[ii];
         if (in.depth != k) continue;
-        uint32_t& taken = write_mask[mask_base[pos] + k * HALVES + lane_of[ii] / RES];
+        const size_t mi = mask_base[pos] + k * HALVES + lane_of[ii] / RES;
+        uint32_t& taken = write_mask[mi];
+        uint32_t& taken_q = write_mask_q[mi * 2 + (lane_of[ii] % RES) / QRES];
         const uint32_t d = gl[in.gate]->dst;
-        if (d < nsig && pers[d]) { in.slot = T.persist.at(d); taken |= 1u << (in.slot % RES); continue; }
+        if (d < nsig && pers[d]) {
+          in.slot = T.persist.at(d);
+          taken |= 1u << (in.slot % RES); taken_q |= 1u << (in.slot % QRES);
+          continue;
+        }
         if (last[ii] == 0) { in.slot = NONE - 1; continue; }  // read only through the register chain
         const bool may_borrow = !in.post && last[ii] <= L;
         auto avail = [&](uint32_t c) { return free_res[c].size() + (may_borrow ? borrow_res[c].size() : 0); };
+        // a residue unused by the other outputs of the half (mod 32) and of the
+        // quarter (mod 16); failing that, of the half only; else accept a conflict
         uint32_t r = RES;
-        for (uint32_t c = 0; c < RES; c++) {
-          if (taken >> c & 1u) continue;
-          if (r == RES) { r = c; continue; }
-          const size_t fc = avail(c), fr = avail(r);
-          if (fc > fr || (fc == 0 && fr == 0 && fresh[c] < fresh[r])) r = c;
-        }
-        if (r == RES) r = 0;  // every residue taken: accept a conflict
-        taken |= 1u << r;
+        for (int strict = 1; strict >= 0 && r == RES; strict--)
+          for (uint32_t c = 0; c < RES; c++) {
+            if (taken >> c & 1u) continue;
+            if (strict && (taken_q >> (c % QRES) & 1u)) continue;
+            if (r == RES) { r = c; continue; }
+            const size_t fc = avail(c), fr = avail(r);
+            if (fc > fr || (fc == 0 && fr == 0 && fresh[c] < fresh[r])) r = c;
+          }
+        if (r == RES) r = 0;
+        taken |= 1u << r; taken_q |= 1u << (r % QRES);
         if (may_borrow && !borrow_res[r].empty()) { in.slot = borrow_res[r].back(); borrow_res[r].pop_back(); }
         else if (!free_res[r].empty()) { in.slot = free_res[r].back(); free_res[r].pop_back(); }
         else { in.slot = fresh[r]; fresh[r] += RES; }
@@ -670,19 +685,28 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
           if (!have_a && (e[0] & 0xFFFFu) != 0xFFFFu) { rep_a = e[0] & 0xFFFFu; have_a = true; }
           if (!have_b && (e[0] >> 16) != 0xFFFFu) { rep_b = e[0] >> 16; have_b = true; }
         }
-        uint32_t next_r = 0;
-        auto sink_slot = [&]() {
-          while (next_r < RES && (taken >> next_r & 1u)) next_r++;
-          const uint32_t r = next_r < RES ? next_r : 0u;
-          if (next_r < RES) taken |= 1u << next_r;
+        // residues mod 16 the real outputs of each quarter write
+        uint32_t qtaken[2] = {0u, 0u};
+        for (uint32_t l = h * RES; l < h * RES + RES; l++) {
+          const uint32_t* e = entry(kst, k, l);
+          if (e[0] != NONE && (e[1] & 0xFFFFu) != 0xFFFFu) qtaken[(l % RES) / QRES] |= 1u << ((e[1] & 0xFFFFu) % QRES);
+        }
+        // a sink residue unused in the half (mod 32) and in the lane's quarter (mod 16)
+        auto sink_slot = [&](uint32_t l) {
+          uint32_t& tq = qtaken[(l % RES) / QRES];
+          uint32_t r = RES;
+          for (uint32_t c = 0; c < RES && r == RES; c++) if (!(taken >> c & 1u) && !(tq >> (c % QRES) & 1u)) r = c;
+          for (uint32_t c = 0; c < RES && r == RES; c++) if (!(taken >> c & 1u)) r = c;
+          if (r == RES) r = 0;
+          taken |= 1u << r; tq |= 1u << (r % QRES);
           return T.sink + r;
         };
         for (uint32_t l = h * RES; l < h * RES + RES; l++) {
           uint32_t* e = entry(kst, k, l);
-          if (e[0] == NONE) { e[0] = rep_a | (rep_b << 16); e[1] = sink_slot() | (G_AND << 16); continue; }
+          if (e[0] == NONE) { e[0] = rep_a | (rep_b << 16); e[1] = sink_slot(l) | (G_AND << 16); continue; }
           if ((e[0] & 0xFFFFu) == 0xFFFFu) e[0] = (e[0] & 0xFFFF0000u) | rep_a;
           if ((e[0] >> 16) == 0xFFFFu) e[0] = (e[0] & 0xFFFFu) | (rep_b << 16);
-          if ((e[1] & 0xFFFFu) == 0xFFFFu) e[1] = (e[1] & 0xFFFF0000u) | sink_slot();
+          if ((e[1] & 0xFFFFu) == 0xFFFFu) e[1] = (e[1] & 0xFFFF0000u) | sink_slot(l);
         }
       }
   // persistent load/store lists
@@ -1140,7 +1164,25 @@ class GateEngine final : public Engine {
       if (i < size_t(KS) * stride) {
         const uint32_t x = T.desc[2 * i], y = T.desc[2 * i + 1];
         const uint32_t op = (y >> 16) & 0xFu;
-        desc[i] = make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16));
+        if (byte_addr) {
+          // position 0: a | b << 16; position 1: the thread's mode bits (xor,
+          // fix per position) | b << 16; later positions: b << 16; the input
+          // inversions as 0x00 / 0xFF bytes of y
+          const size_t k = (i % stride) / TILE_THREADS, row = i - (i % stride), lane = i % TILE_THREADS;
+          uint32_t low = 0;
+          if (k == 0) {
+            low = enc(x & 0xFFFFu);
+          } else if (k == 1) {
+            for (uint32_t q = 0; q < M; q++) {
+              const uint32_t oq = (T.desc[2 * (row + q * TILE_THREADS + lane) + 1] >> 16) & 0xFu;
+              low |= (((oq >> 3) & 1u) | (((oq >> 2) & 1u) << 1)) << (2 * q);
+            }
+          }
+          desc[i] = make_uint2(low | (enc(x >> 16) << 16),
+                               enc(y & 0xFFFFu) | ((op & GI_A) ? 0xFF0000u : 0u) | ((op & GI_B) ? 0xFF000000u : 0u));
+        } else {
+          desc[i] = make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16));
+        }
       } else {
         desc[i] = make_uint2(enc(T.c0) | (enc(T.c0) << 16), enc(sink + uint32_t(i % 32)));
       }
