@@ -173,6 +173,7 @@ struct TileProgram {
   std::vector<uint32_t> desc;
   std::vector<uint32_t> stage_fix;       // per stage: bit k = some gate at position k applies its inversion
   uint32_t n_gates = 0;                  // gates evaluated per cycle (cycle and post programs)
+  uint32_t direct_commits = 0;           // registers whose next value is computed into their own slot
   std::vector<uint32_t> load_slot, load_node;     // persistent slots loaded at start, stored at end
   std::vector<uint32_t> stim_slot, stim_mode, stim_stream;
   std::vector<uint64_t> stim_arg;
@@ -450,6 +451,16 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
       if (node < nsig && pers[node]) reg_last[node] = std::max(reg_last[node], in.pos);
     }
   }
+  // A register's next value may be written straight into the register's own
+  // slot when the register's last read comes before it and nothing else
+  // reads that slot at the commit (the value is the D of this one commit, no
+  // reset signal, not observed): the commit then has nothing to copy.
+  std::vector<uint32_t> direct_reg(N.n_nodes, NONE), val_uses(N.n_nodes, 0);
+  std::vector<uint8_t> observed_node(N.n_nodes, 0);
+  for (uint32_t s : spec.obs_sig) observed_node[s] = 1;
+  for (auto& c : N.commits) { val_uses[c.val]++; if (c.rsig != NONE) val_uses[c.rsig] += 2; }
+  for (auto& c : N.commits)
+    if (val_uses[c.val] == 1 && reg_borrow[c.reg] && !observed_node[c.val] && c.val != c.reg) direct_reg[c.val] = c.reg;
   std::vector<std::vector<uint32_t>> borrow_res(RES);
   std::vector<uint8_t> slot_borrowed(first_free, 0);
   std::multimap<uint32_t, uint32_t> borrow_releases;  // free after this pos -> slot
@@ -468,7 +479,7 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
   std::vector<uint32_t> cnt_a(size_t(M) * HALVES * RES), cnt_b(size_t(M) * HALVES * RES);
   std::vector<std::vector<uint32_t>> half_slots(size_t(M) * HALVES);
   std::vector<std::vector<uint32_t>> by_thread(TILE_THREADS);
-  std::vector<uint32_t> half_fill(HALVES);
+  std::vector<uint32_t> half_fill(HALVES), half_of(TILE_THREADS, 0);
   for (uint32_t pos = 1; pos <= max_pos; pos++) {
     while (!releases.empty() && releases.begin()->first < pos) {
       const uint32_t sl = releases.begin()->second;
@@ -503,7 +514,8 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
             if (in.chained && (op == 1) == in.chain_b) continue;
             const uint32_t s = operand_slot(in, op);
             if (std::find(hs.begin(), hs.end(), s) != hs.end()) continue;
-            const uint32_t c = (op == 0 ? cnt_a : cnt_b)[(in.depth * HALVES + h) * RES + s % RES];
+            // a later position's one read is issued as its b read
+            const uint32_t c = (op == 0 && in.depth == 0 ? cnt_a : cnt_b)[(in.depth * HALVES + h) * RES + s % RES];
             key += c * 1024;
           }
         }
@@ -516,12 +528,72 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
           if (in.chained && (op == 1) == in.chain_b) continue;
           const uint32_t s = operand_slot(in, op);
           if (std::find(hs.begin(), hs.end(), s) != hs.end()) continue;
-          (op == 0 ? cnt_a : cnt_b)[(in.depth * HALVES + best_h) * RES + s % RES]++;
+          (op == 0 && in.depth == 0 ? cnt_a : cnt_b)[(in.depth * HALVES + best_h) * RES + s % RES]++;
           hs.push_back(s);
         }
-        lane_of[ii] = best_h * RES + half_fill[best_h];
       }
+      half_of[t] = best_h;
       half_fill[best_h]++;
+    }
+    // Local search over the greedy placement: swap two threads of different
+    // halves when that lowers sum over (half, read instruction, residue) of
+    // (distinct slots)^2, a smooth stand-in for the passes each read takes.
+    // Read instructions: 0 = operand a at position 0, 1 + k = operand b at
+    // position k.  Deterministic (seeded by the stage).
+    {
+      std::vector<std::vector<std::pair<uint32_t, uint32_t>>> reads(TILE_THREADS);  // (instruction, slot)
+      std::vector<uint32_t> ths;
+      for (uint32_t t = 0; t < TILE_THREADS; t++) {
+        if (by_thread[t].empty()) continue;
+        ths.push_back(t);
+        for (uint32_t ii : by_thread[t]) {
+          const Inst& in = insts[ii];
+          for (int op = 0; op < 2; op++) {
+            if (in.chained && (op == 1) == in.chain_b) continue;
+            reads[t].push_back({op == 0 && in.depth == 0 ? 0u : 1u + in.depth, operand_slot(in, op)});
+          }
+        }
+      }
+      const uint32_t NI = M + 1;
+      std::vector<std::unordered_map<uint32_t, uint32_t>> refs(size_t(HALVES) * NI);
+      std::vector<int32_t> dcnt(size_t(HALVES) * NI * RES, 0);
+      auto move = [&](uint32_t t, uint32_t h, int sign) -> int64_t {
+        int64_t d = 0;
+        for (auto& r : reads[t]) {
+          const size_t hi = size_t(h) * NI + r.first;
+          uint32_t& rc = refs[hi][r.second];
+          const bool change = sign > 0 ? rc++ == 0 : --rc == 0;
+          if (!change) continue;
+          int32_t& c = dcnt[hi * RES + r.second % RES];
+          d += sign > 0 ? 2 * int64_t(c) + 1 : 1 - 2 * int64_t(c);
+          c += sign;
+        }
+        return d;
+      };
+      for (uint32_t t : ths) move(t, half_of[t], +1);
+      if (ths.size() > 1) {
+        uint64_t rs = 0x9E3779B97F4A7C15ull * (pos + 1);
+        auto rng = [&]() {  // xorshift64*
+          rs ^= rs >> 12; rs ^= rs << 25; rs ^= rs >> 27;
+          return uint32_t((rs * 0x2545F4914F6CDD1Dull) >> 32);
+        };
+        const uint32_t tries = uint32_t(ths.size()) * 40;
+        for (uint32_t it = 0; it < tries; it++) {
+          const uint32_t t1 = ths[rng() % ths.size()], t2 = ths[rng() % ths.size()];
+          const uint32_t h1 = half_of[t1], h2 = half_of[t2];
+          if (h1 == h2) continue;
+          int64_t d = move(t1, h1, -1) + move(t2, h2, -1);
+          d += move(t1, h2, +1) + move(t2, h1, +1);
+          if (d < 0) { half_of[t1] = h2; half_of[t2] = h1; continue; }
+          move(t1, h2, -1); move(t2, h1, -1); move(t1, h1, +1); move(t2, h2, +1);
+        }
+      }
+      std::fill(half_fill.begin(), half_fill.end(), 0u);
+      for (uint32_t t : ths) {
+        const uint32_t h = half_of[t];
+        for (uint32_t ii : by_thread[t]) lane_of[ii] = h * RES + half_fill[h];
+        half_fill[h]++;
+      }
     }
     // outputs, position by position within the stage
     for (uint32_t k = 0; k < M; k++)
@@ -538,6 +610,18 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
           continue;
         }
         if (last[ii] == 0) { in.slot = NONE - 1; continue; }  // read only through the register chain
+        if (!in.post && direct_reg[d] != NONE && src_inst(d, false) == ii) {
+          const uint32_t reg = direct_reg[d], qs = T.persist.at(reg);
+          auto& br = borrow_res[qs % RES];
+          auto it = std::find(br.begin(), br.end(), qs);
+          if (reg_last[reg] < in.pos && it != br.end() && !(taken >> (qs % RES) & 1u) && !(taken_q >> (qs % QRES) & 1u)) {
+            br.erase(it);
+            in.slot = qs;
+            taken |= 1u << (qs % RES); taken_q |= 1u << (qs % QRES);
+            releases.emplace(std::max(last[ii], in.pos), qs);
+            continue;
+          }
+        }
         const bool may_borrow = !in.post && last[ii] <= L;
         auto avail = [&](uint32_t c) { return free_res[c].size() + (may_borrow ? borrow_res[c].size() : 0); };
         // a residue unused by the other outputs of the half (mod 32) and of the
@@ -622,6 +706,8 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
       return t;
     };
     const uint32_t vpol = value_pol(c.val, false), rpol = c.rsig == NONE ? 0u : value_pol(c.rsig, false);
+    // the next value already sits in the register's slot with its true polarity
+    if (vs == T.persist.at(c.reg) && !c.has_reset && vpol == 0) { T.direct_commits++; continue; }
     if (committed.count(c.val)) vs = stage(vs);
     if (c.rsig != NONE && committed.count(c.rsig)) rs = stage(rs);
     // flags: reset polarity (with the stored polarity of the reset signal
