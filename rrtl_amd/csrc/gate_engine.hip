@@ -146,6 +146,9 @@ GateNet build_net(const Program& P) {
   N.written.assign(N.nsig, 0);
   // Read-before-write inside one settle would make the result depend on the
   // packet order (a combinational cycle): the levelized engines reject it.
+  // Registers are exempt: their only writers in a settle are async-reset
+  // rewrites, each of which reads the register's committed value before
+  // writing it, as the reference's packet order does (bitparallel.rs:256-300).
   std::vector<int64_t> first_w(N.n_nodes, -1);
   for (size_t i = 0; i < N.settle.size(); i++) {
     uint32_t d = N.settle[i].dst;
@@ -157,7 +160,7 @@ GateNet build_net(const Program& P) {
   }
   for (size_t i = 0; i < N.settle.size(); i++)
     for (uint32_t s : {N.settle[i].a, N.settle[i].b})
-      if (s < N.nsig && first_w[s] > int64_t(i))
+      if (s < N.nsig && first_w[s] > int64_t(i) && P.signals[s].kind != K_REG)
         throw Error("E_HIP_COMB_ORDER: signal `" + P.signals[s].name + "` is read before it is driven (combinational cycle)");
   return N;
 }

@@ -100,3 +100,57 @@ def random_design(seed: int, n_inputs: int = 4, n_regs: int = 4, n_wires: int = 
         o = m.output(f"out{i}", ty)
         m.assign(o, gen(ty, depth))
     return m
+
+
+def random_gate_design(seed: int, n_inputs: int = 24, n_regs: int = 40, n_wires: int = 300,
+                       observe_regs: bool = True, async_resets: bool = False) -> ir.Module:
+    """Random 1-bit netlist for the gate engine's tile compiler: registers with
+    sync (or async) resets of both polarities and without; register-to-register
+    shift chains (a next value that is another register is staged before the
+    commit); next values drawn from early and late wires (registers read only
+    early lend their slot; a late next value may land in its own register's
+    slot); outputs that are registers (observed register slots keep the
+    observation, commit and inputs in separate barrier phases) or wires."""
+    rng = random.Random(seed)
+    m = ir.Module(f"RandGates{seed}")
+    clk = m.input("clk", 1)
+    rst = m.input("rst", 1)
+    rstn = m.input("rstn", 1)
+    ins = [m.input(f"d{i}", 1) for i in range(n_inputs)]
+    regs = [m.reg(f"q{i}", 1) for i in range(n_regs)]
+    for q in regs:
+        m.clock(q, clk)
+    pool: List[ir.Expr] = [s.value() for s in ins + regs]
+    wires = []
+    for w in range(n_wires):
+        # mostly recent values (depth), sometimes anything earlier
+        def pick():
+            if rng.random() < 0.6 and len(pool) > 8:
+                return pool[-1 - rng.randrange(min(len(pool), 24))]
+            return rng.choice(pool)
+        a, b = pick(), pick()
+        k = rng.randrange(8)
+        e = [a & b, a | b, a ^ b, ~(a & b), ~(a | b), ~(a ^ b), ~a, mux(pick(), a, b)][k]
+        s = m.wire(f"w{w}", 1)
+        m.assign(s, e)
+        wires.append(s)
+        pool.append(s.value())
+    for i, q in enumerate(regs):
+        r = rng.random()
+        if r < 0.15 and i > 0:
+            m.next(q, regs[rng.randrange(n_regs)].value())   # shift chain (possibly itself)
+        elif r < 0.55:
+            m.next(q, wires[rng.randrange(n_wires // 4)].value())   # early next value
+        else:
+            m.next(q, wires[n_wires - 1 - rng.randrange(n_wires // 4)].value())   # late next value
+        kind = "async" if async_resets and rng.random() < 0.3 else "sync"
+        r = rng.random()
+        if r < 0.3:
+            m.reset(q, rst, rng.randrange(2), kind, "high")
+        elif r < 0.5:
+            m.reset(q, rstn, rng.randrange(2), kind, "low")
+    for j in range(12):
+        o = m.output(f"o{j}", 1)
+        src = regs[rng.randrange(n_regs)] if observe_regs and j % 3 == 0 else wires[n_wires - 1 - rng.randrange(40)]
+        m.assign(o, src.value())
+    return m
