@@ -13,6 +13,24 @@ constexpr uint32_t NONE = 0xFFFFFFFFu;
 
 // record ops: machine ops keep their numbers, effects follow
 enum : uint32_t { R_STORE = 32, R_ASYNC_CAPTURE, R_CAPTURE, R_COMMIT, R_MEMWRITE, R_SIGLOAD };
+// Fused one-limb records (the fusion menu of the reference's interpreter,
+// crates/rrtl-gpu-sim/src/interp.rs:351-495): an inner operation whose value
+// has no other reader folded into its reader, operands a, b, c in w[2..4].
+// Results are masked to the record's width like every one-limb record.
+enum : uint32_t {
+  FOP_NAND = OP_COUNT,  // ~(a & b)
+  FOP_NOR,              // ~(a | b)
+  FOP_XNOR,             // ~(a ^ b)
+  FOP_ANDOR,            // (a & b) | c
+  FOP_ORAND,            // (a | b) & c
+  FOP_XOR3,             // a ^ b ^ c
+  FOP_ADD3,             // a + b + c
+  FOP_MULADD,           // a * b + c
+  FOP_AOI21,            // ~((a & b) | c)
+  FOP_OAI21,            // ~((a | b) & c)
+  FOP_END
+};
+static_assert(FOP_END <= R_STORE, "fused opcodes must stay below the effect records");
 
 struct Rec { uint32_t w[8]; };  // op|limbs<<8, dst, a, b, c, width, aux0, aux1
 
@@ -146,6 +164,16 @@ __device__ __forceinline__ bool exec_rec1(const Cx& C, const Rec& R) {
   uint32_t v;
   switch (op) {
     case OP_NOT: v = ~C.ld(R.w[2], 0); break;
+    case FOP_NAND: v = ~(C.ld(R.w[2], 0) & C.ld(R.w[3], 0)); break;
+    case FOP_NOR: v = ~(C.ld(R.w[2], 0) | C.ld(R.w[3], 0)); break;
+    case FOP_XNOR: v = ~(C.ld(R.w[2], 0) ^ C.ld(R.w[3], 0)); break;
+    case FOP_ANDOR: v = (C.ld(R.w[2], 0) & C.ld(R.w[3], 0)) | C.ld(R.w[4], 0); break;
+    case FOP_ORAND: v = (C.ld(R.w[2], 0) | C.ld(R.w[3], 0)) & C.ld(R.w[4], 0); break;
+    case FOP_XOR3: v = C.ld(R.w[2], 0) ^ C.ld(R.w[3], 0) ^ C.ld(R.w[4], 0); break;
+    case FOP_ADD3: v = C.ld(R.w[2], 0) + C.ld(R.w[3], 0) + C.ld(R.w[4], 0); break;
+    case FOP_MULADD: v = C.ld(R.w[2], 0) * C.ld(R.w[3], 0) + C.ld(R.w[4], 0); break;
+    case FOP_AOI21: v = ~((C.ld(R.w[2], 0) & C.ld(R.w[3], 0)) | C.ld(R.w[4], 0)); break;
+    case FOP_OAI21: v = ~((C.ld(R.w[2], 0) | C.ld(R.w[3], 0)) & C.ld(R.w[4], 0)); break;
     case OP_AND: v = C.ld(R.w[2], 0) & C.ld(R.w[3], 0); break;
     case OP_OR: v = C.ld(R.w[2], 0) | C.ld(R.w[3], 0); break;
     case OP_XOR: v = C.ld(R.w[2], 0) ^ C.ld(R.w[3], 0); break;

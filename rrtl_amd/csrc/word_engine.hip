@@ -79,9 +79,86 @@ class WordCompiler {
     return at;
   }
 
+  // Fusion plan of a block (the reference interpreter's menu, interp.rs:393-495,
+  // Not-rooted cells first): a one-limb AND/OR/XOR/ADD/MUL (or an already fused
+  // ANDOR/ORAND under a NOT) whose value has exactly one reader, an operation
+  // of the same width, is folded into that reader's record and gets no slot.
+  // Equal widths keep the result exact: the dropped inner mask only clears
+  // bits the reader's own mask clears, and carries and products only move
+  // bits upward.
+  struct Fused { uint32_t fop, a, b, c; };
+  void plan_fusion(const Block& bk, std::vector<uint8_t>& absorbed, std::unordered_map<uint32_t, Fused>& fused) {
+    std::vector<const Instr*> def(bk.n_values, nullptr);
+    std::vector<uint32_t> uses(bk.n_values, 0);
+    for (const Packet& pk : bk.packets) {
+      for (const Instr& in : pk.instrs) {
+        def[in.dst] = &in;
+        if (in.op == OP_LIT || in.op == OP_SIGNAL) continue;
+        if (in.op == OP_MEMREAD) { uses[in.args[1]]++; continue; }
+        const size_t na = in.op == OP_SLICE ? 1 : in.op == OP_LT ? 2 : in.args.size();
+        for (size_t k = 0; k < na; k++) uses[in.args[k]]++;
+      }
+      for (const Effect& ef : pk.effects) {
+        if (ef.kind == EF_MEMWRITE) { uses[ef.enable]++; uses[ef.addr]++; uses[ef.data]++; }
+        else uses[ef.value]++;
+      }
+    }
+    absorbed.assign(bk.n_values, 0);
+    auto one_limb = [&](const Instr& in) { return limbs_of(in.width) == 1; };
+    // a plain inner value of operation `op` that `u` may absorb
+    auto plain = [&](uint32_t v, uint32_t op, const Instr& u) {
+      const Instr* d = def[v];
+      return d && d->op == op && uses[v] == 1 && !absorbed[v] && !fused.count(v) && d->width == u.width && one_limb(*d);
+    };
+    for (const Packet& pk : bk.packets)
+      for (const Instr& u : pk.instrs) {
+        if (!one_limb(u)) continue;
+        const uint32_t d = u.dst;
+        if (u.op == OP_NOT) {
+          const uint32_t x = u.args[0];
+          auto it = fused.find(x);
+          if (it != fused.end() && uses[x] == 1 && def[x]->width == u.width &&
+              (it->second.fop == FOP_ANDOR || it->second.fop == FOP_ORAND)) {
+            fused[d] = {it->second.fop == FOP_ANDOR ? uint32_t(FOP_AOI21) : uint32_t(FOP_OAI21), it->second.a,
+                        it->second.b, it->second.c};
+            fused.erase(it);
+            absorbed[x] = 1;
+            continue;
+          }
+          for (uint32_t op : {uint32_t(OP_AND), uint32_t(OP_OR), uint32_t(OP_XOR)})
+            if (plain(x, op, u)) {
+              fused[d] = {op == OP_AND ? uint32_t(FOP_NAND) : op == OP_OR ? uint32_t(FOP_NOR) : uint32_t(FOP_XNOR),
+                          def[x]->args[0], def[x]->args[1], NONE};
+              absorbed[x] = 1;
+              break;
+            }
+          continue;
+        }
+        // (reader, inner) pairs: OR(AND), AND(OR), XOR(XOR), ADD(ADD), ADD(MUL)
+        static const uint32_t pairs[5][3] = {{OP_OR, OP_AND, FOP_ANDOR}, {OP_AND, OP_OR, FOP_ORAND},
+                                             {OP_XOR, OP_XOR, FOP_XOR3}, {OP_ADD, OP_ADD, FOP_ADD3},
+                                             {OP_ADD, OP_MUL, FOP_MULADD}};
+        bool done = false;
+        for (const auto& pr : pairs) {
+          if (done || u.op != pr[0]) continue;
+          for (int k = 0; k < 2 && !done; k++) {
+            const uint32_t x = u.args[k], other = u.args[1 - k];
+            if (x == other || !plain(x, pr[1], u)) continue;
+            fused[d] = {pr[2], def[x]->args[0], def[x]->args[1], other};
+            absorbed[x] = 1;
+            done = true;
+          }
+        }
+      }
+  }
+
   void lower(int b) {
     const Block& bk = P.blocks[b];
     std::vector<Operand> val(bk.n_values, {NONE, 0, 0});
+    std::vector<uint8_t> absorbed;
+    std::unordered_map<uint32_t, Fused> fused;
+    if (fuse_) plan_fusion(bk, absorbed, fused);
+    else absorbed.assign(bk.n_values, 0);
     // liveness: last record index that reads each value
     std::vector<int64_t> last_use(bk.n_values, -1);
     int64_t pos = 0;
@@ -95,6 +172,10 @@ class WordCompiler {
           last_use[a] = pos;
           if (in.op == OP_MEMREAD) break;
         }
+        // a fused record reads its inner operation's operands
+        auto fz = fused.find(in.dst);
+        if (fz != fused.end())
+          for (uint32_t a : {fz->second.a, fz->second.b, fz->second.c}) if (a != NONE) last_use[a] = pos;
         pos++;
       }
       for (const Effect& ef : pk.effects) {
@@ -140,10 +221,25 @@ class WordCompiler {
           val[in.dst] = {SP_SCRATCH | slot, L, in.width};
           pos++; continue;
         }
+        if (absorbed[in.dst]) { pos++; continue; }  // folded into its one reader's record
         Rec r{};
         r.w[0] = in.op | (L << 8);
         r.w[5] = in.width;
         auto opnd = [&](uint32_t v) -> const Operand& { return val[v]; };
+        auto fz = fused.find(in.dst);
+        if (fz != fused.end()) {
+          const Fused& f = fz->second;
+          r.w[0] = f.fop | (1u << 8);
+          r.w[2] = opnd(f.a).code; r.w[3] = opnd(f.b).code; r.w[4] = f.c == NONE ? 0u : opnd(f.c).code;
+          r.w[7] = opnd(f.a).limbs | (opnd(f.b).limbs << 8);
+          r.w[6] = std::max(opnd(f.a).width, opnd(f.b).width);
+          uint32_t slot = alloc(L, pos, last_use[in.dst]);
+          r.w[1] = slot;
+          recs.push_back(r);
+          val[in.dst] = {SP_SCRATCH | slot, L, in.width};
+          pos++;
+          continue;
+        }
         switch (in.op) {
           case OP_NOT: case OP_ZEXT: case OP_TRUNC: case OP_CAST: case OP_SEXT:
             r.w[2] = opnd(in.args[0]).code; r.w[7] = opnd(in.args[0]).limbs; r.w[6] = opnd(in.args[0]).width; break;
@@ -219,6 +315,7 @@ class WordCompiler {
 
   const Program& P;
   WordProgram out;
+  bool fuse_ = true;  // fused records (plan_fusion)
   std::map<std::pair<uint32_t, std::vector<uint32_t>>, uint32_t> cmemo_;
   std::vector<Rec> commits_;
   uint32_t capture_limbs_ = 0, max_slot_ = 0;
@@ -376,6 +473,26 @@ class WordJit {
     const uint32_t op = R.w[0] & 0xFF;
     switch (op) {
       case OP_NOT: return "~" + opnd(R.w[2], 0);
+      case FOP_NAND: case FOP_NOR: case FOP_XNOR: case FOP_ANDOR: case FOP_ORAND: case FOP_XOR3: case FOP_ADD3:
+      case FOP_MULADD: case FOP_AOI21: case FOP_OAI21: {
+        const std::string a = opnd(R.w[2], 0), b = opnd(R.w[3], 0);
+        switch (op) {
+          case FOP_NAND: return "~(" + a + " & " + b + ")";
+          case FOP_NOR: return "~(" + a + " | " + b + ")";
+          case FOP_XNOR: return "~(" + a + " ^ " + b + ")";
+          default: break;
+        }
+        const std::string c = opnd(R.w[4], 0);
+        switch (op) {
+          case FOP_ANDOR: return "((" + a + " & " + b + ") | " + c + ")";
+          case FOP_ORAND: return "((" + a + " | " + b + ") & " + c + ")";
+          case FOP_XOR3: return "(" + a + " ^ " + b + " ^ " + c + ")";
+          case FOP_ADD3: return "(" + a + " + " + b + " + " + c + ")";
+          case FOP_MULADD: return "(" + a + " * " + b + " + " + c + ")";
+          case FOP_AOI21: return "~((" + a + " & " + b + ") | " + c + ")";
+          default: return "~((" + a + " | " + b + ") & " + c + ")";
+        }
+      }
       case OP_AND: case OP_OR: case OP_XOR: case OP_ADD: case OP_SUB: case OP_MUL: {
         static const char* sym[] = {"&", "|", "^", "+", "-", "*"};
         const std::string a = opnd(R.w[2], 0), b = opnd(R.w[3], 0);
