@@ -154,3 +154,51 @@ def random_gate_design(seed: int, n_inputs: int = 24, n_regs: int = 40, n_wires:
         src = regs[rng.randrange(n_regs)] if observe_regs and j % 3 == 0 else wires[n_wires - 1 - rng.randrange(40)]
         m.assign(o, src.value())
     return m
+
+
+def fusion_design(width: int = 8) -> ir.Module:
+    """Every fusable pattern of the word engine's record encoder at one width
+    (NOT over AND/OR/XOR, over OR-of-AND and AND-of-OR; OR(AND), AND(OR),
+    XOR(XOR), ADD(ADD), ADD(MUL) with the inner value on either side), each on
+    inputs of its own so that the lowering's shared subexpressions do not give
+    an inner value a second reader, plus cases that must stay unfused: an inner
+    value with two readers, an inner value of another width and a stored inner
+    value."""
+    m = ir.Module(f"Fuse{width}")
+    clk = m.input("clk", 1)
+    k = [0]
+
+    def trio():
+        k[0] += 1
+        return tuple(m.input(f"{n}{k[0]}", width).value() for n in "abc")
+
+    exprs = {}
+    A, B, _ = trio(); exprs["nand"] = ~(A & B)
+    A, B, _ = trio(); exprs["nor"] = ~(A | B)
+    A, B, _ = trio(); exprs["xnor"] = ~(A ^ B)
+    A, B, C = trio(); exprs["aoi"] = ~((A & B) | C)
+    A, B, C = trio(); exprs["oai"] = ~((A | B) & C)
+    A, B, C = trio(); exprs["andor"] = (A & B) | C
+    A, B, C = trio(); exprs["orand"] = C & (A | B)
+    A, B, C = trio(); exprs["xor3"] = C ^ (A ^ B)
+    A, B, C = trio(); exprs["add3"] = (A + B) + C
+    A, B, C = trio(); exprs["muladd"] = C + (A * B)
+    A, B, C = trio()
+    shared = A & B
+    exprs["shared1"] = shared | C
+    exprs["shared2"] = shared ^ C
+    A, B, C = trio()
+    exprs["widthchange"] = (A.zext(width + 1) & B.zext(width + 1)).trunc(width) | C
+    A, B, C = trio()
+    stored = m.wire("stored", width)
+    m.assign(stored, A + B)
+    exprs["storedsum"] = stored.value() + C
+    A, B, _ = trio()
+    r = m.reg("acc", width)
+    m.clock(r, clk)
+    m.next(r, (r.value() * A) + B)  # a register's next value through MULADD
+    exprs["acc"] = r.value()
+    for name, e in exprs.items():
+        o = m.output(f"o_{name}", width)
+        m.assign(o, e)
+    return m

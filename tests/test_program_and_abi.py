@@ -179,3 +179,19 @@ def test_autotune_recommendation_lookup(tmp_path):
     assert options_from_recommendations(back, "ctr", 4096, 1000)["block_threads"] == 128
     with pytest.raises(KeyError):
         options_from_recommendations(back, "nope", 1, 1)
+
+
+def test_fused_records_in_generated_source():
+    """The record encoder folds single-reader inner operations (the reference
+    interpreter's fusion menu): the generated kernel source holds the fused
+    expressions and fewer value definitions than the design has operations."""
+    import re
+    from designgen import fusion_design
+    from rrtl_amd import backend
+    src = backend.emit_kernel(lower.lower_module(fusion_design(8)))
+    assert re.search(r"~\(\(\w+ & \w+\) \| \w+\)", src), "AOI21"
+    assert re.search(r"~\(\(\w+ \| \w+\) & \w+\)", src), "OAI21"
+    assert re.search(r"\(\w+ \^ \w+ \^ \w+\)", src), "XOR3"
+    assert re.search(r"\(\w+ \+ \w+ \+ \w+\)", src), "ADD3"
+    assert re.search(r"\(\w+ \* \w+ \+ \w+\)", src), "MULADD"
+    assert re.search(r"~\(\w+ & \w+\)", src), "NAND"
