@@ -5,10 +5,6 @@
 #include "common.hpp"
 #include <type_traits>
 
-#ifndef RRTL_STAGE_AHEAD
-#define RRTL_STAGE_AHEAD 2
-#endif
-
 namespace rrtl {
 namespace gate {
 
@@ -23,7 +19,7 @@ constexpr uint32_t CLAMP_COMMIT = 0xFFFFFFFEu;
 constexpr uint32_t TILE_THREADS = 256;
 // Waves per SIMD the tile kernels are register-budgeted for.  The one-pass
 // kernel runs when every tile is resident at once, which for the 20k-gate
-// netlist (39.5 KB of LDS per 64-lane tile) means four workgroups per CU, one
+// netlist (about 40 KB of LDS per 64-lane tile) means four workgroups per CU, one
 // wave of each on every SIMD: at most 128 VGPRs.  The work-queue kernel runs
 // when tiles outnumber the resident slots, typically at three tiles per CU: at
 // most 168 VGPRs.
@@ -306,7 +302,7 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, const 
   __syncthreads();
 }
 
-constexpr uint32_t STAGE_AHEAD = RRTL_STAGE_AHEAD;     // stages of descriptor look-ahead
+constexpr uint32_t STAGE_AHEAD = 2;  // stages of descriptor look-ahead (3 or 4 measured no faster)
 constexpr uint32_t SINK_SLOTS = 64;     // per-tile slots written by lanes with no output to keep
 // batch job variant: tile words W | descriptors hold byte offsets | positions per stage M << 12
 constexpr uint32_t GATE_VARIANT_BYTE_ADDR = 0x100;
