@@ -264,3 +264,23 @@ def test_random_gate_design_trace_matches_oracle(seed, observe_regs, async_reset
         for r in [i for i, s in enumerate(p.signals) if s.kind == KIND_REG]:
             assert np.array_equal(stim.u32_rows_to_u64(sim.get_signal_bits(r))[: ref.words], ref.get_words(r)), \
                 (run, p.signals[r].name)
+
+
+def test_register_held_tables_overflow_to_memory():
+    """More inputs than threads (300 > 256), more observed signals than wave 0
+    holds in registers (300 > 256) and more reset-free commits than the
+    threads hold (2,600 > 2,048): the overflow goes through the tables in
+    memory, with the same results as the oracle."""
+    flush_print("case: register-held table overflow (300 inputs, 300 outputs, 2600 flops)")
+    p = small_netlist_program(gates=6000, levels=16, dffs=2600, inputs=300, outputs=300, seed=9)
+    lanes, cycles, seed = 640, 20, 23
+    ins, obs = netlist_inputs(p), netlist_outputs(p)
+    sim = HipSimulator(p, lanes, engine="gate", tile_words=2)
+    sim.run(cycles, seed=seed, inputs=ins, observe=obs, trace=True)
+    ref = oracle.BitParallelRef(p, lanes)
+    dg, tr = ref.run(seed=seed, cycles=cycles, inputs=ins, observe=obs, digest=True, trace=True)
+    got = stim.u32_rows_to_u64(sim.trace().reshape(cycles, len(obs), -1))
+    assert np.array_equal(got, tr)
+    assert np.array_equal(stim.u32_rows_to_u64(sim.digest()), dg)
+    for r in [i for i, s in enumerate(p.signals) if s.kind == KIND_REG][::13]:
+        assert np.array_equal(stim.u32_rows_to_u64(sim.get_signal_bits(r))[: ref.words], ref.get_words(r)), p.signals[r].name
