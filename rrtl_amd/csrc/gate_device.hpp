@@ -158,6 +158,12 @@ __device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D)
   va = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[0].x & 0xFFFFu));
 #pragma unroll
   for (int k = 0; k < M; k++) vb[k] = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[k].x >> 16));
+  // Once its reads are issued, a wave runs at raised issue priority until the
+  // stage barrier (stage_step drops it): waves holding their operands finish
+  // their writes and reach the barrier first, so the tile that is furthest
+  // along releases its barrier sooner (+2.6 % on the 20k-gate netlist; raising
+  // it for the reads instead lost 3 %).
+  __builtin_amdgcn_s_setprio(2);
   Words<W> prev;
 #pragma unroll
   for (int k = 0; k < M; k++) {
@@ -445,6 +451,7 @@ __device__ __forceinline__ void stage_step(const TileArgs& A, uint32_t* lds, uin
   eval_stage<W, kBA, M, kFix>(lds, D);
   load_stage<M>(D, A.desc + size_t(S + STAGE_AHEAD) * (uint32_t(M) * TILE_THREADS));
   __syncthreads();
+  __builtin_amdgcn_s_setprio(0);  // raised in eval_stage after the reads
   if (kClamp) { apply_clamps<W>(A, lds, S + 1, fb, fe); __syncthreads(); }
 }
 
