@@ -160,10 +160,9 @@ __device__ __forceinline__ void eval_stage(uint32_t* lds, const StageDesc<M>& D)
   for (int k = 0; k < M; k++) vb[k] = ld_words_p<W>(lds_at<W, kBA>(lds, D.d[k].x >> 16));
   // Once its reads are issued, a wave runs at raised issue priority until the
   // stage barrier (stage_step drops it): waves holding their operands finish
-  // their writes and reach the barrier first, so the tile that is furthest
-  // along releases its barrier sooner (+2.6 % on the 20k-gate netlist; raising
-  // it for the reads instead lost 3 %).
-  __builtin_amdgcn_s_setprio(2);
+  // their writes and reach the barrier first (20k-gate netlist: +4 % at
+  // priority 3, +2.6 % at 2; raising it for the reads instead lost 3 %).
+  __builtin_amdgcn_s_setprio(3);
   Words<W> prev;
 #pragma unroll
   for (int k = 0; k < M; k++) {
@@ -540,8 +539,10 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
       if (A.n_obs) observe_values<W>(A, lds, OR, sreg, c, cyc, w0, det);
       if constexpr (kBA) commit_fast<W>(lds, F);
       commit_general<W>(A, lds);
+      __builtin_amdgcn_s_setprio(3);  // as in the stages: reads issued, on to the barrier
       if (c + 1 < A.cycles) drive_inputs<W>(A, lds, SR, cyc + 1, w0, gkey);
       __syncthreads();
+      __builtin_amdgcn_s_setprio(0);
       if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
     } else {
       if (!A.observe_post && A.n_obs) observe<W>(A, lds, OR, sreg, c, cyc, w0, det);
