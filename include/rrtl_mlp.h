@@ -99,6 +99,21 @@ rrtl_status rrtl_mlp_last_stats(const rrtl_mlp* m, rrtl_mlp_stats* out);
 rrtl_status rrtl_mlp_gemm_nt(rrtl_mlp* m, const uint16_t* a, const uint16_t* b, uint32_t M, uint32_t N, uint32_t K,
                              uint32_t tile, float* c, uint32_t iters, double* ms_per_launch);
 
+/* The same product on caller-owned DEVICE buffers, allocating nothing: no
+ * handle, no hipMalloc/hipFree, no allocating library call.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  a_dev [M][K] and b_dev
+ * [N][K] bf16 patterns, c_dev [M][N] fp32, shapes and `tile` as in
+ * rrtl_mlp_gemm_nt, whose kernels and summation order it shares (results are
+ * identical).  Enqueued asynchronously on `stream` (a hipStream_t; null = the
+ * default stream).  Split-K partial products go to a preallocated 32 MiB
+ * region per GPU (held until the process ends); a product needing more is
+ * rejected with RRTL_E_INVALID ("E_MLP_NOALLOC").  Calls that overlap on one
+ * GPU share that region and race.  On failure the message is in
+ * rrtl_mlp_create_error() (thread-local). */
+rrtl_status rrtl_mlp_gemm_nt_noalloc(const uint16_t* a_dev, const uint16_t* b_dev, uint32_t M, uint32_t N, uint32_t K,
+                                     uint32_t tile, float* c_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 namespace rrtl {
 namespace mlp {
@@ -348,6 +349,103 @@ __global__ void __launch_bounds__(256) sigmoid_kernel(const float* l, float* p, 
   if (i < n) p[i] = 1.f / (1.f + expf(-l[i]));
 }
 
+// ------------------------------------------------------------------ GEMM launch
+// Workgroup tile and K split of one product.  256 x 128 tiles when there are
+// enough of them to fill the device; fp32 products split K (powers of two)
+// while the tiles alone cannot occupy the CUs and every slice keeps at least
+// four K steps.  Shared by the network's own products and the stand-alone
+// entry points, so both run the same kernels in the same summation order.
+struct GemmPlan {
+  bool wide;
+  uint32_t tiles, splits;
+  size_t partial_floats(uint32_t M, uint32_t N) const { return splits > 1 ? size_t(splits) * M * N : 0; }
+};
+
+GemmPlan plan_gemm(int epi, uint32_t M, uint32_t N, uint32_t K, uint32_t tile) {
+  const bool wide = tile ? tile == 2 : (M % 256 == 0 && uint64_t(M / 256) * (N / 128) >= 1024);
+  if (wide && M % 256) throw Error("E_MLP_GEMM: 256-row tiles need M to be a multiple of 256");
+  const uint32_t tiles = wide ? (M / 256) * (N / 128) : (M / 128) * (N / 128);
+  uint32_t splits = 1;
+  if (epi == EPI_F32)
+    while (tiles * splits < 512 && K / (splits * 2) >= 4 * BK && (K / (splits * 2)) % BK == 0) splits *= 2;
+  return GemmPlan{wide, tiles, splits};
+}
+
+// Launches the product on stream s.  With plan.splits > 1 every y-slice writes
+// its whole M x N partial product into `partials` (splits x M x N floats,
+// dense) and a reduction sums them into g.out_f, which must then be dense.
+void launch_gemm(int epi, const GemmPlan& plan, GemmArgs g, float* partials, hipStream_t s) {
+  float* const final_out = g.out_f;
+  if (plan.splits > 1) {
+    if (g.ldf != g.N) throw Error("E_MLP_INTERNAL: split-K output must be dense");
+    g.out_f = partials; g.ldf = g.N; g.split_stride = size_t(g.M) * g.N;
+  }
+  const dim3 grid(plan.tiles, plan.splits);
+  if (plan.wide) {
+    if (epi == EPI_RELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_RELU, 4, 2>), grid, dim3(GT), 0, s, g);
+    else if (epi == EPI_DRELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_DRELU, 4, 2>), grid, dim3(GT), 0, s, g);
+    else hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 4, 2>), grid, dim3(GT), 0, s, g);
+  } else {
+    if (epi == EPI_RELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_RELU, 2, 2>), grid, dim3(GT), 0, s, g);
+    else if (epi == EPI_DRELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_DRELU, 2, 2>), grid, dim3(GT), 0, s, g);
+    else hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 2, 2>), grid, dim3(GT), 0, s, g);
+  }
+  RRTL_HIP_CHECK(hipGetLastError());
+  if (plan.splits > 1) {
+    const size_t n = size_t(g.M) * g.N;
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(uint32_t(std::min<size_t>((n / 4 + 255) / 256, 2048))), dim3(256), 0,
+                       s, partials, n, plan.splits, final_out, n);
+    RRTL_HIP_CHECK(hipGetLastError());
+  }
+}
+
+// ------------------------------------------------------------------ no-allocation GEMM
+// Split-K partial products of rrtl_mlp_gemm_nt_noalloc, for callers that
+// cannot allocate device memory (code under graph capture, serving loops).
+// Every GPU the library runs on holds its own copy until the process ends.
+// Each call overwrites the whole region it uses before reading it, so nothing
+// needs clearing; calls that overlap on one GPU race on it.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) float scratch[kNoAllocScratchBytes / sizeof(float)];
+
+constexpr int kMaxDevices = 64;
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current (host code must never use the array's name
+// as an address).
+float* noalloc_scratch() {
+  static std::mutex mu;
+  static float* ptr[kMaxDevices] = {};
+  int dev = 0;
+  RRTL_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error("E_MLP_NOALLOC: device ordinal beyond the supported 64 devices");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(std::string("E_HIP_RUNTIME: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptr[dev] = static_cast<float*>(p);
+  }
+  return ptr[dev];
+}
+
+void gemm_nt_noalloc(const u16* a, const u16* b, uint32_t M, uint32_t N, uint32_t K, uint32_t tile, float* c,
+                     hipStream_t s) {
+  if (!a || !b || !c) throw Error("E_HIP_NULL: null device buffer");
+  if (tile > 2) throw Error("E_MLP_GEMM: tile must be 0 (automatic), 1 (128 x 128) or 2 (256 x 128)");
+  if (!M || !N || !K || M % 128 || N % 128 || K % BK) throw Error("E_MLP_GEMM: M and N must be multiples of 128, K of 64");
+  const GemmPlan plan = plan_gemm(EPI_F32, M, N, K, tile);
+  const size_t need = plan.partial_floats(M, N) * sizeof(float);
+  if (need > kNoAllocScratchBytes)
+    throw Error("E_MLP_NOALLOC: this product splits K " + std::to_string(plan.splits) + " ways and needs " +
+                std::to_string(need) + " bytes of partial products; the preallocated scratch holds " +
+                std::to_string(kNoAllocScratchBytes) + " (use rrtl_mlp_gemm_nt)");
+  float* partials = plan.splits > 1 ? noalloc_scratch() : nullptr;
+  GemmArgs g{a, b, M, N, K, K, K, nullptr, nullptr, nullptr, 0, nullptr, 0, c, N, K / plan.splits, 0};
+  launch_gemm(EPI_F32, plan, g, partials, s);
+}
+
 // ------------------------------------------------------------------ host
 template <class T>
 struct Buf {
@@ -614,20 +712,9 @@ class Mlp {
             const float* bias, const u16* act, u16* out, uint32_t ldo, u16* out_t, uint32_t ldt, float* out_f,
             uint32_t ldf, uint32_t tile = 0) {
     if (M % 128 || N % 128 || K % BK || lda % 8 || ldb % 8) throw Error("E_MLP_INTERNAL: GEMM shape");
-    // 256 x 128 tiles when there are enough of them to fill the device
-    const bool wide = tile ? tile == 2 : (M % 256 == 0 && uint64_t(M / 256) * (N / 128) >= 1024);
-    if (wide && M % 256) throw Error("E_MLP_GEMM: 256-row tiles need M to be a multiple of 256");
-    const uint32_t tiles = wide ? (M / 256) * (N / 128) : (M / 128) * (N / 128);
-    // split K when the tiles alone cannot occupy the CUs (fp32 products only)
-    uint32_t splits = 1;
-    if (epi == EPI_F32)
-      while (tiles * splits < 512 && K / (splits * 2) >= 4 * BK && (K / (splits * 2)) % BK == 0) splits *= 2;
-    GemmArgs g{A, B, M, N, K, lda, ldb, bias, act, out, ldo, out_t, ldt, out_f, ldf, K / splits, 0};
-    float* final_out = out_f;
-    if (splits > 1) {
-      gsplit_.alloc(size_t(splits) * M * N);
-      g.out_f = gsplit_.p; g.ldf = N; g.split_stride = size_t(M) * N;
-    }
+    const GemmPlan plan = plan_gemm(epi, M, N, K, tile);
+    GemmArgs g{A, B, M, N, K, lda, ldb, bias, act, out, ldo, out_t, ldt, out_f, ldf, K / plan.splits, 0};
+    if (plan.splits > 1) gsplit_.alloc(plan.partial_floats(M, N));
     if (n_gemm_ == gemm_ev_.size()) {
       hipEvent_t a, b;
       RRTL_HIP_CHECK(hipEventCreate(&a));
@@ -635,25 +722,8 @@ class Mlp {
       gemm_ev_.push_back({a, b});
     }
     auto& ev = gemm_ev_[n_gemm_++];
-    const dim3 grid(tiles, splits);
     RRTL_HIP_CHECK(hipEventRecord(ev.first, s_));
-    if (wide) {
-      if (epi == EPI_RELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_RELU, 4, 2>), grid, dim3(GT), 0, s_, g);
-      else if (epi == EPI_DRELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_DRELU, 4, 2>), grid, dim3(GT), 0, s_, g);
-      else hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 4, 2>), grid, dim3(GT), 0, s_, g);
-    } else {
-      if (epi == EPI_RELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_RELU, 2, 2>), grid, dim3(GT), 0, s_, g);
-      else if (epi == EPI_DRELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_DRELU, 2, 2>), grid, dim3(GT), 0, s_, g);
-      else hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 2, 2>), grid, dim3(GT), 0, s_, g);
-    }
-    RRTL_HIP_CHECK(hipGetLastError());
-    if (splits > 1) {
-      if (ldf != N) throw Error("E_MLP_INTERNAL: split-K output must be dense");
-      const size_t n = size_t(M) * N;
-      hipLaunchKernelGGL(splitk_reduce_kernel, dim3(uint32_t(std::min<size_t>((n / 4 + 255) / 256, 2048))), dim3(256), 0,
-                         s_, gsplit_.p, n, splits, final_out, n);
-      RRTL_HIP_CHECK(hipGetLastError());
-    }
+    launch_gemm(epi, plan, g, plan.splits > 1 ? gsplit_.p : nullptr, s_);
     RRTL_HIP_CHECK(hipEventRecord(ev.second, s_));
     stats_.gemm_flop += 2.0 * M * N * K;
   }
@@ -838,6 +908,16 @@ rrtl_status rrtl_mlp_gemm_nt(rrtl_mlp* m, const uint16_t* a, const uint16_t* b, 
     const double ms = x.gemm_nt(a, b, M, N, K, c, iters, tile);
     if (ms_per_launch) *ms_per_launch = ms;
   });
+}
+rrtl_status rrtl_mlp_gemm_nt_noalloc(const uint16_t* a_dev, const uint16_t* b_dev, uint32_t M, uint32_t N, uint32_t K,
+                                     uint32_t tile, float* c_dev, void* stream) {
+  try {
+    rrtl::mlp::gemm_nt_noalloc(a_dev, b_dev, M, N, K, tile, c_dev, static_cast<hipStream_t>(stream));
+    return RRTL_OK;
+  } catch (const std::exception& ex) {
+    g_mlp_create_err = ex.what();
+    return mlp_classify(g_mlp_create_err);
+  }
 }
 rrtl_status rrtl_mlp_last_stats(const rrtl_mlp* m, rrtl_mlp_stats* out) {
   if (!m || !m->impl || !out) return RRTL_E_INVALID;

@@ -30,7 +30,7 @@ class MlpStats(ctypes.Structure):
 EXPORTS = ["rrtl_mlp_create", "rrtl_mlp_destroy", "rrtl_mlp_last_error", "rrtl_mlp_create_error",
            "rrtl_mlp_param_count", "rrtl_mlp_get_params", "rrtl_mlp_set_params", "rrtl_mlp_forward_backward",
            "rrtl_mlp_get_grads", "rrtl_mlp_forward", "rrtl_mlp_set_tables", "rrtl_mlp_train_pairs",
-           "rrtl_mlp_predict_pairs", "rrtl_mlp_last_stats", "rrtl_mlp_gemm_nt"]
+           "rrtl_mlp_predict_pairs", "rrtl_mlp_last_stats", "rrtl_mlp_gemm_nt", "rrtl_mlp_gemm_nt_noalloc"]
 
 _bound = False
 
@@ -57,12 +57,26 @@ def _lib():
         L.rrtl_mlp_predict_pairs.argtypes = [vp, vp, vp, vp, u64, vp]
         L.rrtl_mlp_last_stats.argtypes = [vp, ctypes.POINTER(MlpStats)]
         L.rrtl_mlp_gemm_nt.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, u32, ctypes.POINTER(ctypes.c_double)]
+        L.rrtl_mlp_gemm_nt_noalloc.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp]
         _bound = True
     return L
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def gemm_nt_noalloc(a_dev: int, b_dev: int, M: int, N: int, K: int, c_dev: int, tile: int = 0,
+                    stream: int = 0) -> None:
+    """C = A . B^T on caller-owned device buffers (addresses as integers:
+    A [M][K] and B [N][K] bf16 patterns, C [M][N] fp32), enqueued on `stream`
+    (a hipStream_t as an integer, 0 = the default stream) without allocating
+    device memory; split-K partials use the library's preallocated region."""
+    L = _lib()
+    st = L.rrtl_mlp_gemm_nt_noalloc(ctypes.c_void_p(a_dev), ctypes.c_void_p(b_dev), M, N, K, tile,
+                                    ctypes.c_void_p(c_dev), ctypes.c_void_p(stream))
+    if st != 0:
+        raise HipError(st, L.rrtl_mlp_create_error().decode())
 
 
 class Surrogate:
