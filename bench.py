@@ -52,10 +52,12 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device with torch (scale reference for HBM counters)")
-    ap.add_argument("--workload", choices=("gates", "mixed", "rv32i", "faultsim"), default="gates",
+    ap.add_argument("--workload", choices=("gates", "mixed", "rv32i", "rv32i-core", "faultsim"), default="gates",
                     help="gates: the headline line; mixed: the four-design batch, persistent launch vs one "
-                         "launch per design; rv32i: per-lane random RV32I programs on the word engine; "
-                         "faultsim: surrogate-triaged stuck-at fault simulation on the 20k-gate netlist")
+                         "launch per design; rv32i: picorv32 (imported from the reference's Verilog) running "
+                         "per-lane random RV32I programs on the word engine; rv32i-core: the same programs on this "
+                         "package's single-cycle RV32I core; faultsim: surrogate-triaged stuck-at fault "
+                         "simulation on the 20k-gate netlist")
     ap.add_argument("--epochs", type=int, default=6, help="faultsim: surrogate training epochs")
     ap.add_argument("--no-autotune", action="store_true", help="mixed: default launch shapes instead of the autotuner's")
     ap.add_argument("--skip-persistent", action="store_true", help="mixed: leave out the persistent work-queue kernel")
@@ -68,7 +70,7 @@ def main() -> int:
         return bench_mixed(args)
     if args.workload == "faultsim":
         return bench_faultsim(args)
-    if args.workload == "rv32i":
+    if args.workload in ("rv32i", "rv32i-core"):
         return bench_rv32i(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -234,9 +236,16 @@ def bench_mixed(args) -> int:
 
 def bench_rv32i(args) -> int:
     """Per-lane random RV32I programs (512 instructions: ALU 60 %, load/store
-    25 %, branch 15 %), 16,384 lanes, halting on ecall, cycle cap --cycles."""
+    25 %, branch 15 %, in-range targets), 16,384 lanes, halting on ecall, cycle
+    cap --cycles, on picorv32 (the reference's core and SoC imported from its
+    Verilog) or, with --workload rv32i-core, on the single-cycle core."""
     from rrtl_amd import workloads
-    job = workloads.rv32i_job(16384, args.cycles, args.seed)
+    if args.workload == "rv32i":
+        job = workloads.picorv32_job(16384, args.cycles, args.seed)
+        design = "picorv32_soc (bench/sv/picorv32.v + picorv32_soc.v, imported)"
+    else:
+        job = workloads.rv32i_job(16384, args.cycles, args.seed)
+        design = "single-cycle rv32i core"
     sim = job.make(codegen=args.codegen)
     sim.run(**dict(job.run, cycles=1))   # compile the run plan
     sim = job.make(codegen=args.codegen)
@@ -262,7 +271,7 @@ def bench_rv32i(args) -> int:
             "value": value, "unit": "lane-cycles/s", "n_gpus": 1,
             "higher_is_better": True, "data": "synthetic (random RV32I programs)", "dtype": "u32",
             "wall_ms": wall * 1e3, "kernel_ms": st["kernel_ms"],
-            "config": {"workload": f"rv32i core, 16384 lanes, 512-instruction random programs, cap {args.cycles}",
+            "config": {"workload": f"{design}, 16384 lanes, 512-instruction random programs, cap {args.cycles}",
                        "codegen": args.codegen,
                        "halt_cycles_min": int(halt.min()), "halt_cycles_max": int(halt.max()),
                        "lanes_at_cap": int((halt >= args.cycles).sum())},
@@ -302,12 +311,13 @@ def rv32i_cpu_baseline(job, args) -> dict:
     eng, secs = cpu_run(cyc)
     gpu = job.make(codegen=args.codegen)
     gpu.run(cyc, seed=job.run["seed"], inputs=ins)
-    same = all(np.array_equal(gpu.get_memory_limbs(m), eng.get_memory_limbs(p.memory_index(m))) for m in ("regs", "mem"))
-    for name in ("pc", "trap"):
+    same = all(np.array_equal(gpu.get_memory_limbs(m), eng.get_memory_limbs(p.memory_index(m))) for m in job.check_mems)
+    for name in job.check_sigs:
         same &= bool(np.array_equal(gpu.get_signal_limbs(name), eng.get_signal_limbs(p.signal_index(name))))
     gpu.close()
     return {"value": job.lanes * cyc / secs, "unit": "lane-cycles/s", "cores": threads, "kind": "port",
-            "sample": f"{job.lanes} lanes x {cyc} cycles, every lane stepped ({secs:.1f} s)", "bit_exact_vs_gpu": same}
+            "sample": f"{job.lanes} lanes x {cyc} cycles, every lane stepped ({secs:.1f} s)", "bit_exact_vs_gpu": same,
+            "checked": list(job.check_mems + job.check_sigs)}
 
 
 def bench_faultsim(args) -> int:

@@ -66,15 +66,17 @@ typedef struct rrtl_hip_options {
    * GPU when the engine is created (the counterpart of the reference's
    * per-design code generator, crates/rrtl-sim-ir/src/gpu_codegen.rs:503);
    * RRTL_CODEGEN_OFF runs the record interpreter; RRTL_CODEGEN_AUTO generates
-   * code for designs of at most 4000 records and 128 live scratch words and
+   * code for designs of at most 4000 records (and, when some records stay
+   * interpreted in the generated code, at most 128 live scratch words) and
    * interprets larger ones.  Results are identical.  Read only when
    * struct_size covers it. */
   uint32_t codegen;
   /* Word engine: active lanes per 64-thread wave (4, 8, 16, 32 or 64).  Fewer
    * lanes per wave put more waves on each SIMD, which hides memory latency for
    * a batch too small to do so at 64 lanes per wave, at the price of issuing
-   * each instruction for fewer lanes.  0 = automatic (the largest of 64, 32,
-   * 16 that still gives four waves per CU, else 16).  Read only when
+   * each instruction for fewer lanes.  0 = automatic (programs under 600
+   * records: the largest of 64, 32, 16 that still gives four waves per CU,
+   * else 16; longer programs: 64).  Read only when
    * struct_size covers it. */
   uint32_t wave_lanes;
 } rrtl_hip_options;

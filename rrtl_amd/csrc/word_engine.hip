@@ -709,13 +709,15 @@ class WordEngine final : public Engine {
     if (lanes == 0) throw Error("E_SIM_IR_LANES: packed simulator requires at least one lane");
     WP = WordCompiler(P).compile();
     // Automatic code generation is for designs whose straight-line code stays
-    // small (compile time grows faster than linearly with its length) and whose
-    // values fit in registers; larger designs run the record interpreter.
+    // small (compile time grows faster than linearly with its length); larger
+    // designs run the record interpreter.  Generated code keeps its values in
+    // registers and needs the scratch slots only for records it leaves in
+    // interpreted form, so the scratch limit applies to those designs alone.
     size_t n_rec = 0;
     for (auto& b : WP.blk) n_rec += b.size();
-    const bool small = n_rec <= 4000 && WP.scratch_slots <= 128;
     std::unique_ptr<WordJit> jit;
-    if (opt.codegen == CODEGEN_ON || (opt.codegen == CODEGEN_AUTO && small)) jit.reset(new WordJit(WP));
+    if (opt.codegen == CODEGEN_ON || (opt.codegen == CODEGEN_AUTO && n_rec <= 4000)) jit.reset(new WordJit(WP));
+    if (jit && opt.codegen == CODEGEN_AUTO && jit->uses_scratch() && WP.scratch_slots > 128) jit.reset();
     run_slots_ = jit && !jit->uses_scratch() ? 0 : WP.scratch_slots;
     RRTL_HIP_CHECK(hipSetDevice(opt.device));
     hipDeviceProp_t prop;
@@ -727,15 +729,19 @@ class WordEngine final : public Engine {
     // One lane per thread: blocks of 256 lanes, smaller (down to one wave)
     // when that is what it takes to give every CU a block.  A batch too small
     // to give every SIMD (four per CU) a wave runs 32 or 16 lanes per wave
-    // instead: a wave's time per cycle is a dependent chain that barely
-    // depends on how many of its lanes are active, so more, thinner waves
-    // finish the batch sooner.
+    // instead when the per-cycle program is short: a wave's time per cycle is
+    // then a dependent chain of memory accesses that barely depends on how
+    // many of its lanes are active, so more, thinner waves finish the batch
+    // sooner.  Long programs (600 records or more, e.g. picorv32) are bound by
+    // the instructions each wave issues and keep full waves (measured: 3.5e9
+    // against 2.4e9 lane-cycles/s at 16 lanes per wave).
     const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
     if (opt.wave_lanes) {
       wave_lanes_ = opt.wave_lanes;  // 4, 8, 16, 32 or 64
     } else {
       wave_lanes_ = 64;
-      while (wave_lanes_ > 16 && (uint64_t(lanes) + wave_lanes_ - 1) / wave_lanes_ < 4ull * cus) wave_lanes_ /= 2;
+      while (n_rec < 600 && wave_lanes_ > 16 && (uint64_t(lanes) + wave_lanes_ - 1) / wave_lanes_ < 4ull * cus)
+        wave_lanes_ /= 2;
     }
     if (wave_lanes_ < 64) {
       block_ = 64;

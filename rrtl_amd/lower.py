@@ -27,61 +27,86 @@ from .program import (
     OP_SIGNAL, OP_SLICE, POL_HIGH, POL_LOW, RESET_ASYNC, RESET_SYNC, limbs, to_limbs,
 )
 
-# A packed expression is a nested tuple (op, width, signed, attr, *children) so that
-# structurally equal trees hash equal, as ``PackedExpr`` does under ``#[derive(Hash)]``.
-PExpr = tuple
+# A packed expression node is (op, width, signed, attr, *children) with its
+# children given as node ids: structurally equal trees get one id
+# (hash-consing), which is the structural equality ``PackedExpr`` derives
+# (``#[derive(Hash, Eq)]``) without re-walking shared sub-trees on every hash.
+PExpr = int
+
+
+class _Nodes:
+    def __init__(self):
+        self.ids: Dict[tuple, int] = {}
+        self.nodes: List[tuple] = []
+
+    def intern(self, node: tuple) -> int:
+        got = self.ids.get(node)
+        if got is None:
+            got = len(self.nodes)
+            self.ids[node] = got
+            self.nodes.append(node)
+        return got
+
 
 # packed ops: ("assign", dst, expr) | ("capture", dst, expr, PReset|None)
 #             | ("memwrite", mem, en, addr, data)
 
 
 def _lower_expr(e: ir.Expr, sig_index: Dict[ir.Signal, int],
-                mem_index: Dict[ir.Memory, int], cache: Dict[int, PExpr]) -> PExpr:
+                mem_index: Dict[ir.Memory, int], cache: Dict[int, PExpr], nodes: _Nodes) -> PExpr:
     """``Lowering::lower_raw_expr`` (lib.rs:12460-12560).  Each node keeps its own
     type; for validated designs this equals the type the reference threads down
     (rhs of a binary op takes the lhs type, a mux condition is uint(1))."""
-    key = id(e)
-    got = cache.get(key)
+    got = cache.get(id(e))
     if got is not None:
         return got
-    w, sg = e.ty.width, e.ty.signed
-    if e.op == "lit":
-        out = ("lit", w, sg, tuple(to_limbs(e.attr, w)))
-    elif e.op == "sig":
-        out = ("sig", w, sg, sig_index[e.attr])
-    elif e.op == "memread":
-        out = ("memread", w, sg, mem_index[e.attr],
-               _lower_expr(e.args[0], sig_index, mem_index, cache))
-    elif e.op == "slice":
-        out = ("slice", w, sg, e.attr, _lower_expr(e.args[0], sig_index, mem_index, cache))
-    elif e.op == "lt":
-        lhs = e.args[0]
-        out = ("lt", w, sg, int(lhs.ty.signed),
-               _lower_expr(lhs, sig_index, mem_index, cache),
-               _lower_expr(e.args[1], sig_index, mem_index, cache))
-    else:
-        kids = tuple(_lower_expr(a, sig_index, mem_index, cache) for a in e.args)
-        out = (e.op, w, sg, None) + kids
-    cache[key] = out
-    return out
-
-
-def _signal_deps(pe: PExpr, out: set) -> None:
-    stack = [pe]
+    stack = [(e, False)]
     while stack:
-        x = stack.pop()
+        x, ready = stack.pop()
+        if id(x) in cache:
+            continue
+        if not ready:
+            stack.append((x, True))
+            stack.extend((a, False) for a in x.args if id(a) not in cache)
+            continue
+        w, sg = x.ty.width, x.ty.signed
+        kids = tuple(cache[id(a)] for a in x.args)
+        if x.op == "lit":
+            node = ("lit", w, sg, tuple(to_limbs(x.attr, w)))
+        elif x.op == "sig":
+            node = ("sig", w, sg, sig_index[x.attr])
+        elif x.op == "memread":
+            node = ("memread", w, sg, mem_index[x.attr]) + kids
+        elif x.op == "slice":
+            node = ("slice", w, sg, x.attr) + kids
+        elif x.op == "lt":
+            node = ("lt", w, sg, int(x.args[0].ty.signed)) + kids
+        else:
+            node = (x.op, w, sg, None) + kids
+        cache[id(x)] = nodes.intern(node)
+    return cache[id(e)]
+
+
+def _signal_deps(pe: PExpr, nodes: _Nodes, out: set) -> None:
+    stack, seen = [pe], set()
+    while stack:
+        k = stack.pop()
+        if k in seen:
+            continue
+        seen.add(k)
+        x = nodes.nodes[k]
         if x[0] == "sig":
             out.add(x[3])
-        stack.extend(c for c in x[4:] if isinstance(c, tuple))
+        stack.extend(x[4:])
 
 
-def schedule_assignments(assigns: List[Tuple[int, PExpr]]) -> List[List[tuple]]:
+def schedule_assignments(assigns: List[Tuple[int, PExpr]], nodes: _Nodes) -> List[List[tuple]]:
     """``schedule_assignments`` + ``assignment_level`` (lib.rs:12734-12790)."""
     dst_to = {dst: i for i, (dst, _) in enumerate(assigns)}
     deps = []
     for dst, pe in assigns:
         s: set = set()
-        _signal_deps(pe, s)
+        _signal_deps(pe, nodes, s)
         deps.append([dst_to[x] for x in s if x in dst_to])
     memo: Dict[int, int] = {}
 
@@ -132,34 +157,46 @@ def schedule_assignments(assigns: List[Tuple[int, PExpr]]) -> List[List[tuple]]:
 class _PacketLowerer:
     """``MachinePacketLowerer`` (lib.rs:3060-3138): post-order SSA with memo."""
 
-    def __init__(self, next_value: int):
+    def __init__(self, next_value: int, nodes: _Nodes):
         self.next_value = next_value
+        self.nodes = nodes
         self.memo: Dict[PExpr, int] = {}
         self.instrs: List[Instr] = []
 
-    def lower(self, pe: PExpr) -> int:
-        got = self.memo.get(pe)
+    def lower(self, root: PExpr) -> int:
+        got = self.memo.get(root)
         if got is not None:
             return got
-        op, w, sg, attr = pe[0], pe[1], pe[2], pe[3]
-        kids = pe[4:]
-        if op == "lit":
-            args = tuple(attr)
-        elif op == "sig":
-            args = (attr,)
-        elif op == "memread":
-            args = (attr, self.lower(kids[0]))
-        elif op == "slice":
-            args = (self.lower(kids[0]), attr)
-        elif op == "lt":
-            args = (self.lower(kids[0]), self.lower(kids[1]), attr)
-        else:
-            args = tuple(self.lower(k) for k in kids)
-        v = self.next_value
-        self.next_value += 1
-        self.instrs.append(Instr(OP_OF[op], v, w, bool(sg), args))
-        self.memo[pe] = v
-        return v
+        stack = [(root, False)]
+        while stack:
+            pe, ready = stack.pop()
+            if pe in self.memo:
+                continue
+            x = self.nodes.nodes[pe]
+            kids = x[4:]
+            if not ready:
+                stack.append((pe, True))
+                # children pushed in reverse so they are lowered left to right
+                stack.extend((k, False) for k in reversed(kids) if k not in self.memo)
+                continue
+            op, w, sg, attr = x[0], x[1], x[2], x[3]
+            if op == "lit":
+                args = tuple(attr)
+            elif op == "sig":
+                args = (attr,)
+            elif op == "memread":
+                args = (attr, self.memo[kids[0]])
+            elif op == "slice":
+                args = (self.memo[kids[0]], attr)
+            elif op == "lt":
+                args = (self.memo[kids[0]], self.memo[kids[1]], attr)
+            else:
+                args = tuple(self.memo[k] for k in kids)
+            v = self.next_value
+            self.next_value += 1
+            self.instrs.append(Instr(OP_OF[op], v, w, bool(sg), args))
+            self.memo[pe] = v
+        return self.memo[root]
 
 
 def _instr_deps(i: Instr) -> Tuple[int, ...]:
@@ -193,12 +230,12 @@ def schedule_machine_instrs(instrs: List[Instr]) -> List[Packet]:
     return [p for p in packets if p.instrs or p.effects]
 
 
-def lower_packets_to_block(op_packets: List[List[tuple]]) -> Block:
+def lower_packets_to_block(op_packets: List[List[tuple]], nodes: _Nodes) -> Block:
     """``lower_packets_to_block`` (lib.rs:2997-3048)."""
     block = Block()
     next_value = 0
     for ops in op_packets:
-        low = _PacketLowerer(next_value)
+        low = _PacketLowerer(next_value, nodes)
         effects: List[Effect] = []
         for op in ops:
             if op[0] == "assign":
@@ -243,12 +280,13 @@ def lower_module(m: ir.Module) -> Program:
         moff += mem.depth * lb
 
     cache: Dict[int, PExpr] = {}
-    assigns = [(sig_index[d], _lower_expr(m.assigns[d], sig_index, mem_index, cache))
+    nodes = _Nodes()
+    assigns = [(sig_index[d], _lower_expr(m.assigns[d], sig_index, mem_index, cache, nodes))
                for d in m.assign_order]
 
     regs = []
     for r in m.registers:
-        nxt = _lower_expr(m.nexts[r], sig_index, mem_index, cache)
+        nxt = _lower_expr(m.nexts[r], sig_index, mem_index, cache, nodes)
         rs = m.resets.get(r)
         preset = None
         if rs is not None:
@@ -257,18 +295,18 @@ def lower_module(m: ir.Module) -> Program:
                             POL_LOW if rs.polarity == "low" else POL_HIGH)
         regs.append((sig_index[r], r, nxt, preset))
 
-    comb = schedule_assignments(assigns)
-    async_ops = [("capture", si, ("lit", r.width, r.ty.signed, tuple(pr.value)), pr)
+    comb = schedule_assignments(assigns, nodes)
+    async_ops = [("capture", si, nodes.intern(("lit", r.width, r.ty.signed, tuple(pr.value))), pr)
                  for si, r, _, pr in regs if pr is not None and pr.kind == RESET_ASYNC]
     async_reset_comb = [async_ops] if async_ops else []
     tick_ops = [("capture", si, nxt, pr) for si, _, nxt, pr in regs]
     tick_next = [tick_ops] if tick_ops else []
     writes = [("memwrite", mem_index[w.mem],
-               _lower_expr(w.enable, sig_index, mem_index, cache),
-               _lower_expr(w.addr, sig_index, mem_index, cache),
-               _lower_expr(w.data, sig_index, mem_index, cache)) for w in m.mem_writes]
+               _lower_expr(w.enable, sig_index, mem_index, cache, nodes),
+               _lower_expr(w.addr, sig_index, mem_index, cache, nodes),
+               _lower_expr(w.data, sig_index, mem_index, cache, nodes)) for w in m.mem_writes]
     tick_commit = [writes] if writes else []
 
-    blocks = tuple(lower_packets_to_block(p) for p in
+    blocks = tuple(lower_packets_to_block(p, nodes) for p in
                    (async_reset_comb, comb, tick_next, tick_commit))
     return Program(m.name, signals, memories, blocks, offset, moff)

@@ -251,13 +251,17 @@ DATA_BASE = 0x800        # data region: words 512..1023
 BASE_REG = 31            # holds DATA_BASE in random programs, never overwritten
 
 
-def random_program(rng: random.Random, n: int = 512, alu: float = 0.60, mem: float = 0.25) -> List[int]:
+def random_program(rng: random.Random, n: int = 512, alu: float = 0.60, mem: float = 0.25,
+                   backward: Optional[float] = None, data_base: int = DATA_BASE,
+                   data_span: int = 0x7FC) -> List[int]:
     """``n`` random RV32I instructions (ALU ``alu``, load/store ``mem``, branch
     the rest; in-range targets), preceded by setting the data base register
     and followed by ``ecall``.  Branch targets stay inside the program, so a
-    program may loop until the cycle cap: lanes halt unevenly."""
-    lo = ((DATA_BASE & 0xFFF) ^ 0x800) - 0x800     # addi sign-extends its immediate
-    prog = [asm("lui", BASE_REG, ((DATA_BASE - lo) >> 12) & 0xFFFFF), asm("addi", BASE_REG, BASE_REG, lo)]
+    program may loop until the cycle cap: lanes halt unevenly.  ``backward``
+    (None: targets uniform over the program) is the probability that a branch
+    goes backward; loads and stores address [data_base, data_base + data_span)."""
+    lo = ((data_base & 0xFFF) ^ 0x800) - 0x800     # addi sign-extends its immediate
+    prog = [asm("lui", BASE_REG, ((data_base - lo) >> 12) & 0xFFFFF), asm("addi", BASE_REG, BASE_REG, lo)]
     start = len(prog)
     total = start + n + 1
     regs = list(range(1, 31))
@@ -276,7 +280,7 @@ def random_program(rng: random.Random, n: int = 512, alu: float = 0.60, mem: flo
             else:
                 prog.append(asm(rng.choice(["lui", "auipc"]), rd, rng.randrange(1 << 20)))
         elif u < alu + mem:
-            off = rng.randrange(0, 0x7FC)
+            off = rng.randrange(0, data_span)
             if rng.random() < 0.5:
                 op = rng.choice(list(LOADS))
                 al = {"lb": 1, "lbu": 1, "lh": 2, "lhu": 2, "lw": 4}[op]
@@ -286,7 +290,12 @@ def random_program(rng: random.Random, n: int = 512, alu: float = 0.60, mem: flo
                 al = {"sb": 1, "sh": 2, "sw": 4}[op]
                 prog.append(asm(op, rng.choice(regs + [0]), BASE_REG, off - off % al))
         else:
-            target = rng.randrange(start, total)
+            if backward is None:
+                target = rng.randrange(start, total)
+            elif rng.random() < backward and pos > start:
+                target = rng.randrange(start, pos)
+            else:
+                target = rng.randrange(pos + 1, total)
             prog.append(asm(rng.choice(list(BRANCHES)), rng.choice(regs + [0]), rng.choice(regs + [0]),
                             (target - pos) * 4))
     prog.append(asm("ecall"))

@@ -1,6 +1,6 @@
 """Multi-design batch workload: the four designs of the mixed configuration
 (the README counter, the 20k-gate netlist, a 16-entry ready/valid FIFO with
-32-bit data, the RV32I core running per-lane random programs) with lanes per
+32-bit data, picorv32 running per-lane random programs) with lanes per
 design drawn uniformly from [1k, 64k] and cycles log-uniformly from [1e3, 1e5],
 all from one seed.  Used by the batch benchmark and its tests."""
 from __future__ import annotations
@@ -8,11 +8,11 @@ from __future__ import annotations
 import math
 import random
 from dataclasses import dataclass
-from typing import Callable, List, Optional
+from typing import Callable, List, Optional, Tuple
 
 import numpy as np
 
-from . import designs, lower, riscv
+from . import designs, lower, picorv32, riscv
 from .backend import IN_CONST, IN_PULSE, IN_RANDOM, HipSimulator
 from .program import KIND_INPUT, KIND_OUTPUT, Program
 
@@ -30,6 +30,8 @@ class Job:
     engine: str
     run: dict                                         # HipSimulator.run keyword arguments
     setup: Optional[Callable[[HipSimulator], None]] = None
+    check_mems: Tuple[str, ...] = ()                 # state compared with the host engine after a run
+    check_sigs: Tuple[str, ...] = ()
 
     def make(self, **opts) -> HipSimulator:
         sim = HipSimulator(self.program, self.lanes, engine=self.engine, **opts)
@@ -78,7 +80,25 @@ def rv32i_job(lanes: int, cycles: int, seed: int, n_insns: int = 512, distinct: 
 
     return Job("rv32i", p, lanes, "word", dict(
         cycles=cycles, seed=seed, halt="trap", observe=[],
-        inputs=[(p.signal_index("clk"), IN_CONST, 1), (p.signal_index("rst"), IN_CONST, 0)]), setup)
+        inputs=[(p.signal_index("clk"), IN_CONST, 1), (p.signal_index("rst"), IN_CONST, 0)]), setup,
+        check_mems=("regs", "mem"), check_sigs=("pc", "trap"))
+
+
+def picorv32_job(lanes: int, cycles: int, seed: int, n_insns: int = 512, distinct: int = 256,
+                 backward: Optional[float] = None) -> Job:
+    """The reference's picorv32 SoC (imported from bench/sv) running per-lane
+    random RV32I programs (``distinct`` programs dealt round-robin), halting
+    on ``trap`` after ``ecall``; ``resetn`` low for the first cycles."""
+    p = picorv32.program()
+    rng = random.Random(seed)
+    images = picorv32.images([picorv32.random_program(rng, n=n_insns, backward=backward) for _ in range(distinct)])
+
+    def setup(sim: HipSimulator) -> None:
+        sim.set_memory_limbs("mem", images[np.arange(sim.lanes) % distinct])
+
+    return Job("picorv32", p, lanes, "word", dict(cycles=cycles, seed=seed, halt="trap", observe=[],
+                                                   inputs=picorv32.inputs(p)), setup,
+               check_mems=("cpu.cpuregs", "mem"), check_sigs=("cpu.reg_pc", "trap"))
 
 
 def mixed_workload(seed: int = 1, lanes_range=(1000, 65536), cycles_range=(1e3, 1e5)) -> List[Job]:
@@ -90,7 +110,7 @@ def mixed_workload(seed: int = 1, lanes_range=(1000, 65536), cycles_range=(1e3, 
         return lanes, cycles
 
     jobs = []
-    for k, make in enumerate((counter_job, netlist_job, fifo_job, rv32i_job)):
+    for k, make in enumerate((counter_job, netlist_job, fifo_job, picorv32_job)):
         lanes, cycles = draw()
         jobs.append(make(lanes, cycles, seed * 16 + k))
     return jobs
