@@ -52,12 +52,14 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device with torch (scale reference for HBM counters)")
-    ap.add_argument("--workload", choices=("gates", "mixed", "rv32i", "rv32i-core", "faultsim"), default="gates",
+    ap.add_argument("--workload", choices=("gates", "mixed", "rv32i", "rv32i-core", "faultsim", "divergent"),
+                    default="gates",
                     help="gates: the headline line; mixed: the four-design batch, persistent launch vs one "
                          "launch per design; rv32i: picorv32 (imported from the reference's Verilog) running "
                          "per-lane random RV32I programs on the word engine; rv32i-core: the same programs on this "
                          "package's single-cycle RV32I core; faultsim: surrogate-triaged stuck-at fault "
-                         "simulation on the 20k-gate netlist")
+                         "simulation on the 20k-gate netlist; divergent: lanes halting at data-dependent cycles, "
+                         "with and without lane compaction")
     ap.add_argument("--epochs", type=int, default=6, help="faultsim: surrogate training epochs")
     ap.add_argument("--no-autotune", action="store_true", help="mixed: default launch shapes instead of the autotuner's")
     ap.add_argument("--skip-persistent", action="store_true", help="mixed: leave out the persistent work-queue kernel")
@@ -72,6 +74,8 @@ def main() -> int:
         return bench_faultsim(args)
     if args.workload in ("rv32i", "rv32i-core"):
         return bench_rv32i(args)
+    if args.workload == "divergent":
+        return bench_divergent(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         return 2
@@ -82,6 +86,8 @@ def main() -> int:
     p = build_program()
     ins = [(i, IN_RANDOM, 0) for i, s in enumerate(p.signals) if s.kind == KIND_INPUT]
     obs = [i for i, s in enumerate(p.signals) if s.kind == KIND_OUTPUT]
+    # the host AOT engines compile (gcc, about a minute) while the GPU part runs
+    aot_pending = None if args.no_cpu_baseline else start_aot_builds(p, ins, obs)
     sim = HipSimulator(p, args.lanes, engine="gate", tile_words=args.tile_words, stage_gates=args.stage_gates)
 
     def step():
@@ -115,35 +121,27 @@ def main() -> int:
         regs = [i for i, s in enumerate(p.signals) if s.kind == KIND_REG]
         words64 = args.lanes // 64
         sample = sorted({0, words64 // 2, words64 - 8})
-        ok = True
-        for base in sample:
+        def replay(base):
             ref = oracle.BitParallelFast(p, 512, threads=1)
             for _ in range(args.warmup + args.steps):
                 d, _, _ = ref.run(seed=args.seed, cycles=args.cycles, inputs=ins, observe=obs, word_base=base,
                                   digest=True)
+            return d, {r: ref.get_words(r) for r in regs[::97]}
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(len(sample)) as ex:
+            replays = list(ex.map(replay, sample))
+        ok = True
+        for base, (d, words) in zip(sample, replays):
             ok &= bool(np.array_equal(d, dg[:, base:base + 8]))
-            for r in regs[::97]:
+            for r, w in words.items():
                 got = stim.u32_rows_to_u64(sim.get_signal_bits(r))[base:base + 8]
-                ok &= bool(np.array_equal(got, ref.get_words(r)))
+                ok &= bool(np.array_equal(got, w))
         check = {"bit_exact": ok, "sample": f"{len(sample)} x 512 lanes x {args.warmup + args.steps} x {args.cycles} "
                                             "cycles: output signatures of the last run and sampled flops"}
 
     cpu = None
     if not args.no_cpu_baseline:
-        import oracle
-        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
-        probe_cycles = 20
-        base = oracle.BitParallelFast(p, args.lanes, threads=threads)
-        _, _, secs = base.run(seed=args.seed, cycles=probe_cycles, inputs=ins, observe=obs, digest=True)
-        cyc = max(probe_cycles, min(args.cycles, int(probe_cycles * args.cpu_seconds / max(secs, 1e-6))))
-        base = oracle.BitParallelFast(p, args.lanes, threads=threads)
-        d_cpu, _, secs = base.run(seed=args.seed, cycles=cyc, inputs=ins, observe=obs, digest=True)
-        fresh = HipSimulator(p, args.lanes, engine="gate", tile_words=args.tile_words, stage_gates=args.stage_gates)
-        fresh.run(cyc, seed=args.seed, inputs=ins, observe=obs)
-        same = bool(np.array_equal(stim_u64(fresh.digest()), d_cpu))
-        cpu = {"value": args.lanes * cyc / secs, "unit": "lane-cycles/s", "cores": threads, "kind": "port",
-               "sample": f"{args.lanes} lanes x {cyc} cycles of the same workload ({secs:.1f} s)",
-               "bit_exact_vs_gpu": same}
+        cpu = gates_cpu_baseline(p, ins, obs, args, aot_pending, int(info["gates"]))
 
     traffic = measured_traffic(args.lanes, args.cycles, info["tile_words"])
     line = {
@@ -164,6 +162,83 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+AOT_GROUP_WORDS = (1, 8)   # the reference's 64-lane groups, and 512-lane groups (AVX-512 vectors)
+
+
+def host_threads() -> int:
+    return max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
+
+
+def start_aot_builds(p, ins, obs):
+    """Compiles the host AOT engines (oracle/bp_aot.py) in the background."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import bp_aot
+    ex = ThreadPoolExecutor(len(AOT_GROUP_WORDS))
+    futs = {W: ex.submit(bp_aot.BitParallelAot, p, 64 * W, ins, obs, W, 1) for W in AOT_GROUP_WORDS}
+    ex.shutdown(wait=False)
+    return futs
+
+
+def gates_cpu_baseline(p, ins, obs, args, aot_pending, gpu_gates: int) -> dict:
+    """Host engines on bounded samples of the same workload (about
+    --cpu-seconds each): the interpreted group-major engine and the AOT engine
+    (straight-line C per design, the reference's BitParallelAot) at 64- and
+    512-lane groups, all host threads, plus the fastest at one thread.  Each
+    sample's signature is compared with a fresh GPU run of the same lanes and
+    cycles.  The baseline value is the fastest all-thread engine."""
+    import oracle
+    from oracle import bp_aot
+    from rrtl_amd.backend import HipSimulator
+    threads = host_threads()
+
+    def gpu_digest(lanes, cyc):
+        g = HipSimulator(p, lanes, engine="gate", tile_words=args.tile_words, stage_gates=args.stage_gates)
+        g.run(cyc, seed=args.seed, inputs=ins, observe=obs)
+        d = stim_u64(g.digest())
+        g.close()
+        return d
+
+    def timed(make, run, lanes):
+        probe = 10
+        _, secs = run(make(), probe)
+        cyc = max(probe, min(args.cycles, int(probe * args.cpu_seconds / max(secs, 1e-6))))
+        d, secs = run(make(), cyc)
+        return {"value": lanes * cyc / secs, "sample": f"{lanes} lanes x {cyc} cycles ({secs:.1f} s)",
+                "bit_exact_vs_gpu": bool(np.array_equal(d, gpu_digest(lanes, cyc)))}
+
+    engines = {}
+
+    def run_interp(e, cyc):
+        d, _, secs = e.run(seed=args.seed, cycles=cyc, inputs=ins, observe=obs, digest=True)
+        return d, secs
+    interp = timed(lambda: oracle.BitParallelFast(p, args.lanes, threads=threads), run_interp, args.lanes)
+    interp.update(threads=threads, gates_per_cycle=int(oracle.BitParallelFast(p, 64, threads=1).gate_count),
+                  engine="interpreted gate list (group-major, 512-lane groups)")
+    engines["interpreted"] = interp
+
+    def run_aot(e, cyc):
+        return e.run(args.seed, cyc)
+    for W in AOT_GROUP_WORDS:
+        gates = aot_pending[W].result().gates  # waits for the compile
+        r = timed(lambda W=W: bp_aot.BitParallelAot(p, args.lanes, ins, obs, W, threads), run_aot, args.lanes)
+        r.update(threads=threads, gates_per_cycle=gates,
+                 engine=f"AOT straight-line C (gcc -O3 -march=native), {64 * W}-lane groups")
+        engines[f"aot_{64 * W}"] = r
+    best = max(engines, key=lambda k: engines[k]["value"])
+    one = 4096
+    if best.startswith("aot"):
+        W = int(best[4:]) // 64
+        r1 = timed(lambda: bp_aot.BitParallelAot(p, one, ins, obs, W, 1), run_aot, one)
+    else:
+        r1 = timed(lambda: oracle.BitParallelFast(p, one, threads=1), run_interp, one)
+    r1.update(threads=1, engine=engines[best]["engine"])
+    engines[best + "_1thread"] = r1
+    b = engines[best]
+    return {"value": b["value"], "unit": "lane-cycles/s", "cores": threads, "kind": "port", "engine": best,
+            "sample": b["sample"], "bit_exact_vs_gpu": all(e["bit_exact_vs_gpu"] for e in engines.values()),
+            "gpu_gates_per_cycle": gpu_gates, "engines": engines}
 
 
 def measured_traffic(lanes: int, cycles: int, tile_words: int):
@@ -278,6 +353,52 @@ def bench_rv32i(args) -> int:
             "roofline": {"bound": "valu", "achieved": achieved, "peak": valu_peak, "unit": "Tops/s",
                          "frac": achieved / valu_peak, "traffic": None, "ops_per_lane_cycle": ops},
             "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_divergent(args) -> int:
+    """The divergent countdown (rrtl_amd/divergent.py, the shape of the
+    reference's divergent-batch example): 2,097,152 lanes, each halting when its
+    counter, loaded with its run length, reaches 0; 1/8 of the lanes (scattered)
+    run 20,000 cycles, the rest 1,000.  Simulated lane-cycles (halted lanes stop
+    counting) per second of device time, with lane compaction between
+    512-cycle launches and without it; both runs give the same results."""
+    from rrtl_amd import divergent
+    from rrtl_amd.backend import HipSimulator
+    p = divergent.program()
+    lanes, short, long, din = 1 << 21, 1000, 20000, 0x9E3779B1
+    seeds = divergent.run_lengths(lanes, short, long, args.seed)
+    ins = divergent.run_inputs(p)
+    res, finals = {}, {}
+    for compact in (False, True):
+        def make():
+            sim = HipSimulator(p, lanes, engine="word", compact_lanes=compact, ticks_per_launch=512)
+            sim.set_signal_limbs("seed", seeds.reshape(1, -1))
+            sim.set_signal_limbs("din", np.full((1, lanes), din, dtype=np.uint32))
+            return sim
+        warm = make()
+        warm.run(1, inputs=ins, halt="done")  # compiles the run plan
+        warm.close()
+        sim = make()
+        sim.run(long + 2, inputs=ins, halt="done")
+        st = sim.stats()
+        res["compacted" if compact else "uncompacted"] = {
+            "kernel_ms": st["kernel_ms"], "launches": st["launches"], "lane_cycles": st["lane_cycles"],
+            "lane_cycles_per_s": st["lane_cycles"] / (st["kernel_ms"] / 1e3)}
+        finals[compact] = (sim.halt_cycles(), sim.get_signal_limbs("acc"))
+        sim.close()
+    same = all(np.array_equal(a, b) for a, b in zip(finals[False], finals[True]))
+    sample = np.arange(0, lanes, 997)
+    closed = bool(np.array_equal(finals[True][1][0][sample], divergent.closed_form(seeds[sample], din)))
+    c, u = res["compacted"], res["uncompacted"]
+    line = {"metric": "divergent countdown: simulated lane-cycles/s (halted lanes stop), lane compaction",
+            "value": c["lane_cycles_per_s"], "unit": "lane-cycles/s", "n_gpus": 1, "higher_is_better": True,
+            "data": "synthetic", "dtype": "u32",
+            "speedup_vs_uncompacted": u["kernel_ms"] / c["kernel_ms"],
+            "config": {"workload": f"countdown, {lanes} lanes, 1/8 scattered {long} cycles, rest {short} cycles, "
+                                   f"512-cycle launches", "seed": args.seed},
+            "modes": res, "check": {"compacted_equals_uncompacted": same, "closed_form_sample": closed}}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -79,6 +79,15 @@ typedef struct rrtl_hip_options {
    * else 16; longer programs: 64).  Read only when
    * struct_size covers it. */
   uint32_t wave_lanes;
+  /* Word engine runs with a halt signal: 1 = compact the lanes between
+   * launches.  The run is cut into launches of ticks_per_launch cycles (1024
+   * when that is 0); after each launch the lanes still running are packed into
+   * the lowest threads, so lanes that halted free their waves and the next
+   * launch runs full waves of live lanes only (the GPU counterpart of the
+   * reference's work-stealing schedule over halting instances,
+   * crates/rrtl-sim-ir/examples/divergent_batch.rs).  Results are identical
+   * to an uncompacted run.  0 = off.  Read only when struct_size covers it. */
+  uint32_t compact_lanes;
 } rrtl_hip_options;
 
 enum { RRTL_CODEGEN_AUTO = 0, RRTL_CODEGEN_ON = 1, RRTL_CODEGEN_OFF = 2 };

@@ -75,7 +75,8 @@ class Options(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("engine", ctypes.c_uint32), ("device", ctypes.c_int32),
                 ("block_threads", ctypes.c_uint32), ("tile_words", ctypes.c_uint32),
                 ("ticks_per_launch", ctypes.c_uint32), ("tile_chunks", ctypes.c_uint32),
-                ("stage_gates", ctypes.c_uint32), ("codegen", ctypes.c_uint32), ("wave_lanes", ctypes.c_uint32)]
+                ("stage_gates", ctypes.c_uint32), ("codegen", ctypes.c_uint32), ("wave_lanes", ctypes.c_uint32),
+                ("compact_lanes", ctypes.c_uint32)]
 
 
 class RunSpec(ctypes.Structure):
@@ -194,12 +195,13 @@ class HipSimulator:
 
     def __init__(self, program: Program, lanes: int, engine: str = "auto", device: int = 0,
                  tile_words: int = 0, ticks_per_launch: int = 0, block_threads: int = 0, tile_chunks: int = 0,
-                 stage_gates: int = 0, codegen: str = "auto", wave_lanes: int = 0):
+                 stage_gates: int = 0, codegen: str = "auto", wave_lanes: int = 0, compact_lanes: bool = False):
         self.L = load_library()
         self.program = program
         blob = program.to_blob()
         opt = Options(ctypes.sizeof(Options), ENGINE_NAMES[engine], device, block_threads, tile_words,
-                      ticks_per_launch, tile_chunks, stage_gates, CODEGEN_NAMES[codegen], wave_lanes)
+                      ticks_per_launch, tile_chunks, stage_gates, CODEGEN_NAMES[codegen], wave_lanes,
+                      int(bool(compact_lanes)))
         h = ctypes.c_void_p()
         st = self.L.rrtl_hip_create(blob, len(blob), lanes, ctypes.byref(opt), ctypes.byref(h))
         if st != 0:

@@ -106,6 +106,8 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
         opt.codegen = options->codegen;
       if (options->struct_size >= offsetof(rrtl_hip_options, wave_lanes) + sizeof(uint32_t))
         opt.wave_lanes = options->wave_lanes;
+      if (options->struct_size >= offsetof(rrtl_hip_options, compact_lanes) + sizeof(uint32_t))
+        opt.compact_lanes = options->compact_lanes;
     }
     if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 256))
       throw rrtl::Error("E_HIP_OPTIONS: block_threads must be 64, 128, 192 or 256");

@@ -21,6 +21,7 @@ struct EngineOptions {
   uint32_t stage_gates = 0;        // gate engine: gate positions per thread per stage (0: automatic, 2, 3 or 4)
   uint32_t codegen = 0;            // word engine: CODEGEN_AUTO / CODEGEN_ON (generated kernel) or CODEGEN_OFF (interpreter)
   uint32_t wave_lanes = 0;         // word engine: active lanes per 64-thread wave (16, 32, 64; 0: automatic)
+  uint32_t compact_lanes = 0;      // word engine: pack the live lanes of halting runs between launches
 };
 enum : uint32_t { CODEGEN_AUTO = 0, CODEGEN_ON = 1, CODEGEN_OFF = 2 };
 

@@ -21,6 +21,7 @@
 // start and end of the run.  Gates of one level run in parallel over
 // (gate, word) pairs; a workgroup barrier separates levels.
 #include "engine.hpp"
+#include <cstdlib>
 #include "gate_device.hpp"
 #include <algorithm>
 #include <cmath>
@@ -35,9 +36,16 @@ namespace {  // anon
 
 using namespace gate;
 
-// Clears the work-queue control words (a kernel, so a captured graph could replay it).
-__global__ void queue_reset_kernel(uint32_t* ctl, uint32_t n) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0u;
+// Clears the work-queue control words before a queue launch (a kernel, so a
+// captured graph could replay it): the item counter and the per-tile chunk
+// counts always, the error flag ctl[1] according to err_mode (0: keep, so an
+// error raised by an earlier launch of the run survives to the end of the run;
+// 1: clear, the run's first launch; 2: set, the host's error injection).
+__global__ void queue_reset_kernel(uint32_t* ctl, uint32_t n, uint32_t err_mode) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (i != 1) ctl[i] = 0u;
+    else if (err_mode) ctl[1] = err_mode == 2 ? 1u : 0u;
+  }
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
@@ -891,6 +899,10 @@ class GateEngine final : public Engine {
     int wall_khz = 0;
     RRTL_HIP_CHECK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, opt.device));
     wall_khz_ = wall_khz > 0 ? uint64_t(wall_khz) : 100000u;
+    // Test hook: RRTL_HIP_INJECT_QUEUE_ERROR=k raises the work queue's error
+    // flag from the host before the run's k-th queue launch (0-based), so a
+    // test can check that an error in any launch fails the whole run.
+    if (const char* inj = std::getenv("RRTL_HIP_INJECT_QUEUE_ERROR")) inject_queue_error_ = uint64_t(std::strtoull(inj, nullptr, 10));
     RRTL_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     stage_row_ = N.n_nodes;
     rows_ = N.n_nodes + uint32_t(N.commits.size());
@@ -1417,7 +1429,7 @@ class GateEngine final : public Engine {
   uint64_t enqueue(const Launch& L, hipStream_t s) {
     TileArgs A = L.R.A;
     const uint64_t chunk = opt_.ticks_per_launch ? opt_.ticks_per_launch : std::max<uint64_t>(L.cycles, 1);
-    uint64_t launches = 0;
+    uint64_t launches = 0, queue_launches = 0;
     last_chunks_ = 1;
     for (uint64_t done = 0; done < L.cycles; done += chunk) {
       A.cycle_base = done;
@@ -1429,7 +1441,9 @@ class GateEngine final : public Engine {
         Q.chunk = (A.cycles + L.chunks - 1) / L.chunks;
         Q.n_chunks = uint32_t((A.cycles + Q.chunk - 1) / Q.chunk);
         Q.wait_limit = wall_khz_ * 1000u * 120u;  // two minutes
-        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(256), 0, s, qctl_.p, L.R.tiles + 2);
+        const uint32_t err_mode = queue_launches == inject_queue_error_ ? 2u : queue_launches == 0 ? 1u : 0u;
+        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(256), 0, s, qctl_.p, L.R.tiles + 2, err_mode);
+        queue_launches++;
         const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(L.resident, 1), L.R.tiles * Q.n_chunks);
         launch_queue(L.R, grid, A, Q, s);
         last_chunks_ = Q.n_chunks;
@@ -1558,6 +1572,7 @@ class GateEngine final : public Engine {
   Launch async_;  // the run of a concurrent batch
   uint32_t W_ = 0, M_ = 2, last_slots_ = 0, last_levels_ = 0, last_gates_ = 0, last_chunks_ = 1, cus_ = 1;
   uint64_t wall_khz_ = 100000;
+  uint64_t inject_queue_error_ = ~uint64_t(0);
   DevBuf<uint32_t> qctl_;  // work-queue counter, error flag, per-tile chunk counts
   Prepared plan_;
   std::string plan_key_;

@@ -62,6 +62,11 @@ struct WordArgs {
   uint32_t check_lanes;                                   // lanes below this compare (UINT32_MAX: all)
   uint32_t* mm_count; uint32_t* mm_rec; uint32_t mm_max;  // per lane: mismatch count; first mm_max records
                                                           // [lanes][mm_max][6] = (cycle, check, actual limbs 0..3)
+  // Lane compaction (null lane_map: thread slot t runs lane t).  Workgroups are
+  // dealt round-robin to the XCDs, so the slots of workgroup b belong to
+  // partition b % 8; slot j of partition x runs lane lane_map[x * map_cap + j]
+  // for j < live[x].
+  const uint32_t* lane_map; const uint32_t* live; uint32_t map_cap;
 };
 
 // Where a block keeps its per-lane scratch and signal values during a launch:
@@ -539,8 +544,15 @@ constexpr uint32_t STEP_OBS = 15;  // sequence entry: observe (else a block id)
 template <int kMode, class Exec = InterpExec>
 __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
   const uint32_t per_block = A.block_lanes ? A.block_lanes : blockDim.x;
-  const uint32_t lane = block * per_block + threadIdx.x;
-  if (threadIdx.x >= per_block || lane >= A.lanes) return;
+  const uint32_t slot = block * per_block + threadIdx.x;
+  if (threadIdx.x >= per_block) return;
+  uint32_t lane = slot;
+  if (A.lane_map) {
+    const uint32_t part = block % 8u, j = (block / 8u) * per_block + threadIdx.x;
+    if (j >= A.live[part]) return;
+    lane = A.lane_map[size_t(part) * A.map_cap + j];
+  }
+  if (lane >= A.lanes) return;
   Ctx<kMode> C;
   C.values = A.values; C.mems = A.mems; C.cpool = A.cpool; C.mem_table = A.mem_table;
   C.lanes = A.lanes; C.lane = lane;
@@ -575,6 +587,11 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
     if (first) { seq = settle_seq; n = (A.flags & F_EVAL_FIRST) ? 2 : 0; }
     else if (last) { seq = settle_seq; n = (A.flags & F_FINAL_EVAL) ? 2 : 0; }
     else {
+      // a wave whose lanes have all halted skips to the final step
+      if (A.halt_offset != NONE && __builtin_amdgcn_ballot_w64(!halted) == 0) {
+        s = steps - 2;
+        continue;
+      }
       if (halted) continue;
       c = s - 1;
       if (A.n_in > A.n_in_once) apply_stimulus(A, C, c, A.cycle_base + c, A.n_in_once, A.n_in);

@@ -702,6 +702,86 @@ std::vector<int32_t> lds_memory_rows(const Program& P, uint32_t slots, uint32_t 
   return row;
 }
 
+// ------------------------------------------------------------------ lane compaction
+// Between the launches of a halting run (rrtl_hip_options.compact_lanes).
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2), so a
+// lane stays in the partition (workgroup id mod 8) it started in: partition x
+// owns workgroups x, x + 8, ... and its slot j runs lane map[x * cap + j] for
+// j < live[x].  After a launch each partition keeps its lanes whose halt flag
+// is still clear, in their order (a stable scan), so the live lanes fill the
+// partition's first workgroups, the lanes sharing a cache line of word-major
+// state stay together on one XCD, and the other workgroups exit at once.
+constexpr uint32_t kParts = 8;
+
+__global__ void lane_map_init_kernel(uint32_t* map, uint32_t* live, uint32_t lanes, uint32_t per_block,
+                                     uint32_t blocks, uint32_t cap) {
+  const uint32_t b = blockIdx.x, t = threadIdx.x;
+  if (b < blocks && t < per_block) {
+    const uint32_t part = b % kParts, j = (b / kParts) * per_block + t, lane = b * per_block + t;
+    map[size_t(part) * cap + j] = lane < lanes ? lane : 0xFFFFFFFFu;
+  }
+  if (b == 0 && t < kParts) live[t] = ((blocks + kParts - 1 - t) / kParts) * per_block;
+}
+
+// The kept lanes of a partition go to its slots in their order, but never onto
+// fewer waves than the partition's XCD runs at once (one per SIMD, `floor_slots`
+// slots): while the live lanes fit, they are spread over those waves (the first
+// q slots of each), so a memory-bound design keeps every CU's memory path busy;
+// only beyond that are the waves packed full.  Two passes: count, then place.
+__global__ void __launch_bounds__(1024) lane_compact_kernel(const uint32_t* map_in, const uint32_t* live_in,
+                                                            const uint64_t* halt, uint32_t* map_out,
+                                                            uint32_t* live_out, uint32_t cap, uint32_t lanes,
+                                                            uint32_t wave_slots, uint32_t floor_slots) {
+  __shared__ uint32_t wave_keep[16];
+  const uint32_t part = blockIdx.x, n = live_in[part];
+  const uint32_t* in = map_in + size_t(part) * cap;
+  uint32_t* out = map_out + size_t(part) * cap;
+  const uint32_t w = threadIdx.x >> 6, lid = threadIdx.x & 63u;
+  auto keep_of = [&](uint32_t j) {
+    const uint32_t lane = j < n ? in[j] : 0xFFFFFFFFu;
+    return lane < lanes && !(halt[lane] & 1u);
+  };
+  // pass 1: how many lanes stay
+  uint32_t kept = 0;
+  for (uint32_t c0 = 0; c0 < n; c0 += 1024u) {
+    const uint64_t m = __builtin_amdgcn_ballot_w64(keep_of(c0 + threadIdx.x));
+    if (lid == 0) wave_keep[w] = uint32_t(__popcll(m));
+    __syncthreads();
+    for (uint32_t k = 0; k < 16u; k++) kept += wave_keep[k];
+    __syncthreads();
+  }
+  // slots of the next launch: whole waves, at least floor_slots, at most the current count
+  uint32_t slots = (kept + wave_slots - 1) / wave_slots * wave_slots;
+  slots = std::min(std::max(slots, floor_slots), (n + wave_slots - 1) / wave_slots * wave_slots);
+  slots = std::min(slots, cap);
+  const uint32_t waves = std::max(1u, slots / wave_slots);
+  const uint32_t q = std::max(1u, (kept + waves - 1) / waves);  // live lanes per wave
+  for (uint32_t j = threadIdx.x; j < slots; j += 1024u) out[j] = 0xFFFFFFFFu;
+  __syncthreads();
+  // pass 2: place kept lane k at the k % q-th slot of wave k / q
+  uint32_t base = 0;
+  for (uint32_t c0 = 0; c0 < n; c0 += 1024u) {
+    const uint32_t j = c0 + threadIdx.x;
+    const bool keep = keep_of(j);
+    const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+    if (lid == 0) wave_keep[w] = uint32_t(__popcll(m));
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t k = 0; k < 16u; k++) {
+      const uint32_t c = wave_keep[k];
+      before += k < w ? c : 0u;
+      total += c;
+    }
+    if (keep) {
+      const uint32_t k = base + before + uint32_t(__popcll(m & ((uint64_t(1) << lid) - 1u)));
+      out[(k / q) * wave_slots + k % q] = in[j];
+    }
+    __syncthreads();
+    base += total;
+  }
+  if (threadIdx.x == 0) live_out[part] = slots;
+}
+
 // ------------------------------------------------------------------ engine
 class WordEngine final : public Engine {
  public:
@@ -736,6 +816,7 @@ class WordEngine final : public Engine {
     // the instructions each wave issues and keep full waves (measured: 3.5e9
     // against 2.4e9 lane-cycles/s at 16 lanes per wave).
     const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
+    cus_ = cus;
     if (opt.wave_lanes) {
       wave_lanes_ = opt.wave_lanes;  // 4, 8, 16, 32 or 64
     } else {
@@ -783,6 +864,7 @@ class WordEngine final : public Engine {
     for (auto& b : blk_) b.release();
     for (auto& b : run_bufs_) b.release();
     digest_.release(); trace_.release(); halt_.release(); mm_count_.release(); mm_rec_.release();
+    lane_map_.release(); live_.release();
     if (stream_) (void)hipStreamDestroy(stream_);
   }
   uint32_t kind() const override { return ENGINE_WORD; }
@@ -1074,17 +1156,33 @@ class WordEngine final : public Engine {
   // The run's kernels on stream s (kernels only: no copies, fills or host
   // waits, so the sequence can be captured into a graph).  Returns the launches.
   uint64_t enqueue(const WordArgs& A, uint64_t cycles, bool outputs, hipStream_t s) {
-    const uint64_t chunk = (opt_.ticks_per_launch && outputs) ? opt_.ticks_per_launch : std::max<uint64_t>(cycles, 1);
+    // lane compaction: halting runs cut into launches, live lanes packed between them
+    const bool compact = opt_.compact_lanes && outputs && A.halt_offset != NONE && A.halt_cycles;
+    uint64_t chunk = (opt_.ticks_per_launch && outputs) ? opt_.ticks_per_launch : std::max<uint64_t>(cycles, 1);
+    if (compact && !opt_.ticks_per_launch) chunk = 1024;
     const uint32_t per_block = wave_lanes_ < 64 ? wave_lanes_ : block_;
     const uint32_t grid = (lanes_ + per_block - 1) / per_block;
     const size_t lds = lds_bytes(mode_, block_, run_slots_, lds_mem_words_);
+    const uint32_t cap = (grid + kParts - 1) / kParts * per_block;  // slots per partition
+    if (compact) {
+      lane_map_.alloc(size_t(cap) * kParts * 2);
+      live_.alloc(kParts * 2);
+      hipLaunchKernelGGL(lane_map_init_kernel, dim3(grid), dim3(std::max<uint32_t>(per_block, kParts)), 0, s,
+                         lane_map_.p, live_.p, lanes_, per_block, grid, cap);
+      RRTL_HIP_CHECK(hipGetLastError());
+    }
     uint64_t done = 0, launches = 0;
     do {
       WordArgs B = A;
       B.cycle_base = done;
       B.cycles = std::min<uint64_t>(chunk, cycles - done);
       if (done) B.flags &= ~uint32_t(F_EVAL_FIRST);
-      if (done + B.cycles < cycles) B.flags &= ~uint32_t(F_FINAL_EVAL);
+      // with compaction a lane may leave after any launch: every launch ends with the final settle
+      if (done + B.cycles < cycles && !compact) B.flags &= ~uint32_t(F_FINAL_EVAL);
+      const uint32_t cur = uint32_t(launches & 1u);
+      if (compact) {
+        B.lane_map = lane_map_.p + size_t(cur) * cap * kParts; B.live = live_.p + cur * kParts; B.map_cap = cap;
+      }
       if (A.trace) B.trace = A.trace + done * A.n_obs_limbs * lanes_;
       if (A.trace_in) B.trace_in = A.trace_in + done * A.trace_in_limbs * lanes_;
       if (A.trace_uni) B.trace_uni = A.trace_uni + done * A.trace_uni_limbs;
@@ -1097,6 +1195,15 @@ class WordEngine final : public Engine {
       else if (mode_ == WM_SCRATCH_LDS) hipLaunchKernelGGL(word_kernel<WM_SCRATCH_LDS>, dim3(grid), dim3(block_), lds, s, B);
       else hipLaunchKernelGGL(word_kernel<WM_HBM>, dim3(grid), dim3(block_), 0, s, B);
       RRTL_HIP_CHECK(hipGetLastError());
+      if (compact && done + B.cycles < cycles) {
+        const uint32_t nxt = cur ^ 1u;
+        const uint32_t wave_slots = std::min<uint32_t>(per_block, 64u);
+        const uint32_t floor_slots = std::max<uint32_t>(1u, cus_ / kParts) * 4u * wave_slots;  // a wave per SIMD
+        hipLaunchKernelGGL(lane_compact_kernel, dim3(kParts), dim3(1024), 0, s, lane_map_.p + size_t(cur) * cap * kParts,
+                           live_.p + cur * kParts, A.halt_cycles, lane_map_.p + size_t(nxt) * cap * kParts,
+                           live_.p + nxt * kParts, cap, lanes_, wave_slots, floor_slots);
+        RRTL_HIP_CHECK(hipGetLastError());
+      }
       launches++;
       done += B.cycles;
     } while (done < cycles);
@@ -1153,6 +1260,8 @@ class WordEngine final : public Engine {
   DevBuf<uint32_t> values_, mems_, scratch_, cpool_, mtab_, blk_[4], run_bufs_[8], trace_, mm_count_, mm_rec_;
   uint32_t mm_lanes_ = 0, mm_max_ = 0;  // lanes that checked in the last replay, records kept per lane
   DevBuf<uint64_t> digest_, halt_;
+  DevBuf<uint32_t> lane_map_, live_;    // lane compaction: two lane maps and their live counts
+  uint32_t cus_ = 256;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };
 

@@ -199,6 +199,25 @@ def test_full_size_netlist_sampled_words():
         assert np.array_equal(d1[:, w64], dg[:, 0]), w64
 
 
+def test_work_queue_error_in_an_earlier_launch_fails_the_run(monkeypatch):
+    """The work queue's error flag is cleared once per run, not per launch: an
+    error raised in the first of three launches (injected from the host before
+    that launch) still fails the run; without the injection the run succeeds."""
+    from rrtl_amd.backend import HipError
+    flush_print("case: work queue error flag over three launches")
+    p = small_netlist_program(gates=3000, levels=24, dffs=300, inputs=48, outputs=32, seed=5)
+    ins, obs = netlist_inputs(p), netlist_outputs(p)
+    monkeypatch.setenv("RRTL_HIP_INJECT_QUEUE_ERROR", "0")
+    sim = HipSimulator(p, 1000, engine="gate", tile_chunks=3, ticks_per_launch=16)
+    with pytest.raises(HipError, match="E_HIP_INTERNAL"):
+        sim.run(40, seed=1, inputs=ins, observe=obs)
+    sim.close()
+    monkeypatch.delenv("RRTL_HIP_INJECT_QUEUE_ERROR")
+    ok = HipSimulator(p, 1000, engine="gate", tile_chunks=3, ticks_per_launch=16)
+    ok.run(40, seed=1, inputs=ins, observe=obs)
+    assert ok.stats()["launches"] == 3
+
+
 @pytest.mark.parametrize("chunks,ticks", [(5, 0), (3, 16), (40, 0)])
 def test_work_queue_chunks_match_single_pass(chunks, ticks):
     """The work-queue schedule (each tile's cycles cut into chunks that continue
