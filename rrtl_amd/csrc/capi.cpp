@@ -248,6 +248,84 @@ rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out) {
 rrtl_status rrtl_hip_get_detect(rrtl_hip_engine* e, uint32_t* out) {
   return guard(e, [&](rrtl::Engine& x) { if (!out) throw rrtl::Error("E_HIP_NULL: null buffer"); x.get_detect(out); });
 }
+namespace {
+// Trace replay as the reference's default SimBackend::replay_trace
+// (crates/rrtl-sim-ir/src/lib.rs:5270-5345): per step the inputs are set raw,
+// comb is evaluated, every active check is read back and compared per lane,
+// and tick_from_evaluated_no_post_eval advances; mismatches are counted and
+// the first max_mismatches kept in (step, check, lane) order.  The gate engine
+// runs it through its API calls (each step a few launches and copies); unlike
+// the reference's generic loop it also takes per-lane (independent) inputs on
+// any number of lanes, as the word engine's on-chip replay does.
+void replay_host_loop(rrtl::Engine& x, const rrtl_hip_replay_spec* sp, uint64_t* mismatch_count,
+                      rrtl_hip_replay_mismatch* mismatches, uint32_t* n_mismatches) {
+  const rrtl::Program& P = x.program();
+  const bool indep = sp->lane_mode == RRTL_REPLAY_INDEPENDENT;
+  const uint64_t lanes = x.lanes();
+  const uint64_t check_lanes = (!indep && sp->check_mode == RRTL_REPLAY_LANE0) ? 1 : lanes;
+  uint32_t in_limbs = 0, check_limbs = 0;
+  for (uint32_t i = 0; i < sp->n_inputs; i++) {
+    if (sp->input_signal[i] >= P.signals.size()) throw rrtl::Error("E_SIM_IR_REPLAY_SIGNAL: input signal out of range");
+    in_limbs += P.signals[sp->input_signal[i]].limbs;
+  }
+  for (uint32_t j = 0; j < sp->n_checks; j++) {
+    if (sp->check_signal[j] >= P.signals.size()) throw rrtl::Error("E_SIM_IR_REPLAY_SIGNAL: check signal out of range");
+    if (P.signals[sp->check_signal[j]].limbs > 4) throw rrtl::Error("E_HIP_WIDE: checked signals are limited to 128 bits");
+    check_limbs += P.signals[sp->check_signal[j]].limbs;
+  }
+  const uint32_t check_words = (sp->n_checks + 31) / 32;
+  std::vector<uint32_t> buf, got;
+  uint64_t count = 0;
+  uint32_t kept = 0;
+  for (uint64_t step = 0; step < sp->steps; step++) {
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < sp->n_inputs; i++) {
+      const uint32_t lb = P.signals[sp->input_signal[i]].limbs;
+      buf.assign(size_t(lb) * lanes, 0u);
+      for (uint32_t l = 0; l < lb; l++)
+        for (uint64_t ln = 0; ln < lanes; ln++)
+          buf[l * lanes + ln] = indep ? sp->input_values[((step * in_limbs) + off + l) * lanes + ln]
+                                      : sp->input_values[step * in_limbs + off + l];
+      x.set_signal_limbs(sp->input_signal[i], buf.data(), false);
+      off += lb;
+    }
+    x.eval_combinational();
+    off = 0;
+    for (uint32_t j = 0; j < sp->n_checks; j++) {
+      const uint32_t sig = sp->check_signal[j], lb = P.signals[sig].limbs;
+      const bool active = !sp->check_active || ((sp->check_active[step * check_words + j / 32] >> (j % 32)) & 1u);
+      if (active) {
+        got.assign(size_t(lb) * lanes, 0u);
+        x.get_signal_limbs(sig, got.data());
+        for (uint64_t ln = 0; ln < check_lanes; ln++) {
+          bool same = true;
+          uint32_t exp[4] = {0, 0, 0, 0}, act[4] = {0, 0, 0, 0};
+          for (uint32_t l = 0; l < lb; l++) {
+            exp[l] = indep ? sp->expected[((step * check_limbs) + off + l) * lanes + ln]
+                           : sp->expected[step * check_limbs + off + l];
+            act[l] = got[l * lanes + ln];
+            same = same && exp[l] == act[l];
+          }
+          if (same) continue;
+          count++;
+          if (kept < sp->max_mismatches) {
+            rrtl_hip_replay_mismatch& o = mismatches[kept++];
+            o.step = step;
+            o.check_index = sp->check_index ? sp->check_index[j] : j;
+            o.lane = uint32_t(ln);
+            for (uint32_t l = 0; l < 4; l++) { o.expected[l] = exp[l]; o.actual[l] = act[l]; }
+          }
+        }
+      }
+      off += lb;
+    }
+    x.tick_from_evaluated();
+  }
+  *mismatch_count = count;
+  *n_mismatches = kept;
+}
+}  // namespace
+
 rrtl_status rrtl_hip_replay(rrtl_hip_engine* e, const rrtl_hip_replay_spec* sp, uint64_t* mismatch_count,
                             rrtl_hip_replay_mismatch* mismatches, uint32_t* n_mismatches) {
   return guard(e, [&](rrtl::Engine& x) {
@@ -258,6 +336,10 @@ rrtl_status rrtl_hip_replay(rrtl_hip_engine* e, const rrtl_hip_replay_spec* sp, 
       throw rrtl::Error("E_HIP_RUN_SPEC: unknown replay lane or check mode");
     if ((sp->n_inputs && (!sp->input_signal || !sp->input_values)) || (sp->n_checks && (!sp->check_signal || !sp->expected)))
       throw rrtl::Error("E_HIP_NULL: null input or check array");
+    if (x.kind() == RRTL_HIP_GATE) {
+      replay_host_loop(x, sp, mismatch_count, mismatches, n_mismatches);
+      return;
+    }
     const rrtl::Program& P = x.program();
     const bool indep = sp->lane_mode == RRTL_REPLAY_INDEPENDENT;
     rrtl::RunSpec r;

@@ -12,11 +12,12 @@ constexpr uint32_t SP_SCRATCH = 0u << 30, SP_SIGNAL = 1u << 30, SP_CONST = 2u <<
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 
 // record ops: machine ops keep their numbers, effects follow
-enum : uint32_t { R_STORE = 32, R_ASYNC_CAPTURE, R_CAPTURE, R_COMMIT, R_MEMWRITE, R_SIGLOAD };
+enum : uint32_t { R_STORE = 40, R_ASYNC_CAPTURE, R_CAPTURE, R_COMMIT, R_MEMWRITE, R_SIGLOAD };
 // Fused one-limb records (the fusion menu of the reference's interpreter,
 // crates/rrtl-gpu-sim/src/interp.rs:351-495): an inner operation whose value
-// has no other reader folded into its reader, operands a, b, c in w[2..4].
-// Results are masked to the record's width like every one-limb record.
+// has no other reader folded into its reader, operands a, b, c in w[2..4] and
+// for the four-input cells d in w[6].  Results are masked to the record's width
+// like every one-limb record.
 enum : uint32_t {
   FOP_NAND = OP_COUNT,  // ~(a & b)
   FOP_NOR,              // ~(a | b)
@@ -28,6 +29,9 @@ enum : uint32_t {
   FOP_MULADD,           // a * b + c
   FOP_AOI21,            // ~((a & b) | c)
   FOP_OAI21,            // ~((a | b) & c)
+  FOP_MULSUB,           // a * b - c
+  FOP_AOI22,            // ~((a & b) | (c & d))
+  FOP_OAI22,            // ~((a | b) & (c | d))
   FOP_END
 };
 static_assert(FOP_END <= R_STORE, "fused opcodes must stay below the effect records");
@@ -179,6 +183,9 @@ __device__ __forceinline__ bool exec_rec1(const Cx& C, const Rec& R) {
     case FOP_MULADD: v = C.ld(R.w[2], 0) * C.ld(R.w[3], 0) + C.ld(R.w[4], 0); break;
     case FOP_AOI21: v = ~((C.ld(R.w[2], 0) & C.ld(R.w[3], 0)) | C.ld(R.w[4], 0)); break;
     case FOP_OAI21: v = ~((C.ld(R.w[2], 0) | C.ld(R.w[3], 0)) & C.ld(R.w[4], 0)); break;
+    case FOP_MULSUB: v = C.ld(R.w[2], 0) * C.ld(R.w[3], 0) - C.ld(R.w[4], 0); break;
+    case FOP_AOI22: v = ~((C.ld(R.w[2], 0) & C.ld(R.w[3], 0)) | (C.ld(R.w[4], 0) & C.ld(R.w[6], 0))); break;
+    case FOP_OAI22: v = ~((C.ld(R.w[2], 0) | C.ld(R.w[3], 0)) & (C.ld(R.w[4], 0) | C.ld(R.w[6], 0))); break;
     case OP_AND: v = C.ld(R.w[2], 0) & C.ld(R.w[3], 0); break;
     case OP_OR: v = C.ld(R.w[2], 0) | C.ld(R.w[3], 0); break;
     case OP_XOR: v = C.ld(R.w[2], 0) ^ C.ld(R.w[3], 0); break;

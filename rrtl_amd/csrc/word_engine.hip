@@ -86,7 +86,7 @@ class WordCompiler {
   // Equal widths keep the result exact: the dropped inner mask only clears
   // bits the reader's own mask clears, and carries and products only move
   // bits upward.
-  struct Fused { uint32_t fop, a, b, c; };
+  struct Fused { uint32_t fop, a, b, c, d = NONE; };
   void plan_fusion(const Block& bk, std::vector<uint8_t>& absorbed, std::unordered_map<uint32_t, Fused>& fused) {
     std::vector<const Instr*> def(bk.n_values, nullptr);
     std::vector<uint32_t> uses(bk.n_values, 0);
@@ -119,8 +119,17 @@ class WordCompiler {
           auto it = fused.find(x);
           if (it != fused.end() && uses[x] == 1 && def[x]->width == u.width &&
               (it->second.fop == FOP_ANDOR || it->second.fop == FOP_ORAND)) {
-            fused[d] = {it->second.fop == FOP_ANDOR ? uint32_t(FOP_AOI21) : uint32_t(FOP_OAI21), it->second.a,
-                        it->second.b, it->second.c};
+            // OR(AND, c) / AND(OR, c) under a NOT: AOI22 / OAI22 when c is itself a
+            // single-reader AND / OR, else AOI21 / OAI21
+            const bool aoi = it->second.fop == FOP_ANDOR;
+            const uint32_t c = it->second.c;
+            if (plain(c, aoi ? uint32_t(OP_AND) : uint32_t(OP_OR), u)) {
+              fused[d] = {aoi ? uint32_t(FOP_AOI22) : uint32_t(FOP_OAI22), it->second.a, it->second.b,
+                          def[c]->args[0], def[c]->args[1]};
+              absorbed[c] = 1;
+            } else {
+              fused[d] = {aoi ? uint32_t(FOP_AOI21) : uint32_t(FOP_OAI21), it->second.a, it->second.b, c};
+            }
             fused.erase(it);
             absorbed[x] = 1;
             continue;
@@ -134,14 +143,15 @@ class WordCompiler {
             }
           continue;
         }
-        // (reader, inner) pairs: OR(AND), AND(OR), XOR(XOR), ADD(ADD), ADD(MUL)
-        static const uint32_t pairs[5][3] = {{OP_OR, OP_AND, FOP_ANDOR}, {OP_AND, OP_OR, FOP_ORAND},
+        // (reader, inner) pairs: OR(AND), AND(OR), XOR(XOR), ADD(ADD), ADD(MUL),
+        // SUB(MUL) with the product as the minuend (SUB does not commute)
+        static const uint32_t pairs[6][3] = {{OP_OR, OP_AND, FOP_ANDOR}, {OP_AND, OP_OR, FOP_ORAND},
                                              {OP_XOR, OP_XOR, FOP_XOR3}, {OP_ADD, OP_ADD, FOP_ADD3},
-                                             {OP_ADD, OP_MUL, FOP_MULADD}};
+                                             {OP_ADD, OP_MUL, FOP_MULADD}, {OP_SUB, OP_MUL, FOP_MULSUB}};
         bool done = false;
         for (const auto& pr : pairs) {
           if (done || u.op != pr[0]) continue;
-          for (int k = 0; k < 2 && !done; k++) {
+          for (int k = 0; k < (u.op == OP_SUB ? 1 : 2) && !done; k++) {
             const uint32_t x = u.args[k], other = u.args[1 - k];
             if (x == other || !plain(x, pr[1], u)) continue;
             fused[d] = {pr[2], def[x]->args[0], def[x]->args[1], other};
@@ -175,7 +185,7 @@ class WordCompiler {
         // a fused record reads its inner operation's operands
         auto fz = fused.find(in.dst);
         if (fz != fused.end())
-          for (uint32_t a : {fz->second.a, fz->second.b, fz->second.c}) if (a != NONE) last_use[a] = pos;
+          for (uint32_t a : {fz->second.a, fz->second.b, fz->second.c, fz->second.d}) if (a != NONE) last_use[a] = pos;
         pos++;
       }
       for (const Effect& ef : pk.effects) {
@@ -232,7 +242,7 @@ class WordCompiler {
           r.w[0] = f.fop | (1u << 8);
           r.w[2] = opnd(f.a).code; r.w[3] = opnd(f.b).code; r.w[4] = f.c == NONE ? 0u : opnd(f.c).code;
           r.w[7] = opnd(f.a).limbs | (opnd(f.b).limbs << 8);
-          r.w[6] = std::max(opnd(f.a).width, opnd(f.b).width);
+          r.w[6] = f.d != NONE ? opnd(f.d).code : std::max(opnd(f.a).width, opnd(f.b).width);
           uint32_t slot = alloc(L, pos, last_use[in.dst]);
           r.w[1] = slot;
           recs.push_back(r);
@@ -474,7 +484,7 @@ class WordJit {
     switch (op) {
       case OP_NOT: return "~" + opnd(R.w[2], 0);
       case FOP_NAND: case FOP_NOR: case FOP_XNOR: case FOP_ANDOR: case FOP_ORAND: case FOP_XOR3: case FOP_ADD3:
-      case FOP_MULADD: case FOP_AOI21: case FOP_OAI21: {
+      case FOP_MULADD: case FOP_AOI21: case FOP_OAI21: case FOP_MULSUB: case FOP_AOI22: case FOP_OAI22: {
         const std::string a = opnd(R.w[2], 0), b = opnd(R.w[3], 0);
         switch (op) {
           case FOP_NAND: return "~(" + a + " & " + b + ")";
@@ -490,6 +500,9 @@ class WordJit {
           case FOP_ADD3: return "(" + a + " + " + b + " + " + c + ")";
           case FOP_MULADD: return "(" + a + " * " + b + " + " + c + ")";
           case FOP_AOI21: return "~((" + a + " & " + b + ") | " + c + ")";
+          case FOP_MULSUB: return "(" + a + " * " + b + " - " + c + ")";
+          case FOP_AOI22: return "~((" + a + " & " + b + ") | (" + c + " & " + opnd(R.w[6], 0) + "))";
+          case FOP_OAI22: return "~((" + a + " | " + b + ") & (" + c + " | " + opnd(R.w[6], 0) + "))";
           default: return "~((" + a + " | " + b + ") & " + c + ")";
         }
       }

@@ -158,8 +158,9 @@ def random_gate_design(seed: int, n_inputs: int = 24, n_regs: int = 40, n_wires:
 
 def fusion_design(width: int = 8) -> ir.Module:
     """Every fusable pattern of the word engine's record encoder at one width
-    (NOT over AND/OR/XOR, over OR-of-AND and AND-of-OR; OR(AND), AND(OR),
-    XOR(XOR), ADD(ADD), ADD(MUL) with the inner value on either side), each on
+    (NOT over AND/OR/XOR, over OR-of-AND and AND-of-OR with one or two inner
+    cells; OR(AND), AND(OR), XOR(XOR), ADD(ADD), ADD(MUL) with the inner value
+    on either side, SUB(MUL) with the product as minuend), each on
     inputs of its own so that the lowering's shared subexpressions do not give
     an inner value a second reader, plus cases that must stay unfused: an inner
     value with two readers, an inner value of another width and a stored inner
@@ -183,6 +184,10 @@ def fusion_design(width: int = 8) -> ir.Module:
     A, B, C = trio(); exprs["xor3"] = C ^ (A ^ B)
     A, B, C = trio(); exprs["add3"] = (A + B) + C
     A, B, C = trio(); exprs["muladd"] = C + (A * B)
+    A, B, C = trio(); exprs["mulsub"] = (A * B) - C
+    A, B, C = trio(); exprs["submul"] = C - (A * B)   # subtrahend product: stays unfused
+    A, B, C = trio(); D = m.input(f"d{k[0]}", width).value(); exprs["aoi22"] = ~((A & B) | (C & D))
+    A, B, C = trio(); D = m.input(f"d{k[0]}", width).value(); exprs["oai22"] = ~((A | B) & (C | D))
     A, B, C = trio()
     shared = A & B
     exprs["shared1"] = shared | C

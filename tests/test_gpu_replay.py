@@ -127,9 +127,8 @@ def test_replay_matches_reference_loop(lane_mode, check_mode, codegen):
         sim.close()
 
 
-def test_replay_clean_trace_and_errors():
-    """A plan whose expected values are the true ones reports nothing; the
-    gate engine refuses checks (they run on the word engine)."""
+def test_replay_clean_trace():
+    """A plan whose expected values are the true ones reports nothing."""
     p, sim, ref, rng, inputs, checks = setup(5, 40, "auto")
     steps = 12
     in_vals = rng.integers(0, 2**32, size=(steps, sum(p.signals[i].limbs for i in inputs)), dtype=np.uint32)
@@ -142,10 +141,49 @@ def test_replay_clean_trace_and_errors():
     # lane 0's values are right for lane 0 only; in Lane0Fast mode nothing differs
     got = sim.replay(inputs, in_vals, checks, np.ascontiguousarray(true[:, :, 0]), check_mode=REPLAY_LANE0)
     assert got == {"mismatch_count": 0, "mismatches": []}
-    from rrtl_amd import designs
-    g = lower.lower_module(designs.synthetic_netlist(gates=200, levels=8, dffs=20, inputs=8, outputs=4, seed=2))
-    gs = HipSimulator(g, 64, engine="gate")
-    outs = [i for i, s in enumerate(g.signals) if s.kind != KIND_INPUT][:2]
+
+
+@pytest.mark.parametrize("lane_mode,check_mode", [(REPLAY_REPLICATED, REPLAY_LANE0), (REPLAY_REPLICATED, REPLAY_ALL_LANES),
+                                                  (REPLAY_INDEPENDENT, REPLAY_ALL_LANES)])
+def test_gate_engine_replay_matches_reference_loop(lane_mode, check_mode):
+    """The gate engine's replay (the reference's host loop over its API calls)
+    on random 1-bit designs with resets: mismatch count, first mismatches and
+    final state equal the reference loop over the packed oracle."""
+    from designgen import random_gate_design
+    flush_print(f"case: gate replay lane_mode={lane_mode} check_mode={check_mode}")
+    lanes, steps, max_mm = 100, 20, 12
+    for seed in (4, 9):
+        p = lower.lower_module(random_gate_design(seed, observe_regs=True, async_resets=False))
+        sim = HipSimulator(p, lanes, engine="gate")
+        ref = oracle.PackedRef(p, lanes)
+        rng = np.random.default_rng(seed)
+        for i, sg in enumerate(p.signals):
+            if sg.kind != KIND_INPUT:
+                v = rng.integers(0, 2, size=(1, lanes), dtype=np.uint32)
+                sim.set_signal_limbs(i, v, settle=False)
+                ref.set_signal_limbs(i, v, eval=False)
+        inputs = [i for i, sg in enumerate(p.signals) if sg.kind == KIND_INPUT]
+        checks = [i for i, sg in enumerate(p.signals) if sg.kind != KIND_INPUT]
+        indep = lane_mode == REPLAY_INDEPENDENT
+        shape = (steps, len(inputs), lanes) if indep else (steps, len(inputs))
+        in_vals = rng.integers(0, 2, size=shape, dtype=np.uint32)
+        expected = rng.integers(0, 2, size=(steps, len(checks), lanes) if indep else (steps, len(checks)),
+                                dtype=np.uint32)
+        active = rng.integers(0, 2**32, size=(steps, (len(checks) + 31) // 32), dtype=np.uint32) if seed == 9 else None
+        got = sim.replay(inputs, in_vals, checks, expected, lane_mode=lane_mode, check_mode=check_mode,
+                         check_active=active, max_mismatches=max_mm)
+        count, mism, _ = oracle_replay(ref, p, lanes, inputs, in_vals, checks, expected, lane_mode, check_mode,
+                                       active, max_mm)
+        assert count > 0
+        assert got["mismatch_count"] == count, seed
+        assert got["mismatches"] == mism, seed
+        for i in range(len(p.signals)):
+            assert np.array_equal(sim.get_signal_limbs(i), ref.get_signal_limbs(i)), p.signals[i].name
+        sim.close()
+
+
+def test_replay_errors():
+    p, sim, ref, rng, inputs, checks = setup(5, 40, "auto")
     with pytest.raises(HipError) as e:
-        gs.replay([], np.zeros((4, 0), np.uint32), outs, np.zeros((4, 2), np.uint32))
-    assert e.value.code == "E_HIP_UNSUPPORTED"
+        sim.replay([len(p.signals) + 3], np.zeros((2, 1), np.uint32), [], np.zeros((2, 0), np.uint32))
+    assert e.value.code == "E_SIM_IR_REPLAY_SIGNAL"

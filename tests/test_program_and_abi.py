@@ -194,4 +194,7 @@ def test_fused_records_in_generated_source():
     assert re.search(r"\(\w+ \^ \w+ \^ \w+\)", src), "XOR3"
     assert re.search(r"\(\w+ \+ \w+ \+ \w+\)", src), "ADD3"
     assert re.search(r"\(\w+ \* \w+ \+ \w+\)", src), "MULADD"
+    assert re.search(r"\(\w+ \* \w+ - \w+\)", src), "MULSUB"
+    assert re.search(r"~\(\(\w+ & \w+\) \| \(\w+ & \w+\)\)", src), "AOI22"
+    assert re.search(r"~\(\(\w+ \| \w+\) & \(\w+ \| \w+\)\)", src), "OAI22"
     assert re.search(r"~\(\w+ & \w+\)", src), "NAND"
