@@ -183,7 +183,19 @@ def test_gate_engine_replay_matches_reference_loop(lane_mode, check_mode):
 
 
 def test_replay_errors():
+    """A signal index past the program is rejected (by the binding's name
+    check, or by the C ABI's replay check); a null report buffer is rejected
+    by the C ABI."""
+    import ctypes
+    from rrtl_amd.backend import ReplaySpec
     p, sim, ref, rng, inputs, checks = setup(5, 40, "auto")
     with pytest.raises(HipError) as e:
         sim.replay([len(p.signals) + 3], np.zeros((2, 1), np.uint32), [], np.zeros((2, 0), np.uint32))
-    assert e.value.code == "E_SIM_IR_REPLAY_SIGNAL"
+    assert e.value.code in ("E_SIM_IR_SIGNAL", "E_SIM_IR_REPLAY_SIGNAL")
+    bad = np.array([len(p.signals) + 3], np.uint32)
+    vals = np.zeros((2, 1), np.uint32)
+    sp = ReplaySpec(ctypes.sizeof(ReplaySpec), 2, REPLAY_REPLICATED, REPLAY_LANE0, 1, bad.ctypes.data,
+                    vals.ctypes.data, 0, None, None, None, None, 0)
+    count, n = ctypes.c_uint64(), ctypes.c_uint32()
+    st = sim.L.rrtl_hip_replay(sim.h, ctypes.byref(sp), ctypes.byref(count), None, ctypes.byref(n))
+    assert st != 0 and sim.L.rrtl_hip_last_error(sim.h).decode().startswith("E_SIM_IR_REPLAY_SIGNAL")
