@@ -10,8 +10,10 @@ metric.
 
 Knobs: the gate engine's tile width (u32 lane-words per workgroup, i.e. how
 many lanes share one LDS-resident copy of the live gate values, which sets
-LDS per workgroup and so workgroups per CU) and the cycles per launch; the
-word engine's workgroup size, active lanes per wave and cycles per launch.
+LDS per workgroup and so workgroups per CU), gate positions per thread per
+stage and cycles per launch; the word engine's workgroup size, active lanes
+per wave (lanes per thread below one wave's width), cycles per launch and,
+for runs that halt, lane compaction between launches.
 """
 from __future__ import annotations
 
@@ -33,23 +35,31 @@ class HipAutotuneRecommendation:
     block_threads: int = 0
     ticks_per_launch: int = 0
     wave_lanes: int = 0
+    stage_gates: int = 0
+    compact_lanes: bool = False
     autotune_metric: str = "kernel_ns_per_lane_cycle"
     autotune_metric_ns: Optional[float] = None
     candidates: List[dict] = field(default_factory=list)
 
     def options(self) -> dict:
         return {"engine": self.engine, "tile_words": self.tile_words, "block_threads": self.block_threads,
-                "ticks_per_launch": self.ticks_per_launch, "wave_lanes": self.wave_lanes}
+                "ticks_per_launch": self.ticks_per_launch, "wave_lanes": self.wave_lanes,
+                "stage_gates": self.stage_gates, "compact_lanes": self.compact_lanes}
 
 
-def _candidates(engine: str, steps: int, probe: int) -> List[dict]:
+def _candidates(engine: str, steps: int, probe: int, halting: bool = False) -> List[dict]:
     # a cycles-per-launch candidate only differs from one launch when the probe
     # spans at least two of its launches
     tpl = [0] + [t for t in (2048,) if t < steps and 2 * t <= probe]
     if engine == "gate":
-        return [{"tile_words": w, "ticks_per_launch": t} for w in (1, 2, 4, 8) for t in tpl]
-    return ([{"block_threads": b, "wave_lanes": 64, "ticks_per_launch": t} for b in (64, 128, 256) for t in tpl] +
-            [{"block_threads": 64, "wave_lanes": w, "ticks_per_launch": t} for w in (8, 16, 32) for t in tpl])
+        return [{"tile_words": w, "stage_gates": m, "ticks_per_launch": t}
+                for w in (1, 2, 4, 8) for m in (2, 3, 4) for t in tpl]
+    out = ([{"block_threads": b, "wave_lanes": 64, "ticks_per_launch": t} for b in (64, 128, 256) for t in tpl] +
+           [{"block_threads": 64, "wave_lanes": w, "ticks_per_launch": t} for w in (8, 16, 32) for t in tpl])
+    if halting:
+        ct = [t for t in (256, 1024) if 2 * t <= probe] or [max(1, probe // 2)]
+        out += [{"block_threads": 0, "wave_lanes": 0, "ticks_per_launch": t, "compact_lanes": True} for t in ct]
+    return out
 
 
 def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = "auto",
@@ -64,7 +74,7 @@ def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = 
     run_kwargs = dict(run_kwargs or {})
     results = []
     best = None
-    for cand in _candidates(engine, steps, probe):
+    for cand in _candidates(engine, steps, probe, halting=run_kwargs.get("halt") is not None):
         try:
             sim = HipSimulator(program, lanes, engine=engine, device=device, **cand)
             if setup:
@@ -93,6 +103,8 @@ def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = 
                                      block_threads=best[0].get("block_threads", 0),
                                      ticks_per_launch=best[0].get("ticks_per_launch", 0),
                                      wave_lanes=best[0].get("wave_lanes", 0),
+                                     stage_gates=best[0].get("stage_gates", 0),
+                                     compact_lanes=bool(best[0].get("compact_lanes", False)),
                                      autotune_metric_ns=best[1], candidates=results)
 
 

@@ -1166,8 +1166,22 @@ class WordEngine final : public Engine {
     return flags;
   }
 
-  // The run's kernels on stream s (kernels only: no copies, fills or host
-  // waits, so the sequence can be captured into a graph).  Returns the launches.
+  // Slots per XCD partition in a lane map of a compacting run.
+  uint32_t compact_cap() const {
+    const uint32_t per_block = wave_lanes_ < 64 ? wave_lanes_ : block_;
+    const uint32_t grid = (lanes_ + per_block - 1) / per_block;
+    return (grid + kParts - 1) / kParts * per_block;
+  }
+  // The two lane maps and live counts of a compacting run, allocated ahead of
+  // enqueue() (whose launches a batch captures into a graph).
+  void reserve_compaction() {
+    if (!opt_.compact_lanes) return;
+    lane_map_.alloc(size_t(compact_cap()) * kParts * 2);
+    live_.alloc(kParts * 2);
+  }
+
+  // The run's kernels on stream s (kernels only: no copies, fills, allocations
+  // or host waits, so the sequence can be captured into a graph).  Returns the launches.
   uint64_t enqueue(const WordArgs& A, uint64_t cycles, bool outputs, hipStream_t s) {
     // lane compaction: halting runs cut into launches, live lanes packed between them
     const bool compact = opt_.compact_lanes && outputs && A.halt_offset != NONE && A.halt_cycles;
@@ -1176,10 +1190,11 @@ class WordEngine final : public Engine {
     const uint32_t per_block = wave_lanes_ < 64 ? wave_lanes_ : block_;
     const uint32_t grid = (lanes_ + per_block - 1) / per_block;
     const size_t lds = lds_bytes(mode_, block_, run_slots_, lds_mem_words_);
-    const uint32_t cap = (grid + kParts - 1) / kParts * per_block;  // slots per partition
+    const uint32_t cap = compact_cap();  // slots per partition
     if (compact) {
-      lane_map_.alloc(size_t(cap) * kParts * 2);
-      live_.alloc(kParts * 2);
+      // allocated by reserve_compaction() before any launch is issued: these launches may be captured
+      if (lane_map_.n < size_t(cap) * kParts * 2 || live_.n < kParts * 2)
+        throw Error("E_HIP_INTERNAL: lane maps not reserved");
       hipLaunchKernelGGL(lane_map_init_kernel, dim3(grid), dim3(std::max<uint32_t>(per_block, kParts)), 0, s,
                          lane_map_.p, live_.p, lanes_, per_block, grid, cap);
       RRTL_HIP_CHECK(hipGetLastError());
@@ -1233,6 +1248,7 @@ class WordEngine final : public Engine {
 
   void launch(uint32_t flags, uint64_t cycles, const RunSpec* spec, bool outputs = false) {
     const WordArgs A = make_args(flags, cycles, spec, outputs);
+    reserve_compaction();
     stats_ = RunStats{};
     RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
     stats_.launches = enqueue(A, cycles, outputs, stream_);
@@ -1248,6 +1264,7 @@ class WordEngine final : public Engine {
   void async_prepare(const RunSpec& spec) override {
     async_args_ = make_args(run_flags(spec), spec.cycles, &spec, true);
     async_cycles_ = spec.cycles;
+    reserve_compaction();
     batch_spec_halt_ = has_halt_;
     batch_digest_ = has_digest_;
     RRTL_HIP_CHECK(hipStreamSynchronize(stream_));
