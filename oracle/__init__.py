@@ -26,12 +26,15 @@ from typing import Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BUILD = os.path.join(HERE, "_build")
+# RRTL_ORACLE_SANITIZE=1 loads the engines built with ASan + UBSan (make
+# sanitize); the process must preload the ASan runtime.
+SANITIZE = os.environ.get("RRTL_ORACLE_SANITIZE") == "1"
+BUILD = os.path.join(HERE, "_build", "san") if SANITIZE else os.path.join(HERE, "_build")
 
 
 def build() -> None:
     """Compile the C oracles with the committed Makefile."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-C", HERE] + (["sanitize"] if SANITIZE else []), check=True)
 
 
 def _lib(name: str) -> ctypes.CDLL:

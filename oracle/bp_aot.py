@@ -36,7 +36,8 @@ from rrtl_amd.program import (EF_CAPTURE, EF_STORE, KIND_INPUT, KIND_REG, OP_AND
                               OP_SIGNAL, OP_XOR, POL_LOW, RESET_ASYNC, Program)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BUILD = os.path.join(HERE, "_build")
+SANITIZE = os.environ.get("RRTL_ORACLE_SANITIZE") == "1"   # compile with ASan + UBSan (sanitizer test)
+BUILD = os.path.join(HERE, "_build", "san") if SANITIZE else os.path.join(HERE, "_build")
 IN_RANDOM, IN_CONST, IN_PULSE = 1, 2, 3
 _GATE_OPS = (OP_NOT, OP_AND, OP_OR, OP_XOR, OP_MUX)
 
@@ -258,8 +259,10 @@ class BitParallelAot:
             with open(c, "w") as f:
                 f.write(src)
             tmp = so + f".tmp{os.getpid()}"
-            r = subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-std=gnu11",
-                                "-I", HERE, "-o", tmp, c], capture_output=True, text=True)
+            san = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+                   "-fno-sanitize-recover=undefined"] if SANITIZE else ["-O3"]
+            r = subprocess.run(["gcc"] + san + ["-march=native", "-fopenmp", "-fPIC", "-shared", "-std=gnu11",
+                                                "-I", HERE, "-o", tmp, c], capture_output=True, text=True)
             if r.returncode != 0:
                 raise RuntimeError("E_BITPARALLEL_AOT: gcc failed: " + r.stderr[-2000:])
             os.replace(tmp, so)
