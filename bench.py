@@ -156,7 +156,11 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None, "traffic_detail": traffic,
-                     "b_alg_bytes_per_lane_cycle": B_ALG, "kernel_ms_avg": avg_kernel_s * 1e3},
+                     "b_alg_bytes_per_lane_cycle": B_ALG, "kernel_ms_avg": avg_kernel_s * 1e3,
+                     "note": "B_alg is the notional traffic of streaming every live bit through HBM; the state "
+                             "stays in LDS, so the measured traffic is far lower and the kernel is bound on "
+                             "chip (on_chip_roofs)"},
+        "on_chip_roofs": on_chip_roofs(lane_cycles / avg_kernel_s, info["tile_words"]),
         "cpu_baseline": cpu,
         "check": check,
     }
@@ -239,6 +243,38 @@ def gates_cpu_baseline(p, ins, obs, args, aot_pending, gpu_gates: int) -> dict:
     return {"value": b["value"], "unit": "lane-cycles/s", "cores": threads, "kind": "port", "engine": best,
             "sample": b["sample"], "bit_exact_vs_gpu": all(e["bit_exact_vs_gpu"] for e in engines.values()),
             "gpu_gates_per_cycle": gpu_gates, "engines": engines}
+
+
+LDS_PEAK_BYTES_PER_CLK_CU = 128   # LDS bandwidth per CU per clock (CDNA3 / CDNA4 documentation)
+VALU_WAVE_INSTS_PER_CLK_CU = 1    # four SIMDs, one wave64 VALU instruction each per four clocks
+DEVICE_CUS, DEVICE_CLK_GHZ = 256, 2.4
+
+
+def on_chip_roofs(lane_cycles_per_s: float, tile_words: int):
+    """The units that bound the tile kernel: LDS bytes and VALU wave-instructions
+    per lane-cycle from the committed SQ counters of the same kernel on this
+    workload (profiles/gate_tile_kernel_20k_netlist_sq.json), times the
+    measured rate, against the device's LDS bandwidth and VALU issue rate at
+    2.4 GHz.  None when no such profile exists."""
+    path = os.path.join(ROOT, "profiles", "gate_tile_kernel_20k_netlist_sq.json")
+    if not os.path.exists(path) or tile_words != 2:
+        return None
+    with open(path) as f:
+        m = json.load(f)
+    c, lc = m["counters"], m["lanes"] * m["cycles"]
+    lds_b = c["SQ_INSTS_LDS"] * m["lds_bytes_per_wave_instruction"] / lc
+    valu = c["SQ_INSTS_VALU"] / lc
+    lds_peak = LDS_PEAK_BYTES_PER_CLK_CU * DEVICE_CUS * DEVICE_CLK_GHZ * 1e9
+    valu_peak = VALU_WAVE_INSTS_PER_CLK_CU * DEVICE_CUS * DEVICE_CLK_GHZ * 1e9
+    return {
+        "lds": {"achieved": lds_b * lane_cycles_per_s / 1e12, "peak": lds_peak / 1e12, "unit": "TB/s",
+                "frac": lds_b * lane_cycles_per_s / lds_peak, "bytes_per_lane_cycle": lds_b,
+                "bank_conflict_cycles_per_lane_cycle": c["SQ_LDS_BANK_CONFLICT"] / lc},
+        "valu": {"achieved": valu * lane_cycles_per_s / 1e12, "peak": valu_peak / 1e12,
+                 "unit": "T wave-instructions/s", "frac": valu * lane_cycles_per_s / valu_peak,
+                 "wave_instructions_per_lane_cycle": valu},
+        "profiled": m["workload"],
+    }
 
 
 def measured_traffic(lanes: int, cycles: int, tile_words: int):
