@@ -63,6 +63,8 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=6, help="faultsim: surrogate training epochs")
     ap.add_argument("--no-autotune", action="store_true", help="mixed: default launch shapes instead of the autotuner's")
     ap.add_argument("--skip-persistent", action="store_true", help="mixed: leave out the persistent work-queue kernel")
+    ap.add_argument("--lane-waves", type=int, default=0,
+                    help="rv32i: waves per 64 lanes for the generated kernel (0: automatic, 1: off, 2, 4)")
     ap.add_argument("--codegen", choices=("auto", "on", "off"), default="auto",
                     help="rv32i: generated per-design kernel (auto/on) or the record interpreter (off)")
     args = ap.parse_args()
@@ -357,9 +359,10 @@ def bench_rv32i(args) -> int:
     else:
         job = workloads.rv32i_job(16384, args.cycles, args.seed)
         design = "single-cycle rv32i core"
-    sim = job.make(codegen=args.codegen)
+    sim = job.make(codegen=args.codegen, lane_waves=args.lane_waves)
     sim.run(**dict(job.run, cycles=1))   # compile the run plan
-    sim = job.make(codegen=args.codegen)
+    sim = job.make(codegen=args.codegen, lane_waves=args.lane_waves)
+    launch = sim.launch_info()
     t0 = time.perf_counter()
     sim.run(**job.run)
     sim.synchronize()
@@ -383,7 +386,7 @@ def bench_rv32i(args) -> int:
             "higher_is_better": True, "data": "synthetic (random RV32I programs)", "dtype": "u32",
             "wall_ms": wall * 1e3, "kernel_ms": st["kernel_ms"],
             "config": {"workload": f"{design}, 16384 lanes, 512-instruction random programs, cap {args.cycles}",
-                       "codegen": args.codegen,
+                       "codegen": args.codegen, "launch": launch,
                        "halt_cycles_min": int(halt.min()), "halt_cycles_max": int(halt.max()),
                        "lanes_at_cap": int((halt >= args.cycles).sum())},
             "roofline": {"bound": "valu", "achieved": achieved, "peak": valu_peak, "unit": "Tops/s",

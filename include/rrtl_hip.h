@@ -88,6 +88,16 @@ typedef struct rrtl_hip_options {
    * crates/rrtl-sim-ir/examples/divergent_batch.rs).  Results are identical
    * to an uncompacted run.  0 = off.  Read only when struct_size covers it. */
   uint32_t compact_lanes;
+  /* Word engine with generated code: waves per 64 lanes (1, 2 or 4).  Above
+   * 1, a workgroup of that many waves runs one set of 64 lanes and the
+   * generated code of each evaluation is split between them: every wave runs
+   * its part, values cross between waves through LDS, barriers order the
+   * parts.  This fills the SIMDs of a batch too small to give each SIMD a
+   * 64-lane wave of its own (several threads per lane; lanes per thread below
+   * one).  0 = automatic: 4 (else 2) when the batch gives fewer than one
+   * 64-lane wave per SIMD and the split is estimated to pay; 1 = off.
+   * Results are identical.  Read only when struct_size covers it. */
+  uint32_t lane_waves;
 } rrtl_hip_options;
 
 enum { RRTL_CODEGEN_AUTO = 0, RRTL_CODEGEN_ON = 1, RRTL_CODEGEN_OFF = 2 };
@@ -193,6 +203,10 @@ rrtl_status rrtl_hip_get_halt_cycles(rrtl_hip_engine* e, uint64_t* out);
 rrtl_status rrtl_hip_get_detect(rrtl_hip_engine* e, uint32_t* out_words);
 /* Gate engine: [tile_words, live_slots, levels, gates] of the last compiled run. */
 rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4);
+/* Word engine: [generated code (1) or interpreter (0), waves per 64 lanes,
+ * active lanes per wave, threads per workgroup].  No counterpart in the
+ * reference: the launch shape the engine chose, for tuning and tests. */
+rrtl_status rrtl_hip_word_launch_info(rrtl_hip_engine* e, uint32_t* out4);
 
 /* ---- trace replay (SimBackend::replay_trace, crates/rrtl-sim-ir/src/lib.rs:5270-5345,
  * with ReplayOptions / ReplayReport :3463-3491, :4328-4407) moved on chip: per
@@ -244,6 +258,13 @@ rrtl_status rrtl_hip_replay(rrtl_hip_engine* e, const rrtl_hip_replay_spec* spec
  * 0 to size the buffer.  Needs no device. */
 rrtl_status rrtl_hip_emit_kernel(const void* program_blob, size_t blob_bytes, uint32_t storage, uint32_t block_threads,
                                  char* out, size_t cap, size_t* len);
+/* The generated kernel split between `lane_waves` (2 or 4) waves per 64
+ * lanes, as the engine builds it for rrtl_hip_options.lane_waves; the first
+ * line is a comment with the split's estimated speed-up over one wave, its
+ * barrier phases and exchange rows.  E_HIP_OPTIONS when the design's code
+ * cannot be split.  No counterpart in the reference.  Needs no device. */
+rrtl_status rrtl_hip_emit_split_kernel(const void* program_blob, size_t blob_bytes, uint32_t lane_waves, char* out,
+                                       size_t cap, size_t* len);
 /* Compiles HIP source for gfx950 at run time (hiprtc) against the library's
  * device headers, as the word engine does for its generated kernels; sets
  * *code_bytes to the code object's size.  Needs no device.  On failure the

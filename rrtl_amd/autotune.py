@@ -12,8 +12,11 @@ Knobs: the gate engine's tile width (u32 lane-words per workgroup, i.e. how
 many lanes share one LDS-resident copy of the live gate values, which sets
 LDS per workgroup and so workgroups per CU), gate positions per thread per
 stage and cycles per launch; the word engine's workgroup size, active lanes
-per wave (lanes per thread below one wave's width), cycles per launch and,
-for runs that halt, lane compaction between launches.
+per wave (lanes per thread below one wave's width), waves per 64 lanes (the
+generated code split between several threads per lane), cycles per launch
+and, for runs that halt, lane compaction between launches.  Shapes a design
+cannot take (a split of code with interpreted records) are recorded with
+their error and skipped.
 """
 from __future__ import annotations
 
@@ -37,6 +40,7 @@ class HipAutotuneRecommendation:
     wave_lanes: int = 0
     stage_gates: int = 0
     compact_lanes: bool = False
+    lane_waves: int = 0
     autotune_metric: str = "kernel_ns_per_lane_cycle"
     autotune_metric_ns: Optional[float] = None
     candidates: List[dict] = field(default_factory=list)
@@ -44,7 +48,7 @@ class HipAutotuneRecommendation:
     def options(self) -> dict:
         return {"engine": self.engine, "tile_words": self.tile_words, "block_threads": self.block_threads,
                 "ticks_per_launch": self.ticks_per_launch, "wave_lanes": self.wave_lanes,
-                "stage_gates": self.stage_gates, "compact_lanes": self.compact_lanes}
+                "stage_gates": self.stage_gates, "compact_lanes": self.compact_lanes, "lane_waves": self.lane_waves}
 
 
 def _candidates(engine: str, steps: int, probe: int, halting: bool = False) -> List[dict]:
@@ -54,11 +58,15 @@ def _candidates(engine: str, steps: int, probe: int, halting: bool = False) -> L
     if engine == "gate":
         return [{"tile_words": w, "stage_gates": m, "ticks_per_launch": t}
                 for w in (1, 2, 4, 8) for m in (2, 3, 4) for t in tpl]
-    out = ([{"block_threads": b, "wave_lanes": 64, "ticks_per_launch": t} for b in (64, 128, 256) for t in tpl] +
-           [{"block_threads": 64, "wave_lanes": w, "ticks_per_launch": t} for w in (8, 16, 32) for t in tpl])
+    out = ([{"block_threads": b, "wave_lanes": 64, "lane_waves": 1, "ticks_per_launch": t}
+            for b in (64, 128, 256) for t in tpl] +
+           [{"block_threads": 64, "wave_lanes": w, "lane_waves": 1, "ticks_per_launch": t}
+            for w in (8, 16, 32) for t in tpl] +
+           [{"lane_waves": g, "ticks_per_launch": t} for g in (2, 4) for t in tpl])
     if halting:
         ct = [t for t in (256, 1024) if 2 * t <= probe] or [max(1, probe // 2)]
         out += [{"block_threads": 0, "wave_lanes": 0, "ticks_per_launch": t, "compact_lanes": True} for t in ct]
+        out += [{"lane_waves": g, "ticks_per_launch": t, "compact_lanes": True} for g in (2, 4) for t in ct]
     return out
 
 
@@ -105,6 +113,7 @@ def autotune(program: Program, lanes: int, steps: int, case: str, engine: str = 
                                      wave_lanes=best[0].get("wave_lanes", 0),
                                      stage_gates=best[0].get("stage_gates", 0),
                                      compact_lanes=bool(best[0].get("compact_lanes", False)),
+                                     lane_waves=best[0].get("lane_waves", 0),
                                      autotune_metric_ns=best[1], candidates=results)
 
 

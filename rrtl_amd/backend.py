@@ -44,8 +44,8 @@ EXPORTS = [
     "rrtl_hip_synchronize", "rrtl_hip_run", "rrtl_hip_last_run_stats", "rrtl_hip_digest_bytes",
     "rrtl_hip_get_digest", "rrtl_hip_trace_bytes", "rrtl_hip_get_trace", "rrtl_hip_get_halt_cycles",
     "rrtl_hip_get_detect",
-    "rrtl_hip_gate_tile_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
-    "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy", "rrtl_hip_emit_kernel", "rrtl_hip_compile_kernel",
+    "rrtl_hip_gate_tile_info", "rrtl_hip_word_launch_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
+    "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy", "rrtl_hip_emit_kernel", "rrtl_hip_emit_split_kernel", "rrtl_hip_compile_kernel",
     "rrtl_hip_replay",
 ]
 BATCH_MODES = {"persistent": 0, "per_design": 1, "concurrent": 2, "streams": 3}
@@ -76,7 +76,7 @@ class Options(ctypes.Structure):
                 ("block_threads", ctypes.c_uint32), ("tile_words", ctypes.c_uint32),
                 ("ticks_per_launch", ctypes.c_uint32), ("tile_chunks", ctypes.c_uint32),
                 ("stage_gates", ctypes.c_uint32), ("codegen", ctypes.c_uint32), ("wave_lanes", ctypes.c_uint32),
-                ("compact_lanes", ctypes.c_uint32)]
+                ("compact_lanes", ctypes.c_uint32), ("lane_waves", ctypes.c_uint32)]
 
 
 class RunSpec(ctypes.Structure):
@@ -134,6 +134,7 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_get_trace.argtypes = [vp, vp]
     L.rrtl_hip_get_halt_cycles.argtypes = [vp, vp]
     L.rrtl_hip_gate_tile_info.argtypes = [vp, vp]
+    L.rrtl_hip_word_launch_info.argtypes = [vp, vp]
     L.rrtl_hip_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(RunSpec), u32, ctypes.POINTER(vp)]
     L.rrtl_hip_batch_run.argtypes = [vp, u32]
     L.rrtl_hip_batch_stats.argtypes = [vp, ctypes.POINTER(RunStats)]
@@ -141,6 +142,7 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_batch_last_error.argtypes = [vp]
     L.rrtl_hip_batch_destroy.argtypes = [vp]
     L.rrtl_hip_emit_kernel.argtypes = [ctypes.c_char_p, sz, u32, u32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+    L.rrtl_hip_emit_split_kernel.argtypes = [ctypes.c_char_p, sz, u32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
     L.rrtl_hip_compile_kernel.argtypes = [ctypes.c_char_p, ctypes.POINTER(sz)]
     L.rrtl_hip_replay.argtypes = [vp, ctypes.POINTER(ReplaySpec), ctypes.POINTER(u64), ctypes.POINTER(ReplayMismatch),
                                   ctypes.POINTER(u32)]
@@ -179,6 +181,23 @@ def emit_kernel(program: Program, storage: int = STORAGE_ALL_LDS, block_threads:
     return buf.value.decode()
 
 
+def emit_split_kernel(program: Program, lane_waves: int) -> str:
+    """The generated kernel split between ``lane_waves`` waves per 64 lanes
+    (first line: estimated speed-up, barrier phases, exchange rows).  Needs no
+    GPU."""
+    L = load_library()
+    blob = program.to_blob()
+    n = ctypes.c_size_t()
+    st = L.rrtl_hip_emit_split_kernel(blob, len(blob), lane_waves, None, 0, ctypes.byref(n))
+    if st != 0:
+        raise HipError(st, L.rrtl_hip_create_error().decode())
+    buf = ctypes.create_string_buffer(n.value + 1)
+    st = L.rrtl_hip_emit_split_kernel(blob, len(blob), lane_waves, buf, n.value + 1, ctypes.byref(n))
+    if st != 0:
+        raise HipError(st, L.rrtl_hip_create_error().decode())
+    return buf.value.decode()
+
+
 def compile_kernel(source: str) -> int:
     """Compiles HIP source for gfx950 with the library's run-time compiler;
     returns the code object's size.  Needs no GPU."""
@@ -195,13 +214,14 @@ class HipSimulator:
 
     def __init__(self, program: Program, lanes: int, engine: str = "auto", device: int = 0,
                  tile_words: int = 0, ticks_per_launch: int = 0, block_threads: int = 0, tile_chunks: int = 0,
-                 stage_gates: int = 0, codegen: str = "auto", wave_lanes: int = 0, compact_lanes: bool = False):
+                 stage_gates: int = 0, codegen: str = "auto", wave_lanes: int = 0, compact_lanes: bool = False,
+                 lane_waves: int = 0):
         self.L = load_library()
         self.program = program
         blob = program.to_blob()
         opt = Options(ctypes.sizeof(Options), ENGINE_NAMES[engine], device, block_threads, tile_words,
                       ticks_per_launch, tile_chunks, stage_gates, CODEGEN_NAMES[codegen], wave_lanes,
-                      int(bool(compact_lanes)))
+                      int(bool(compact_lanes)), lane_waves)
         h = ctypes.c_void_p()
         st = self.L.rrtl_hip_create(blob, len(blob), lanes, ctypes.byref(opt), ctypes.byref(h))
         if st != 0:
@@ -438,6 +458,13 @@ class HipSimulator:
         out = np.zeros(self.lanes, dtype=np.uint64)
         self._check(self.L.rrtl_hip_get_halt_cycles(self.h, _ptr(out)))
         return out
+
+    def launch_info(self) -> dict:
+        """Word engine: the launch shape it chose."""
+        out = np.zeros(4, dtype=np.uint32)
+        self._check(self.L.rrtl_hip_word_launch_info(self.h, _ptr(out)))
+        return {"generated": bool(out[0]), "lane_waves": int(out[1]), "wave_lanes": int(out[2]),
+                "block_threads": int(out[3])}
 
     def tile_info(self) -> dict:
         out = np.zeros(4, dtype=np.uint32)

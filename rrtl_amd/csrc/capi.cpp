@@ -12,12 +12,14 @@
 namespace rrtl {
 void gate_engine_get_words(Engine* e, uint32_t sig, uint32_t* out);
 void gate_engine_tile_info(Engine* e, uint32_t* info);
+void word_engine_launch_info(Engine* e, uint32_t* info);
 class Batch;
 Batch* make_batch(std::vector<Engine*> engines, std::vector<RunSpec> specs);
 void batch_run(Batch* b, uint32_t mode);
 const RunStats& batch_stats(const Batch* b);
 void batch_destroy(Batch* b);
 std::string word_kernel_source(const Program& prog, int mode, uint32_t block);
+std::string word_split_kernel_source(const Program& prog, uint32_t lane_waves);
 }
 
 struct rrtl_hip_batch {
@@ -108,7 +110,11 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
         opt.wave_lanes = options->wave_lanes;
       if (options->struct_size >= offsetof(rrtl_hip_options, compact_lanes) + sizeof(uint32_t))
         opt.compact_lanes = options->compact_lanes;
+      if (options->struct_size >= offsetof(rrtl_hip_options, lane_waves) + sizeof(uint32_t))
+        opt.lane_waves = options->lane_waves;
     }
+    if (opt.lane_waves > 4 || opt.lane_waves == 3)
+      throw rrtl::Error("E_HIP_OPTIONS: lane_waves must be 0, 1, 2 or 4");
     if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 256))
       throw rrtl::Error("E_HIP_OPTIONS: block_threads must be 64, 128, 192 or 256");
     if (opt.tile_words && opt.tile_words != 1 && opt.tile_words != 2 && opt.tile_words != 4 && opt.tile_words != 8)
@@ -132,6 +138,28 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
     *out = h;
     g_create_err.clear();
     return RRTL_OK;
+  } catch (const std::exception& ex) {
+    g_create_err = ex.what();
+    return classify(g_create_err);
+  }
+}
+
+static rrtl_status emit_source(const std::string& src, char* out, size_t cap, size_t* len) {
+  *len = src.size();
+  if (cap) {
+    const size_t n = std::min(cap - 1, src.size());
+    std::memcpy(out, src.data(), n);
+    out[n] = '\0';
+  }
+  g_create_err.clear();
+  return RRTL_OK;
+}
+
+rrtl_status rrtl_hip_emit_split_kernel(const void* blob, size_t bytes, uint32_t lane_waves, char* out, size_t cap,
+                                       size_t* len) {
+  try {
+    if (!blob || !len || (cap && !out)) throw rrtl::Error("E_HIP_NULL: null argument");
+    return emit_source(rrtl::word_split_kernel_source(rrtl::Program::parse(blob, bytes), lane_waves), out, cap, len);
   } catch (const std::exception& ex) {
     g_create_err = ex.what();
     return classify(g_create_err);
@@ -391,6 +419,14 @@ rrtl_status rrtl_hip_gate_tile_info(rrtl_hip_engine* e, uint32_t* out4) {
     if (!out4) throw rrtl::Error("E_HIP_NULL: null buffer");
     if (x.kind() != RRTL_HIP_GATE) throw rrtl::Error("E_HIP_UNSUPPORTED: not a gate engine");
     rrtl::gate_engine_tile_info(&x, out4);
+  });
+}
+
+rrtl_status rrtl_hip_word_launch_info(rrtl_hip_engine* e, uint32_t* out4) {
+  return guard(e, [&](rrtl::Engine& x) {
+    if (!out4) throw rrtl::Error("E_HIP_NULL: null buffer");
+    if (x.kind() != RRTL_HIP_WORD) throw rrtl::Error("E_HIP_UNSUPPORTED: not a word engine");
+    rrtl::word_engine_launch_info(&x, out4);
   });
 }
 

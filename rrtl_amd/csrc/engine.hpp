@@ -22,6 +22,7 @@ struct EngineOptions {
   uint32_t codegen = 0;            // word engine: CODEGEN_AUTO / CODEGEN_ON (generated kernel) or CODEGEN_OFF (interpreter)
   uint32_t wave_lanes = 0;         // word engine: active lanes per 64-thread wave (16, 32, 64; 0: automatic)
   uint32_t compact_lanes = 0;      // word engine: pack the live lanes of halting runs between launches
+  uint32_t lane_waves = 0;         // word engine, generated code: waves per 64 lanes (0: automatic, 1, 2, 4)
 };
 enum : uint32_t { CODEGEN_AUTO = 0, CODEGEN_ON = 1, CODEGEN_OFF = 2 };
 

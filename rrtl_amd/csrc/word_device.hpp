@@ -102,6 +102,7 @@ struct Ctx {
   uint32_t lanes, lane;
   uint32_t* scr; uint32_t sstride;   // HBM scratch (WM_HBM)
   uint32_t col, stride, sig0, mem0;  // LDS: thread column, block stride, first signal / memory row offsets
+  uint32_t wave;                     // which of the waves sharing these lanes (generated multi-wave kernels)
   __device__ __forceinline__ uint32_t ld(uint32_t code, uint32_t limb) const {
     const uint32_t idx = code & ~SP_MASK;
     switch (code & SP_MASK) {
@@ -519,7 +520,7 @@ __device__ __forceinline__ void observe(const WordArgs& A, const Cx& C, uint64_t
     uint32_t diff = 0, act[4] = {0, 0, 0, 0};
     for (uint32_t l = 0; l < limbs; l++, k++) {
       const uint32_t v = C.ld(SP_SIGNAL | off, l);
-      if (A.trace) A.trace[(size_t(c) * A.n_obs_limbs + k) * A.lanes + C.lane] = v;
+      if (A.trace && C.wave == 0) A.trace[(size_t(c) * A.n_obs_limbs + k) * A.lanes + C.lane] = v;
       if (A.digest) d = lane_digest(d, v, j, l);
       if (check) {
         const uint32_t e = A.expect_per_lane ? A.expect[(size_t(c) * A.n_obs_limbs + k) * A.lanes + C.lane]
@@ -532,7 +533,7 @@ __device__ __forceinline__ void observe(const WordArgs& A, const Cx& C, uint64_t
       }
     }
     if (check && diff && (!A.check_active || ((A.check_active[size_t(c) * A.check_words + j / 32] >> (j % 32)) & 1u))) {
-      if (mm < A.mm_max) {
+      if (mm < A.mm_max && C.wave == 0) {
         uint32_t* r = A.mm_rec + (size_t(C.lane) * A.mm_max + mm) * 6;
         r[0] = uint32_t(A.cycle_base + c); r[1] = j;
         r[2] = act[0]; r[3] = act[1]; r[4] = act[2]; r[5] = act[3];
@@ -548,14 +549,21 @@ __device__ __forceinline__ void observe(const WordArgs& A, const Cx& C, uint64_t
 // observation points through a single inlined copy of the interpreter.
 constexpr uint32_t STEP_OBS = 15;  // sequence entry: observe (else a block id)
 
-template <int kMode, class Exec = InterpExec>
+// kWaves > 1 (generated kernels only): a workgroup of kWaves waves runs one
+// set of 64 lanes, every wave on the same LDS columns.  The generated code
+// splits each evaluation between the waves (the wave index selects its part,
+// barriers order the parts); the rest of the step -- stimulus, observation,
+// halting -- runs in every wave alike, so its LDS writes are the same values
+// and its decisions agree.  Only wave 0 writes results back to HBM.
+template <int kMode, class Exec = InterpExec, int kWaves = 1>
 __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
-  const uint32_t per_block = A.block_lanes ? A.block_lanes : blockDim.x;
-  const uint32_t slot = block * per_block + threadIdx.x;
-  if (threadIdx.x >= per_block) return;
+  const uint32_t col = kWaves > 1 ? threadIdx.x & 63u : threadIdx.x;
+  const uint32_t per_block = kWaves > 1 ? 64u : (A.block_lanes ? A.block_lanes : blockDim.x);
+  const uint32_t slot = block * per_block + col;
+  if (col >= per_block) return;
   uint32_t lane = slot;
   if (A.lane_map) {
-    const uint32_t part = block % 8u, j = (block / 8u) * per_block + threadIdx.x;
+    const uint32_t part = block % 8u, j = (block / 8u) * per_block + col;
     if (j >= A.live[part]) return;
     lane = A.lane_map[size_t(part) * A.map_cap + j];
   }
@@ -564,14 +572,19 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
   C.values = A.values; C.mems = A.mems; C.cpool = A.cpool; C.mem_table = A.mem_table;
   C.lanes = A.lanes; C.lane = lane;
   C.scr = A.scratch + lane; C.sstride = A.lanes;
-  C.col = threadIdx.x; C.stride = blockDim.x; C.sig0 = A.n_scratch * blockDim.x;
-  C.mem0 = C.sig0 + A.n_sig_words * blockDim.x;
+  const uint32_t stride = kWaves > 1 ? 64u : blockDim.x;
+  C.col = col; C.stride = stride; C.sig0 = A.n_scratch * stride;
+  C.mem0 = C.sig0 + A.n_sig_words * stride;
+  C.wave = kWaves > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0u;
+  const bool primary = C.wave == 0;
   if constexpr (kMode == WM_ALL_LDS)
     for (uint32_t w = 0; w < A.n_sig_words; w++) lds_words()[C.sig0 + w * C.stride + C.col] = A.values[size_t(w) * A.lanes + lane];
   Exec::enter(C);
   // inputs that hold one value all run: the design never writes an input, so
   // once per launch is the same as every cycle
   if (A.n_in_once) apply_stimulus(A, C, 0, A.cycle_base, 0, A.n_in_once);
+  // every wave's copies are in LDS before any part of an evaluation writes state
+  if constexpr (kWaves > 1) __syncthreads();
   uint64_t d = A.digest ? A.digest[lane] : 0;
   uint32_t mm = A.mm_count ? A.mm_count[lane] : 0;
   uint64_t done = 0;
@@ -616,6 +629,7 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
       if (A.halt_offset != NONE && (C.ld(SP_SIGNAL | A.halt_offset, 0) & 1u)) halted = true;
     }
   }
+  if (!primary) return;
   Exec::leave(C);
   if constexpr (kMode == WM_ALL_LDS)
     for (uint32_t w = 0; w < A.n_sig_words; w++) A.values[size_t(w) * A.lanes + lane] = lds_words()[C.sig0 + w * C.stride + C.col];

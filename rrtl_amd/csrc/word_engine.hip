@@ -24,12 +24,16 @@
 #include "word_device.hpp"
 #include "rtc.hpp"
 #include <algorithm>
+#include <cctype>
 #include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <numeric>
 #include <tuple>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <unordered_set>
 #include <unordered_map>
 
 namespace rrtl {
@@ -361,28 +365,59 @@ class WordJit {
   explicit WordJit(const WordProgram& WP, JitLayout lay = {}) : WP(WP), lay_(std::move(lay)) {
     mem_row_ = lay_.mem_row;
     mem_row_.resize(WP.mem_table.size() / 4, -1);
-    std::string eval, edge;
-    begin(eval); emit_block(B_COMB); scr_.clear(); emit_block(B_ASYNC);
-    begin(edge); emit_block(B_NEXT); scr_.clear(); emit_block(B_COMMIT);
-    body_ = "namespace rrtl {\nnamespace word {\nnamespace jit {\n\n"
-            "template <class Cx>\n__device__ __forceinline__ void jit_eval(const Cx& C) {\n" + eval + "}\n\n"
-            "template <class Cx>\n__device__ __forceinline__ void jit_edge(const Cx& C) {\n" + edge + "}\n\n"
-            "struct JitExec {\n"
-            "  template <class Cx>\n"
-            "  __device__ __forceinline__ static void run(const WordArgs&, uint32_t id, const Cx& C) {\n"
-            "    if (id == B_COMB) jit_eval(C);\n"
-            "    else if (id == B_COMMIT) jit_edge(C);\n"
-            "  }\n" + staging() +
-            "};\n\n}  // namespace jit\n}  // namespace word\n}  // namespace rrtl\n";
+    begin(fn_[0]); emit_block(B_COMB); scr_.clear(); emit_block(B_ASYNC);
+    begin(fn_[1]); emit_block(B_NEXT); scr_.clear(); emit_block(B_COMMIT);
   }
   // Scratch slots the generated code needs (only records kept in interpreted form use them).
   bool uses_scratch() const { return generic_; }
   uint32_t generic_records() const { return n_generic_; }
-  // Translation unit with the kernel "rrtl_word_jit" for one storage mode.
-  std::string source(int mode) const {
-    return "#include \"word_device.hpp\"\n\n" + body_ +
+  // The evaluations split between `waves` waves of one lane set (WM_ALL_LDS
+  // with 64-lane columns); empty plans when the code cannot be split.
+  struct Split {
+    bool ok = false;
+    uint32_t phases = 0, xslots = 0;     // barrier phases, exchange rows
+    double est = 0, serial = 0;          // estimated time of the split and of one wave (cost units)
+    std::vector<int32_t> wave, phase;    // per statement (-1: a signal load, placed per wave)
+    std::vector<std::vector<int32_t>> load_phase;  // [wave][statement]: phase of a signal load in that wave, or -1
+    std::vector<int32_t> xslot;          // per statement: exchange row of an exported value, or -1
+  };
+  // Plans for both evaluations; estimated speed-up over one wave.
+  double plan_waves(uint32_t waves) {
+    double t1 = 0, tg = 0;
+    xslots_ = 0;
+    for (int f = 0; f < 2; f++) {
+      split_[f] = split(fn_[f], waves);
+      if (!split_[f].ok) return 0;
+      t1 += split_[f].serial; tg += split_[f].est;
+      xslots_ = std::max(xslots_, split_[f].xslots);
+    }
+    waves_ = waves;
+    return tg > 0 ? t1 / tg : 0;
+  }
+  uint32_t exchange_rows() const { return xslots_; }
+  uint32_t phases(int f) const { return split_[f].phases; }
+  // Translation unit with the kernel "rrtl_word_jit" for one storage mode;
+  // after plan_waves(G) the kernel runs G waves per 64 lanes, exchanging
+  // values through LDS rows from xrow0 on.
+  std::string source(int mode, uint32_t xrow0 = 0) const {
+    const uint32_t G = waves_;
+    std::string fn[2];
+    for (int f = 0; f < 2; f++) fn[f] = G > 1 ? emit_split(fn_[f], split_[f], G, xrow0) : emit_serial(fn_[f]);
+    const std::string body =
+        "namespace rrtl {\nnamespace word {\nnamespace jit {\n\n"
+        "template <class Cx>\n__device__ __forceinline__ void jit_eval(const Cx& C) {\n" + fn[0] + "}\n\n"
+        "template <class Cx>\n__device__ __forceinline__ void jit_edge(const Cx& C) {\n" + fn[1] + "}\n\n"
+        "struct JitExec {\n"
+        "  template <class Cx>\n"
+        "  __device__ __forceinline__ static void run(const WordArgs&, uint32_t id, const Cx& C) {\n"
+        "    if (id == B_COMB) jit_eval(C);\n"
+        "    else if (id == B_COMMIT) jit_edge(C);\n"
+        "  }\n" + staging() +
+        "};\n\n}  // namespace jit\n}  // namespace word\n}  // namespace rrtl\n";
+    return "#include \"word_device.hpp\"\n\n" + body +
            "\nextern \"C\" __global__ void __launch_bounds__(256) rrtl_word_jit(rrtl::word::WordArgs A) {\n"
-           "  rrtl::word::word_body<" + std::to_string(mode) + ", rrtl::word::jit::JitExec>(A, blockIdx.x);\n}\n";
+           "  rrtl::word::word_body<" + std::to_string(mode) + ", rrtl::word::jit::JitExec, " + std::to_string(G) +
+           ">(A, blockIdx.x);\n}\n";
   }
 
  private:
@@ -417,9 +452,25 @@ class WordJit {
   static std::string masked(const std::string& e, uint32_t width) {
     return width % 32 ? "(" + e + " & " + hex(final_mask(width)) + ")" : e;
   }
-  void begin(std::string& out) {
-    out_ = &out; sig_.clear(); scr_.clear(); stage_.clear();
-    if (lay_.stride) line("uint32_t* const L = lds_words() + threadIdx.x;");
+  // One generated statement (or a guarded group of them) with what the wave
+  // split needs to know: the value it defines, the signal words it loads or
+  // stores, the memory it accesses.
+  enum : int { S_PURE = 0, S_LOAD = 1, S_STORE = 2, S_MEM = 3, S_GENERIC = 4 };
+  struct Stmt {
+    std::string text, def;
+    bool is_bool = false;
+    int kind = S_PURE;
+    std::vector<uint32_t> words;
+    int32_t mem = -1;
+  };
+  void begin(std::vector<Stmt>& out) {
+    st_ = &out; sig_.clear(); scr_.clear(); stage_.clear();
+  }
+  std::string header() const { return lay_.stride ? "  uint32_t* const L = lds_words() + C.col;\n" : ""; }
+  std::string emit_serial(const std::vector<Stmt>& S) const {
+    std::string out = header();
+    for (auto& x : S) out += x.text;
+    return out;
   }
   // Signal word w as an lvalue expression.
   std::string sig_ref(uint32_t w) const {
@@ -429,12 +480,33 @@ class WordJit {
   void store_sig(uint32_t off, uint32_t l, const std::string& v) {
     if (lay_.stride) line(sig_ref(off + l) + " = " + v + ";");
     else line("C.stsig(" + num(off) + ", " + num(l) + ", " + v + ");");
+    if (!grouping_) st_->back().kind = S_STORE;
+    st_->back().words.push_back(off + l);
   }
-  void line(const std::string& s) { *out_ += "  " + s + "\n"; }
+  void line(const std::string& s) {
+    if (grouping_) { st_->back().text += "  " + s + "\n"; return; }
+    Stmt x;
+    x.text = "  " + s + "\n";
+    st_->push_back(std::move(x));
+  }
+  // A guarded group: the lines up to end_group() form one statement of `kind`.
+  void begin_group(const std::string& s, int kind, int32_t mem = -1) {
+    line(s);
+    st_->back().kind = kind; st_->back().mem = mem;
+    grouping_ = true;
+  }
+  void end_group() { st_->back().text += "  }\n"; grouping_ = false; }
   std::string fresh(char p) { return p + std::to_string(tmp_++); }
   std::string def(char p, const std::string& e) {
     std::string n = fresh(p);
     line("const uint32_t " + n + " = " + e + ";");
+    st_->back().def = n;
+    return n;
+  }
+  std::string def_bool(char p, const std::string& e) {
+    std::string n = fresh(p);
+    line("const bool " + n + " = " + e + ";");
+    st_->back().def = n; st_->back().is_bool = true;
     return n;
   }
   uint32_t cp(uint32_t i) const {
@@ -455,20 +527,22 @@ class WordJit {
         const uint32_t w = idx + limb;
         auto it = sig_.find(w);
         if (it != sig_.end()) return it->second;
-        return sig_[w] = def('s', lay_.stride ? sig_ref(w) : "C.ld(" + hex(SP_SIGNAL | w) + ", 0)");
+        const std::string n = def('s', lay_.stride ? sig_ref(w) : "C.ld(" + hex(SP_SIGNAL | w) + ", 0)");
+        st_->back().kind = S_LOAD; st_->back().words = {w};
+        return sig_[w] = n;
       }
       default: {
         auto it = scr_.find(idx + limb);
         if (it != scr_.end()) return it->second;
-        return def('t', "C.ld(" + hex(idx + limb) + ", 0)");
+        const std::string n = def('t', "C.ld(" + hex(idx + limb) + ", 0)");
+        st_->back().kind = S_GENERIC;
+        return n;
       }
     }
   }
   std::string reset_cond(uint32_t code, uint32_t pol) {
     const std::string r = opnd(code, 0);
-    std::string n = fresh('r');
-    line("const bool " + n + " = (" + r + " & 1u) " + (pol == 0 ? "!=" : "==") + " 0u;");
-    return n;
+    return def_bool('r', "(" + r + " & 1u) " + (pol == 0 ? "!=" : "==") + " 0u");
   }
   // In-range test of an address operand of `n` limbs against `depth`.
   std::string addr_ok(uint32_t code, uint32_t n, uint32_t depth, std::string& lo) {
@@ -571,10 +645,10 @@ class WordJit {
         if (mt[1] == 0) return "0u";
         std::string lo;
         const std::string ok = addr_ok(R.w[2], R.w[7], mt[1], lo);
-        const std::string o = fresh('o');
-        line("const bool " + o + " = " + ok + ";");
+        const std::string o = def_bool('o', ok);
         // out-of-range lanes read word 0 (in bounds) and take 0
         const std::string m = def('m', mem_ref(R.w[6], "uint32_t(" + o + " ? " + lo + " : 0u) * " + num(mt[2])));
+        st_->back().kind = S_MEM; st_->back().mem = int32_t(R.w[6]);
         return "(" + o + " ? " + (mt[2] == 1 ? masked(m, mt[3]) : m) + " : 0u)";
       }
       default: return "";
@@ -585,7 +659,7 @@ class WordJit {
     if ((code & SP_MASK) != SP_SCRATCH) return;
     for (uint32_t l = 0; l < limbs; l++) {
       auto it = scr_.find((code & ~SP_MASK) + l);
-      if (it != scr_.end()) line("C.st(" + hex(it->first) + ", 0, " + it->second + ");");
+      if (it != scr_.end()) { line("C.st(" + hex(it->first) + ", 0, " + it->second + ");"); st_->back().kind = S_GENERIC; }
     }
   }
   void emit_generic(const Rec& R) {
@@ -604,6 +678,7 @@ class WordJit {
     std::string w;
     for (int k = 0; k < 8; k++) w += (k ? ", " : "") + hex(R.w[k]);
     line("{ const Rec R = {{" + w + "}}; exec_rec(C, R); }");
+    st_->back().kind = S_GENERIC;
     for (uint32_t l = 0; l < L; l++) scr_.erase(R.w[1] + l);
     generic_ = true;
     n_generic_++;
@@ -628,9 +703,9 @@ class WordJit {
           v[l] = l < R.w[7] ? opnd(R.w[2], l) : "0u";
           if (l + 1 == L) v[l] = masked(v[l], R.w[5]);
         }
-        line("if (" + r + ") {");
+        begin_group("if (" + r + ") {", S_STORE);
         for (uint32_t l = 0; l < L; l++) store_sig(R.w[1], l, v[l]);
-        line("}");
+        end_group();
         for (uint32_t l = 0; l < L; l++) sig_.erase(R.w[1] + l);
         break;
       }
@@ -664,14 +739,388 @@ class WordJit {
           v[l] = l < ld ? opnd(R.w[4], l) : "0u";
           if (l + 1 == L) v[l] = masked(v[l], mt[3]);
         }
-        line("if ((" + en + " & 1u) && " + ok + ") {");
+        begin_group("if ((" + en + " & 1u) && " + ok + ") {", S_MEM, int32_t(R.w[1]));
         for (uint32_t l = 0; l < L; l++)
           line("  " + mem_ref(R.w[1], lo + " * " + num(mt[2]) + " + " + num(l)) + " = " + v[l] + ";");
-        line("}");
+        end_group();
         break;
       }
       default: throw Error("E_PROGRAM: unknown effect record");
     }
+  }
+
+  // Cost units of a statement (about one VALU instruction each).
+  static double cost_of(const Stmt& x) {
+    if (x.kind == S_LOAD) return 1;
+    if (x.kind == S_STORE) return double(x.words.size());
+    if (x.kind == S_MEM) return 4;
+    double c = 1;
+    const std::string& t = x.text;
+    for (size_t i = 0; i < t.size(); i++) {
+      const char ch = t[i];
+      if (ch == '&' || ch == '|' || ch == '^' || ch == '+' || ch == '*' || ch == '~' || ch == '?' || ch == '-') c++;
+      else if ((ch == '<' || ch == '>') && !(i + 1 < t.size() && t[i + 1] == ch)) c++;
+      else if (ch == '=' && i > 0 && (t[i - 1] == '=' || t[i - 1] == '!')) c++;
+    }
+    return c;
+  }
+  // Names of generated values a statement mentions (fresh() names: a letter
+  // and digits, not inside a longer identifier or a literal).
+  static std::vector<std::string> names_in(const std::string& t) {
+    std::vector<std::string> out;
+    for (size_t i = 0; i < t.size(); i++) {
+      const char ch = t[i];
+      if (ch < 'a' || ch > 'z') continue;
+      if (i > 0 && (std::isalnum(static_cast<unsigned char>(t[i - 1])) || t[i - 1] == '_' || t[i - 1] == '.')) continue;
+      size_t j = i + 1;
+      while (j < t.size() && std::isdigit(static_cast<unsigned char>(t[j]))) j++;
+      if (j == i + 1 || (j < t.size() && (std::isalnum(static_cast<unsigned char>(t[j])) || t[j] == '_'))) continue;
+      out.push_back(t.substr(i, j - i));
+      i = j - 1;
+    }
+    return out;
+  }
+
+  // Splits one evaluation between G waves in barrier phases (bulk-synchronous
+  // list scheduling).  Statements run in program order within a wave; a value
+  // used by another wave is written to an LDS exchange row in its phase and
+  // read in a later one; signal loads are repeated in every wave that needs
+  // them; all accesses of one memory stay in one wave; a signal store runs in
+  // a phase after every other wave's last load of that word (and after the
+  // earlier stores of it), so no wave reads a word another wave has already
+  // rewritten.  Budgets of work per wave and phase are tried from one phase
+  // per wave down to many; the estimate (per phase the busiest wave's work
+  // plus its exchange accesses, plus a barrier) picks the plan.  Code with
+  // interpreted records or with a load after a store of the same word cannot
+  // be split.
+  static Split split(const std::vector<Stmt>& S, uint32_t G) {
+    // Cost units of a barrier, of an exchange access and the largest cone
+    // recomputed per wave (picorv32 at 16,384 lanes, 4 waves: 4.61e9
+    // lane-cycles/s at 100 / 8 against 4.40e9 at 24 / 4 and 3.52e9 unsplit).
+    constexpr double kBarrier = 100, kXchg = 1, kRecompute = 8;
+    Split best;
+    const size_t n = S.size();
+    std::unordered_map<std::string, uint32_t> def_of;
+    std::vector<std::vector<uint32_t>> uses(n);
+    std::unordered_set<uint32_t> stored;
+    double serial = 0;
+    for (size_t i = 0; i < n; i++) {
+      const Stmt& x = S[i];
+      if (x.kind == S_GENERIC) return best;
+      if (x.kind == S_LOAD && stored.count(x.words[0])) return best;
+      if (x.kind == S_STORE) for (uint32_t w : x.words) stored.insert(w);
+      for (auto& nm : names_in(x.text)) {
+        if (nm == x.def) continue;
+        auto it = def_of.find(nm);
+        if (it != def_of.end()) uses[i].push_back(it->second);
+      }
+      std::sort(uses[i].begin(), uses[i].end());
+      uses[i].erase(std::unique(uses[i].begin(), uses[i].end()), uses[i].end());
+      if (!x.def.empty()) def_of[x.def] = uint32_t(i);
+      serial += cost_of(x);
+    }
+    best.serial = serial;
+    // Cheap values of signal loads alone are recomputed in every wave that
+    // uses them (like the loads) instead of crossing between waves.
+    std::vector<uint8_t> rep(n, 0);
+    std::vector<double> cone(n, 0);
+    for (size_t i = 0; i < n; i++) {
+      if (S[i].kind != S_PURE || S[i].def.empty()) continue;
+      double c = cost_of(S[i]);
+      bool ok = true;
+      for (uint32_t q : uses[i]) {
+        if (S[q].kind == S_LOAD) continue;
+        if (!rep[q]) { ok = false; break; }
+        c += cone[q];
+      }
+      if (ok && c <= kRecompute) { rep[i] = 1; cone[i] = c; }
+    }
+    auto is_work = [&](size_t i) { return (S[i].kind == S_PURE && !rep[i]) || S[i].kind == S_MEM; };
+    // work predecessors: values used, and the previous access of the same memory
+    std::vector<std::vector<uint32_t>> pred(n), succ(n);
+    std::unordered_map<int32_t, uint32_t> last_mem;
+    int32_t n_mem = 0;
+    for (size_t i = 0; i < n; i++) {
+      if (!is_work(i)) continue;
+      for (uint32_t q : uses[i]) if (is_work(q)) pred[i].push_back(q);
+      if (S[i].kind == S_MEM) {
+        n_mem = std::max(n_mem, S[i].mem + 1);
+        auto it = last_mem.find(S[i].mem);
+        if (it != last_mem.end()) pred[i].push_back(it->second);
+        last_mem[S[i].mem] = uint32_t(i);
+      }
+      for (uint32_t q : pred[i]) succ[q].push_back(uint32_t(i));
+    }
+    std::vector<double> cost(n), bl(n, 0);
+    double total = 0;
+    for (size_t i = 0; i < n; i++) { cost[i] = cost_of(S[i]); if (is_work(i)) total += cost[i]; }
+    for (size_t k = n; k-- > 0;) {
+      if (!is_work(k)) continue;
+      double m = 0;
+      for (uint32_t u : succ[k]) m = std::max(m, bl[u]);
+      bl[k] = cost[k] + m;
+    }
+    std::vector<double> budgets;
+    for (uint32_t k = 1; k <= 48; k++) budgets.push_back(std::max(4.0, total / G / k));
+    for (double B : budgets) {
+      Split W;
+      W.ok = true; W.serial = serial;
+      W.wave.assign(n, -1); W.phase.assign(n, -1);
+      std::vector<uint32_t> left(n, 0), ready;
+      size_t nwork = 0;
+      for (size_t i = 0; i < n; i++) {
+        if (!is_work(i)) continue;
+        nwork++;
+        left[i] = uint32_t(pred[i].size());
+        if (!left[i]) ready.push_back(uint32_t(i));
+      }
+      std::vector<int32_t> pin(size_t(n_mem), -1);
+      std::vector<double> load(G, 0);
+      int32_t p = 0;
+      size_t placed = 0;
+      while (placed < nwork) {
+        std::sort(ready.begin(), ready.end(), [&](uint32_t a, uint32_t b) { return bl[a] != bl[b] ? bl[a] > bl[b] : a < b; });
+        int32_t bi = -1, bg = -1;
+        for (size_t r = 0; r < ready.size() && bi < 0; r++) {
+          const uint32_t i = ready[r];
+          double bkey = 0;
+          for (uint32_t g = 0; g < G; g++) {
+            if (S[i].kind == S_MEM && pin[S[i].mem] >= 0 && pin[S[i].mem] != int32_t(g)) continue;
+            int32_t er = 0, away = 0;
+            for (uint32_t q : pred[i]) {
+              er = std::max(er, W.wave[q] == int32_t(g) ? W.phase[q] : W.phase[q] + 1);
+              away += W.wave[q] != int32_t(g);
+            }
+            if (er > p) continue;
+            if (load[g] > 0 && load[g] + cost[i] > B) continue;
+            const double key = load[g] + 2 * kXchg * away;
+            if (bg < 0 || key < bkey) { bg = int32_t(g); bkey = key; }
+          }
+          if (bg >= 0) bi = int32_t(r);
+        }
+        if (bi < 0) { p++; std::fill(load.begin(), load.end(), 0.0); continue; }
+        const uint32_t i = ready[size_t(bi)];
+        ready.erase(ready.begin() + bi);
+        W.wave[i] = bg; W.phase[i] = p; load[size_t(bg)] += cost[i];
+        if (S[i].kind == S_MEM) pin[S[i].mem] = bg;
+        placed++;
+        for (uint32_t u : succ[i]) if (--left[u] == 0) ready.push_back(u);
+      }
+      // signal loads and recomputed values: in each wave at the phase of their first user there
+      W.load_phase.assign(G, std::vector<int32_t>(n, -1));
+      std::function<void(uint32_t, int32_t, int32_t)> need_load = [&](uint32_t l, int32_t g, int32_t ph) {
+        int32_t& lp = W.load_phase[size_t(g)][l];
+        if (lp >= 0 && lp <= ph) return;
+        lp = ph;
+        if (rep[l]) for (uint32_t q : uses[l]) need_load(q, g, ph);
+      };
+      for (size_t i = 0; i < n; i++)
+        if (is_work(i))
+          for (uint32_t q : uses[i]) if (S[q].kind == S_LOAD || rep[q]) need_load(q, W.wave[i], W.phase[i]);
+      // stores: in the wave of the latest value they store; the loads and
+      // recomputed values a store uses itself (register copies) are taken in
+      // phase 0, before any store, so every load is known before the stores
+      // are ordered after them
+      std::vector<int32_t> store_wave(n, 0);
+      for (size_t i = 0; i < n; i++) {
+        if (S[i].kind != S_STORE) continue;
+        int32_t g = 0, gp = -1;
+        for (uint32_t q : uses[i]) if (is_work(q) && W.phase[q] > gp) { gp = W.phase[q]; g = W.wave[q]; }
+        store_wave[i] = g;
+        for (uint32_t q : uses[i]) if (S[q].kind == S_LOAD || rep[q]) need_load(q, g, 0);
+      }
+      std::unordered_map<uint32_t, std::vector<int32_t>> last_load;  // word -> per wave: last load phase
+      auto note_loads = [&]() {
+        last_load.clear();
+        for (uint32_t g = 0; g < G; g++)
+          for (size_t l = 0; l < n; l++)
+            if (W.load_phase[g][l] >= 0 && S[l].kind == S_LOAD) {
+              auto& v = last_load[S[l].words[0]];
+              if (v.empty()) v.assign(G, -1);
+              v[g] = std::max(v[g], W.load_phase[g][l]);
+            }
+      };
+      note_loads();
+      std::unordered_map<uint32_t, std::pair<int32_t, int32_t>> last_store;  // word -> (wave, phase)
+      for (size_t i = 0; i < n; i++) {
+        if (S[i].kind != S_STORE) continue;
+        const int32_t g = store_wave[i];
+        int32_t ph = 0;
+        for (uint32_t q : uses[i]) if (is_work(q)) ph = std::max(ph, W.wave[q] == g ? W.phase[q] : W.phase[q] + 1);
+        for (uint32_t w : S[i].words) {
+          auto it = last_load.find(w);
+          if (it != last_load.end())
+            for (uint32_t h = 0; h < G; h++)
+              if (it->second[h] >= 0) ph = std::max(ph, int32_t(h) == g ? it->second[h] : it->second[h] + 1);
+          auto js = last_store.find(w);
+          if (js != last_store.end()) ph = std::max(ph, js->second.first == g ? js->second.second : js->second.second + 1);
+        }
+        W.wave[i] = g; W.phase[i] = ph;
+        for (uint32_t w : S[i].words) last_store[w] = {g, ph};
+      }
+      int32_t P = 0;
+      for (size_t i = 0; i < n; i++) P = std::max(P, W.phase[i] + 1);
+      for (uint32_t g = 0; g < G; g++) for (size_t l = 0; l < n; l++) P = std::max(P, W.load_phase[g][l] + 1);
+      W.phases = uint32_t(std::max(P, 1));
+      // exchange rows: a value used in another wave, live from its phase to its last reader's
+      std::vector<int32_t> last_read(n, -1);
+      std::vector<std::vector<double>> busy(G, std::vector<double>(W.phases, 0));
+      for (size_t i = 0; i < n; i++) {
+        if (W.wave[i] >= 0) busy[size_t(W.wave[i])][size_t(W.phase[i])] += cost[i];
+        if (W.wave[i] < 0) continue;
+        std::vector<uint8_t> imported(G, 0);
+        for (uint32_t q : uses[i]) {
+          if (!is_work(q) || W.wave[q] == W.wave[i]) continue;
+          last_read[q] = std::max(last_read[q], W.phase[i]);
+        }
+      }
+      for (uint32_t g = 0; g < G; g++)
+        for (size_t l = 0; l < n; l++) if (W.load_phase[g][l] >= 0) busy[g][size_t(W.load_phase[g][l])] += cost[l];
+      // imports: one read per (value, reading wave)
+      for (uint32_t g = 0; g < G; g++) {
+        std::vector<int32_t> first(n, -1);
+        for (size_t i = 0; i < n; i++) {
+          if (W.wave[i] != int32_t(g)) continue;
+          for (uint32_t q : uses[i])
+            if (is_work(q) && W.wave[q] != int32_t(g) && (first[q] < 0 || W.phase[i] < first[q])) first[q] = W.phase[i];
+        }
+        for (size_t q = 0; q < n; q++) if (first[q] >= 0) busy[g][size_t(first[q])] += kXchg;
+      }
+      W.xslot.assign(n, -1);
+      std::vector<int32_t> row_free;  // per row: first phase it may be written again
+      for (int32_t ph = 0; ph < int32_t(W.phases); ph++)
+        for (size_t q = 0; q < n; q++) {
+          if (last_read[q] < 0 || W.phase[q] != ph) continue;
+          busy[size_t(W.wave[q])][size_t(ph)] += kXchg;
+          int32_t r = -1;
+          for (size_t k = 0; k < row_free.size() && r < 0; k++) if (row_free[k] <= ph) r = int32_t(k);
+          if (r < 0) { r = int32_t(row_free.size()); row_free.push_back(0); }
+          row_free[size_t(r)] = last_read[q] + 1;
+          W.xslot[q] = r;
+        }
+      W.xslots = uint32_t(row_free.size());
+      double est = 0;
+      for (uint32_t ph = 0; ph < W.phases; ph++) {
+        double m = 0;
+        for (uint32_t g = 0; g < G; g++) m = std::max(m, busy[g][ph]);
+        est += m + kBarrier;
+      }
+      W.est = est;
+      if (!best.ok || W.est < best.est) best = std::move(W);
+    }
+    verify_split(S, best, G, uses, rep);
+    return best;
+  }
+
+  // Checks a split against the sequential program: every value is computed
+  // before its use (in the same wave, or in an earlier phase of another wave
+  // through an exchange row); any two accesses of one signal word of which
+  // one is a store, and any two accesses of one memory, keep their program
+  // order (within a wave by phase and position, across waves by phase);
+  // exchange rows are not rewritten before their last read.  A violation is
+  // an internal error.
+  static void verify_split(const std::vector<Stmt>& S, const Split& W, uint32_t G,
+                           const std::vector<std::vector<uint32_t>>& uses, const std::vector<uint8_t>& rep) {
+    const size_t n = S.size();
+    auto fail = [&](const std::string& what, size_t i) {
+      throw Error("E_HIP_INTERNAL: wave split: " + what + " at statement " + std::to_string(i) + ": " + S[i].text);
+    };
+    struct Occ { uint32_t g; int32_t ph; uint32_t i; };
+    std::vector<std::vector<Occ>> occ(n);
+    for (size_t i = 0; i < n; i++) {
+      if (W.wave[i] >= 0) occ[i].push_back({uint32_t(W.wave[i]), W.phase[i], uint32_t(i)});
+      else
+        for (uint32_t g = 0; g < G; g++)
+          if (W.load_phase[g][i] >= 0) occ[i].push_back({g, W.load_phase[g][i], uint32_t(i)});
+    }
+    auto before = [](const Occ& a, const Occ& b) { return a.ph < b.ph || (a.ph == b.ph && a.i < b.i); };
+    std::vector<int32_t> last_read(n, -1);
+    for (size_t i = 0; i < n; i++)
+      for (const Occ& o : occ[i])
+        for (uint32_t q : uses[i]) {
+          bool ok = false;
+          if (W.wave[q] < 0) {
+            for (const Occ& d : occ[q]) ok |= d.g == o.g && before(d, o);
+          } else if (uint32_t(W.wave[q]) == o.g) {
+            ok = before(occ[q][0], o);
+          } else {
+            ok = W.phase[q] < o.ph && W.xslot[q] >= 0;
+            last_read[q] = std::max(last_read[q], o.ph);
+          }
+          if (!ok) fail("value " + S[q].def + " not available", i);
+        }
+    std::unordered_map<uint64_t, std::vector<std::pair<Occ, bool>>> acc;  // (kind, word or memory) -> accesses
+    for (size_t i = 0; i < n; i++) {
+      if (S[i].kind == S_LOAD || S[i].kind == S_STORE)
+        for (uint32_t w : S[i].words)
+          for (const Occ& o : occ[i]) acc[uint64_t(w)].push_back({o, S[i].kind == S_STORE});
+      if (S[i].kind == S_MEM)
+        for (const Occ& o : occ[i]) acc[(1ull << 40) | uint32_t(S[i].mem)].push_back({o, true});
+    }
+    for (auto& kv : acc) {
+      const auto& v = kv.second;
+      for (size_t a = 0; a < v.size(); a++)
+        for (size_t b = 0; b < v.size(); b++) {
+          const Occ &x = v[a].first, &y = v[b].first;
+          if (x.i >= y.i || !(v[a].second || v[b].second)) continue;
+          const bool ok = x.g == y.g ? before(x, y) : x.ph < y.ph;
+          if (!ok) fail(kv.first >> 40 ? "memory access order" : "signal access order", y.i);
+        }
+    }
+    std::unordered_map<int32_t, std::vector<uint32_t>> rows;
+    for (size_t q = 0; q < n; q++) if (W.xslot[q] >= 0) rows[W.xslot[q]].push_back(uint32_t(q));
+    for (auto& kv : rows)
+      for (uint32_t a : kv.second)
+        for (uint32_t b : kv.second)
+          if (a != b && W.phase[a] <= W.phase[b] && last_read[a] >= W.phase[b]) fail("exchange row reused while live", b);
+    (void)rep;
+  }
+
+  // jit_eval / jit_edge of the G-wave kernel: per wave its statements phase by
+  // phase in program order (exchange reads before first use, exchange writes
+  // after the definition), a barrier between phases, and one after the last
+  // phase so the next step sees every wave's stores.
+  std::string emit_split(const std::vector<Stmt>& S, const Split& W, uint32_t G, uint32_t xrow0) const {
+    const size_t n = S.size();
+    std::vector<std::vector<uint32_t>> uses(n);
+    std::unordered_map<std::string, uint32_t> def_of;
+    for (size_t i = 0; i < n; i++) {
+      for (auto& nm : names_in(S[i].text)) {
+        if (nm == S[i].def) continue;
+        auto it = def_of.find(nm);
+        if (it != def_of.end()) uses[i].push_back(it->second);
+      }
+      if (!S[i].def.empty()) def_of[S[i].def] = uint32_t(i);
+    }
+    auto xref = [&](uint32_t q) { return "L[" + num((xrow0 + uint32_t(W.xslot[q])) * 64u) + "]"; };
+    std::string out = header();
+    for (uint32_t g = 0; g < G; g++) {
+      out += "  if (C.wave == " + num(g) + ") {\n";
+      std::vector<uint8_t> have(n, 0);
+      for (uint32_t ph = 0; ph < W.phases; ph++) {
+        for (size_t i = 0; i < n; i++) {
+          const bool mine = W.wave[i] == int32_t(g) && W.phase[i] == int32_t(ph);
+          const bool load = W.wave[i] < 0 && W.load_phase[g][i] == int32_t(ph);
+          if (!mine && !load) continue;
+          for (uint32_t q : uses[i]) {
+            if (have[q] || W.wave[q] < 0 || W.wave[q] == int32_t(g) || W.xslot[q] < 0) continue;
+            out += S[q].is_bool ? "    const bool " + S[q].def + " = " + xref(q) + " != 0u;\n"
+                                : "    const uint32_t " + S[q].def + " = " + xref(q) + ";\n";
+            have[q] = 1;
+          }
+          for (size_t a = 0, b; a < S[i].text.size(); a = b + 1) {
+            b = S[i].text.find('\n', a);
+            out += "  " + S[i].text.substr(a, b - a + 1);
+          }
+          if (mine && W.xslot[i] >= 0)
+            out += "    " + xref(uint32_t(i)) + " = " + (S[i].is_bool ? "uint32_t(" + S[i].def + ")" : S[i].def) + ";\n";
+        }
+        if (ph + 1 < W.phases) out += "    __syncthreads();\n";
+      }
+      out += "  }\n";
+    }
+    out += "  __syncthreads();\n";
+    return out;
   }
 
   void emit_block(int b) {
@@ -687,8 +1136,11 @@ class WordJit {
   const WordProgram& WP;
   JitLayout lay_;
   std::vector<int32_t> mem_row_;  // per memory: first LDS row, or -1 (HBM)
-  std::string body_;
-  std::string* out_ = nullptr;
+  std::vector<Stmt> fn_[2];       // jit_eval, jit_edge
+  Split split_[2];
+  uint32_t waves_ = 1, xslots_ = 0;
+  std::vector<Stmt>* st_ = nullptr;
+  bool grouping_ = false;
   std::unordered_map<uint32_t, std::string> sig_, scr_, stage_;  // word -> register holding it
   uint32_t tmp_ = 0, n_generic_ = 0;
   bool generic_ = false;
@@ -830,6 +1282,36 @@ class WordEngine final : public Engine {
     // against 2.4e9 lane-cycles/s at 16 lanes per wave).
     const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
     cus_ = cus;
+    // Several waves per 64 lanes when the batch gives fewer than one 64-lane
+    // wave per SIMD (four per CU): the generated code of each evaluation is
+    // split between the waves of a workgroup, which share the lanes' LDS
+    // columns (WordJit::split).  Automatic when the split is estimated to pay
+    // a fifth or more.
+    std::unique_ptr<WordJit> split_jit;
+    if (jit && !jit->uses_scratch() && opt.lane_waves != 1) {
+      const uint64_t blocks64 = (uint64_t(lanes) + 63) / 64, simds = 4ull * cus;
+      uint32_t G = opt.lane_waves;
+      if (!G) G = blocks64 * 4 <= simds ? 4 : (blocks64 * 2 <= simds ? 2 : 1);
+      if (G > 1) {
+        JitLayout lay;
+        uint32_t mem_words = 0;
+        lay.mem_row = lds_memory_rows(P, 0, 64, &mem_words);
+        lay.stride = 64; lay.slots = 0; lay.sig_words = P.total_signal_words;
+        std::unique_ptr<WordJit> sj(new WordJit(WP, std::move(lay)));
+        const double gain = sj->plan_waves(G);
+        const size_t rows = size_t(P.total_signal_words) + mem_words + sj->exchange_rows();
+        const bool fits = rows * 64 * 4 <= std::min<size_t>(lds_max_, 64 * 1024);
+        if (gain > 0 && fits && (opt.lane_waves || gain >= 1.2)) {
+          lane_waves_ = G; lds_mem_words_ = mem_words; split_jit = std::move(sj);
+          split_lds_ = rows * 64 * 4;
+        } else if (opt.lane_waves) {
+          throw Error(gain > 0 ? "E_HIP_OPTIONS: lane_waves: the split design does not fit in LDS"
+                               : "E_HIP_OPTIONS: lane_waves: the generated code cannot be split between waves");
+        }
+      }
+    } else if (opt.lane_waves > 1) {
+      throw Error("E_HIP_OPTIONS: lane_waves needs generated code without interpreted records");
+    }
     if (opt.wave_lanes) {
       wave_lanes_ = opt.wave_lanes;  // 4, 8, 16, 32 or 64
     } else {
@@ -837,7 +1319,10 @@ class WordEngine final : public Engine {
       while (n_rec < 600 && wave_lanes_ > 16 && (uint64_t(lanes) + wave_lanes_ - 1) / wave_lanes_ < 4ull * cus)
         wave_lanes_ /= 2;
     }
-    if (wave_lanes_ < 64) {
+    if (lane_waves_ > 1) {
+      wave_lanes_ = 64;
+      block_ = 64 * lane_waves_;
+    } else if (wave_lanes_ < 64) {
       block_ = 64;
     } else if (opt.block_threads) {
       block_ = opt.block_threads;
@@ -845,8 +1330,14 @@ class WordEngine final : public Engine {
       const uint32_t per_cu = (lanes + cus - 1) / cus;
       block_ = std::min<uint32_t>(256, std::max<uint32_t>(64, (per_cu + 63) / 64 * 64));
     }
-    mode_ = lds_mode(block_, run_slots_);
+    mode_ = lane_waves_ > 1 ? WM_ALL_LDS : lds_mode(block_, run_slots_);
     use_lds_ = mode_ != WM_HBM;
+    if (lane_waves_ > 1) {
+      run_slots_ = 0;
+      jit_fn_ = rtc_kernel(split_jit->source(WM_ALL_LDS, P.total_signal_words + lds_mem_words_), "rrtl_word_jit",
+                           opt.device);
+      jit.reset();
+    }
     if (jit) {
       // the final layout: LDS-staged memories and constant LDS offsets
       JitLayout lay;
@@ -857,6 +1348,8 @@ class WordEngine final : public Engine {
       jit.reset(new WordJit(WP, std::move(lay)));
     }
     if (jit) jit_fn_ = rtc_kernel(jit->source(mode_), "rrtl_word_jit", opt.device);
+    if (lane_waves_ > 1 && split_lds_ > 65536)
+      throw Error("E_HIP_INTERNAL: split kernel LDS above 64 KiB");
     if (!use_lds_) { scratch_.alloc(size_t(WP.scratch_slots) * lanes); scratch_.zero(stream_); }
     for (int b = 0; b < 4; b++) {
       std::vector<uint32_t> flat;
@@ -1034,6 +1527,9 @@ class WordEngine final : public Engine {
   }
   uint32_t scratch_slots() const { return run_slots_; }
   bool compiled() const { return jit_fn_ != nullptr; }
+  void launch_info(uint32_t* out) const {
+    out[0] = jit_fn_ ? 1u : 0u; out[1] = lane_waves_; out[2] = wave_lanes_; out[3] = block_;
+  }
   bool scratch_in_lds() const { return use_lds_; }
 
  private:
@@ -1051,6 +1547,8 @@ class WordEngine final : public Engine {
 
   // LDS bytes a block of `block` lanes with `slots` scratch slots (and, in
   // WM_ALL_LDS, `mem_words` words of LDS-staged memories) needs in `mode`.
+  // Lanes one workgroup runs.
+  uint32_t lanes_per_block() const { return lane_waves_ > 1 ? 64u : (wave_lanes_ < 64 ? wave_lanes_ : block_); }
   size_t lds_bytes(int mode, uint32_t block, uint32_t slots, uint32_t mem_words = 0) const {
     if (mode == WM_HBM) return 0;
     const size_t words = slots + (mode == WM_ALL_LDS ? size_t(P.total_signal_words) + mem_words : 0);
@@ -1168,7 +1666,7 @@ class WordEngine final : public Engine {
 
   // Slots per XCD partition in a lane map of a compacting run.
   uint32_t compact_cap() const {
-    const uint32_t per_block = wave_lanes_ < 64 ? wave_lanes_ : block_;
+    const uint32_t per_block = lanes_per_block();
     const uint32_t grid = (lanes_ + per_block - 1) / per_block;
     return (grid + kParts - 1) / kParts * per_block;
   }
@@ -1187,9 +1685,10 @@ class WordEngine final : public Engine {
     const bool compact = opt_.compact_lanes && outputs && A.halt_offset != NONE && A.halt_cycles;
     uint64_t chunk = (opt_.ticks_per_launch && outputs) ? opt_.ticks_per_launch : std::max<uint64_t>(cycles, 1);
     if (compact && !opt_.ticks_per_launch) chunk = 1024;
-    const uint32_t per_block = wave_lanes_ < 64 ? wave_lanes_ : block_;
+    const uint32_t per_block = lanes_per_block();
     const uint32_t grid = (lanes_ + per_block - 1) / per_block;
-    const size_t lds = lds_bytes(mode_, block_, run_slots_, lds_mem_words_);
+    // the engine's own launches (batch items size theirs in batch_prepare)
+    const size_t lds = lane_waves_ > 1 ? split_lds_ : lds_bytes(mode_, block_, run_slots_, lds_mem_words_);
     const uint32_t cap = compact_cap();  // slots per partition
     if (compact) {
       // allocated by reserve_compaction() before any launch is issued: these launches may be captured
@@ -1226,7 +1725,8 @@ class WordEngine final : public Engine {
       if (compact && done + B.cycles < cycles) {
         const uint32_t nxt = cur ^ 1u;
         const uint32_t wave_slots = std::min<uint32_t>(per_block, 64u);
-        const uint32_t floor_slots = std::max<uint32_t>(1u, cus_ / kParts) * 4u * wave_slots;  // a wave per SIMD
+        // a wave per SIMD (a workgroup per CU when its waves share the lanes)
+        const uint32_t floor_slots = std::max<uint32_t>(1u, cus_ / kParts) * (lane_waves_ > 1 ? 1u : 4u) * wave_slots;
         hipLaunchKernelGGL(lane_compact_kernel, dim3(kParts), dim3(1024), 0, s, lane_map_.p + size_t(cur) * cap * kParts,
                            live_.p + cur * kParts, A.halt_cycles, lane_map_.p + size_t(nxt) * cap * kParts,
                            live_.p + nxt * kParts, cap, lanes_, wave_slots, floor_slots);
@@ -1284,6 +1784,8 @@ class WordEngine final : public Engine {
   size_t lds_max_ = 65536;
   uint32_t block_ = 256;
   uint32_t wave_lanes_ = 64;            // active lanes per 64-thread wave (16, 32 or 64)
+  uint32_t lane_waves_ = 1;             // waves per 64 lanes (generated code split between them)
+  size_t split_lds_ = 0;                // LDS per workgroup of the split kernel
   int mode_ = WM_SCRATCH_LDS;
   bool use_lds_ = true, has_digest_ = false, has_halt_ = false, batch_spec_halt_ = false, batch_digest_ = false;
   size_t trace_bytes_ = 0;
@@ -1312,8 +1814,30 @@ std::string word_kernel_source(const Program& prog, int mode, uint32_t block) {
   return WordJit(wp).source(mode);
 }
 
+// The kernel the engine generates for `lane_waves` waves per 64 lanes, with a
+// first comment line giving the split's estimated speed-up over one wave, its
+// barrier phases and exchange rows.
+std::string word_split_kernel_source(const Program& prog, uint32_t lane_waves) {
+  if (lane_waves != 2 && lane_waves != 4) throw Error("E_HIP_OPTIONS: lane_waves must be 2 or 4");
+  const WordProgram wp = WordCompiler(prog).compile();
+  if (WordJit(wp).uses_scratch()) throw Error("E_HIP_OPTIONS: lane_waves needs generated code without interpreted records");
+  JitLayout lay;
+  uint32_t mem_words = 0;
+  lay.mem_row = lds_memory_rows(prog, 0, 64, &mem_words);
+  lay.stride = 64; lay.sig_words = prog.total_signal_words;
+  WordJit j(wp, std::move(lay));
+  const double gain = j.plan_waves(lane_waves);
+  if (gain <= 0) throw Error("E_HIP_OPTIONS: lane_waves: the generated code cannot be split between waves");
+  char head[160];
+  snprintf(head, sizeof head, "// split: %u waves, estimated speed-up %.3f, exchange rows %u, phases %u/%u\n",
+           lane_waves, gain, j.exchange_rows(), j.phases(0), j.phases(1));
+  return head + j.source(WM_ALL_LDS, prog.total_signal_words + mem_words);
+}
+
 Engine* make_word_engine(Program prog, uint32_t lanes, const EngineOptions& opt) {
   return new WordEngine(std::move(prog), lanes, opt);
 }
+
+void word_engine_launch_info(Engine* e, uint32_t* info) { static_cast<WordEngine*>(e)->launch_info(info); }
 
 }  // namespace rrtl

@@ -59,8 +59,9 @@ def test_word_autotune_halting_run_pick_matches_closed_form():
     rec = autotune(p, lanes, cap, "countdown", run_kwargs=kw, setup=setup, repeats=1)
     assert rec.engine == "word"
     compacting = [c for c in rec.candidates if c.get("compact_lanes")]
-    assert len(compacting) == 2 and all("error" not in c for c in compacting), rec.candidates
+    assert len(compacting) == 6 and all("error" not in c for c in compacting), rec.candidates
     assert all("error" not in c for c in rec.candidates), rec.candidates
+    assert any(c.get("lane_waves", 1) > 1 for c in rec.candidates)
 
     sim = HipSimulator(p, lanes, **rec.options())
     setup(sim)

@@ -198,3 +198,45 @@ def test_fused_records_in_generated_source():
     assert re.search(r"~\(\(\w+ & \w+\) \| \(\w+ & \w+\)\)", src), "AOI22"
     assert re.search(r"~\(\(\w+ \| \w+\) & \(\w+ \| \w+\)\)", src), "OAI22"
     assert re.search(r"~\(\w+ & \w+\)", src), "NAND"
+
+
+def test_split_kernel_source_compiles():
+    """The picorv32 kernel split between 2 and 4 waves per 64 lanes: one code
+    block per wave, estimated to beat one wave, and hiprtc compiles it for
+    gfx950 (no device needed); a wave count other than 2 or 4 is refused."""
+    from rrtl_amd import backend, picorv32 as pv
+    p = pv.program()
+    for waves in (2, 4):
+        src = backend.emit_split_kernel(p, waves)
+        head = src.splitlines()[0]
+        m = re.search(r"estimated speed-up ([0-9.]+), exchange rows (\d+)", head)
+        assert m and float(m.group(1)) > 1.4, head
+        assert all(f"if (C.wave == {g}u)" in src for g in range(waves))
+        assert f"jit::JitExec, {waves}>" in src
+        assert backend.compile_kernel(src) > 0
+    with pytest.raises(backend.HipError) as e:
+        backend.emit_split_kernel(p, 3)
+    assert e.value.code == "E_HIP_OPTIONS"
+
+
+def test_split_plans_verified_on_designs():
+    """Every split the engine would build passes its own hazard check (a
+    violation raises E_HIP_INTERNAL): picorv32, the single-cycle RV32I core
+    and random one-limb designs, at 2 and 4 waves; designs with interpreted
+    records are refused with E_HIP_OPTIONS."""
+    import sys
+    from rrtl_amd import backend, lower, picorv32 as pv, riscv
+    sys.path.insert(0, os.path.dirname(__file__))
+    from designgen import random_design
+    progs = [pv.program(), lower.lower_module(riscv.rv32i_core())]
+    progs += [lower.lower_module(random_design(s, async_resets=False, widths=(1, 4, 8, 13, 16, 31, 32)))
+              for s in range(12)]
+    split = 0
+    for p in progs:
+        for waves in (2, 4):
+            try:
+                backend.emit_split_kernel(p, waves)
+                split += 1
+            except backend.HipError as e:
+                assert e.code == "E_HIP_OPTIONS", str(e)
+    assert split >= 10
