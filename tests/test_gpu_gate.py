@@ -199,6 +199,27 @@ def test_full_size_netlist_sampled_words():
         assert np.array_equal(d1[:, w64], dg[:, 0]), w64
 
 
+def test_full_size_netlist_every_lane_against_host_aot_engine():
+    """The 20k-gate workload at full shape: 65,536 lanes x 2,000 cycles, every
+    lane's output signature and every register against the host AOT engine
+    (straight-line C per design, pinned to the faithful bit-parallel oracle
+    in the oracle tests), in two consecutive runs."""
+    from oracle import bp_aot
+    flush_print("case: 20k-gate netlist 65536 lanes x 2000 cycles, every lane vs host AOT engine")
+    p = small_netlist_program()
+    lanes, cycles = 65536, 2000
+    ins, obs = netlist_inputs(p), netlist_outputs(p)
+    regs = [i for i, s in enumerate(p.signals) if s.kind == KIND_REG]
+    sim = HipSimulator(p, lanes, engine="gate")
+    aot = bp_aot.BitParallelAot(p, lanes, ins, obs, group_words=8, threads=max(1, min(16, os.cpu_count() or 1)))
+    for seed in (7, 8):
+        sim.run(cycles, seed=seed, inputs=ins, observe=obs)
+        da, _ = aot.run(seed, cycles)
+        assert np.array_equal(stim.u32_rows_to_u64(sim.digest()), da), seed
+    for r in regs:
+        assert np.array_equal(stim.u32_rows_to_u64(sim.get_signal_bits(r)), aot.get_words(r)), p.signals[r].name
+
+
 def test_work_queue_error_in_an_earlier_launch_fails_the_run(monkeypatch):
     """The work queue's error flag is cleared once per run, not per launch: an
     error raised in the first of three launches (injected from the host before
