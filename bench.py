@@ -329,6 +329,8 @@ def bench_mixed(args) -> int:
                      "launches": st["launches"], "lane_cycles": st["lane_cycles"]}
         if mode == "per_design":
             res[mode]["design_ms"] = {j.name: s.stats()["kernel_ms"] for s, j in zip(sims, jobs)}
+        if mode == "persistent":
+            res[mode]["kernel"] = batch.describe()
         batch.close()
         for s in sims:
             s.close()

@@ -283,12 +283,19 @@ typedef struct rrtl_hip_batch rrtl_hip_batch;
 enum { RRTL_BATCH_PERSISTENT = 0, RRTL_BATCH_PER_DESIGN = 1, RRTL_BATCH_CONCURRENT = 2, RRTL_BATCH_STREAMS = 3 };
 rrtl_status rrtl_hip_batch_create(rrtl_hip_engine* const* engines, const rrtl_hip_run_spec* specs, uint32_t n,
                                   rrtl_hip_batch** out);
-/* mode RRTL_BATCH_PERSISTENT: one graph launch of the shared work-queue kernel (word designs run the record
- * interpreter inside it); RRTL_BATCH_PER_DESIGN: one launch per design, back to back (the comparison);
- * RRTL_BATCH_CONCURRENT: every design on its own kernels and stream, the fork/join captured into one graph
- * and launched once; RRTL_BATCH_STREAMS: the same launches issued directly on the streams. */
+/* mode RRTL_BATCH_PERSISTENT: one graph launch of a work-queue kernel generated for the batch and compiled at
+ * run time, with every design's item body compiled in (gate tiles at their variant, word designs' own
+ * generated code, or the record interpreter for engines without it); RRTL_BATCH_PER_DESIGN: one launch per
+ * design, back to back (the comparison); RRTL_BATCH_CONCURRENT: every design on its own kernels and stream, the
+ * fork/join captured into one graph and launched once; RRTL_BATCH_STREAMS: the same launches issued directly
+ * on the streams. */
 rrtl_status rrtl_hip_batch_run(rrtl_hip_batch* b, uint32_t mode);
 rrtl_status rrtl_hip_batch_stats(const rrtl_hip_batch* b, rrtl_hip_run_stats* out);
+/* The last persistent launch's shape as one line of JSON text:
+ * {"grid": G, "per_cu": P, "forms": ["<how design 0's items run>", ...]}.  Copies at most cap bytes
+ * (NUL-terminated) and sets *len to the text's length; a NULL out with cap 0 asks for the length only.
+ * No counterpart in the reference. */
+rrtl_status rrtl_hip_batch_describe(const rrtl_hip_batch* b, char* out, size_t cap, size_t* len);
 const char* rrtl_hip_batch_last_error(const rrtl_hip_batch* b);
 void rrtl_hip_batch_destroy(rrtl_hip_batch* b);
 

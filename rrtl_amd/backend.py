@@ -15,6 +15,7 @@ There is no fallback: if ``librrtl_hip.so`` is missing the import fails.
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 from typing import Iterable, List, Optional, Sequence, Tuple, Union
 
@@ -45,7 +46,7 @@ EXPORTS = [
     "rrtl_hip_get_digest", "rrtl_hip_trace_bytes", "rrtl_hip_get_trace", "rrtl_hip_get_halt_cycles",
     "rrtl_hip_get_detect",
     "rrtl_hip_gate_tile_info", "rrtl_hip_word_launch_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
-    "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy", "rrtl_hip_emit_kernel", "rrtl_hip_emit_split_kernel", "rrtl_hip_compile_kernel",
+    "rrtl_hip_batch_describe", "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy", "rrtl_hip_emit_kernel", "rrtl_hip_emit_split_kernel", "rrtl_hip_compile_kernel",
     "rrtl_hip_replay",
 ]
 BATCH_MODES = {"persistent": 0, "per_design": 1, "concurrent": 2, "streams": 3}
@@ -138,6 +139,7 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(RunSpec), u32, ctypes.POINTER(vp)]
     L.rrtl_hip_batch_run.argtypes = [vp, u32]
     L.rrtl_hip_batch_stats.argtypes = [vp, ctypes.POINTER(RunStats)]
+    L.rrtl_hip_batch_describe.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
     L.rrtl_hip_batch_last_error.restype = ctypes.c_char_p
     L.rrtl_hip_batch_last_error.argtypes = [vp]
     L.rrtl_hip_batch_destroy.argtypes = [vp]
@@ -507,6 +509,15 @@ class Batch:
         s = RunStats()
         self.L.rrtl_hip_batch_stats(self.h, ctypes.byref(s))
         return {"kernel_ms": s.kernel_ms, "launches": s.launches, "lane_cycles": s.lane_cycles}
+
+    def describe(self) -> dict:
+        """Shape of the last persistent launch: grid, workgroups per CU and
+        how each design's items run inside the generated kernel."""
+        n = ctypes.c_size_t()
+        self.L.rrtl_hip_batch_describe(self.h, None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        self.L.rrtl_hip_batch_describe(self.h, buf, n.value + 1, ctypes.byref(n))
+        return json.loads(buf.value.decode())
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -20,7 +20,7 @@ OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "librrtl_hip.so")
 SOURCES = ["gate_engine.hip", "word_engine.hip", "batch.hip", "mlp.hip", "capi.cpp", "rtc.cpp"]
 # device headers that kernels generated at run time include (embedded as text)
-RTC_HEADERS = ["device_defs.hpp", "word_device.hpp"]
+RTC_HEADERS = ["device_defs.hpp", "word_device.hpp", "gate_device.hpp", "batch_device.hpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RRTL_OFFLOAD_ARCH", "gfx950")

@@ -2,14 +2,25 @@
 // in one persistent, HIP-graph-captured launch.
 //
 // Each design contributes work items -- gate-engine tiles (W lane-words of one
-// bit-sliced netlist, the whole run on chip) or word-engine lane blocks (256
-// lanes of one word-level design).  Items are ordered longest first by an
-// estimate of their cost and handed out by an atomic counter, so every
-// workgroup keeps pulling work until the queue drains: short designs no longer
-// leave the GPU idle while a long one runs, and a design too small to fill 256
-// CUs shares the chip with the others.  The counter reset and the batch kernel
-// are captured once into a graph and replayed.  The alternative the batch is
-// measured against is one launch per design on one stream.
+// bit-sliced netlist, the whole run on chip) or word-engine lane blocks (64 or
+// 256 lanes of one word-level design).  Items are ordered most expensive
+// first by an estimate of their cost and handed out by an atomic counter, so
+// every workgroup keeps pulling work until the queue drains: short designs no
+// longer leave the GPU idle while a long one runs, and a design too small to
+// fill 256 CUs shares the chip with the others.
+//
+// The persistent kernel is generated per batch and compiled at run time: each
+// design's item body is compiled in at its own shape -- the gate tile body at
+// the variant (tile words, positions per stage, addressing) its tile program
+// needs, a word design's own generated straight-line code (or the record
+// interpreter when the engine runs without it) -- behind one switch on the
+// job index.  The workgroup's LDS is the largest item's, and word designs take
+// the form that fits within the gate tiles' LDS (batch_prepare), so the tiles
+// keep the residency of their own kernel; the register budget is the tile
+// kernel's (four waves per SIMD) when the batch has gate items.  The counter
+// reset and the batch kernel are captured once into a graph and replayed.
+// The alternative the batch is measured against is one launch per design on
+// one stream.
 //
 // The concurrent form keeps every design on its own kernels (the gate tile
 // kernel, the word engine's generated or interpreted kernel) and overlaps them
@@ -22,8 +33,8 @@
 // no cross-design launch.
 #include "../../include/rrtl_hip.h"
 #include "engine.hpp"
-#include "gate_device.hpp"
-#include "word_device.hpp"
+#include "batch_device.hpp"
+#include "rtc.hpp"
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -32,52 +43,28 @@
 namespace rrtl {
 namespace {
 
-struct JobDev {
-  uint32_t kind, variant, pad0, pad1;
-  union U {
-    gate::TileArgs g;
-    word::WordArgs w;
-  } u;
-};
-
-struct BatchArgs {
-  const JobDev* jobs;
-  const uint2* items;   // (job, item) longest first
-  uint32_t n_items;
-  uint32_t* counter;
-};
+using batch::BatchArgs;
+using batch::JobDev;
 
 // Clears the work counter; a kernel (not a memset node) so the graph replays it.
 __global__ void batch_reset_kernel(uint32_t* counter) {
   if (threadIdx.x == 0) *counter = 0;
 }
 
-__global__ void __launch_bounds__(256) batch_kernel(BatchArgs B) {
-  extern __shared__ __align__(16) uint32_t lds[];
-  __shared__ uint32_t s_item;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(B.counter, 1u);
-    __syncthreads();
-    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);  // uniform exit branch
-    __syncthreads();
-    if (it >= B.n_items) break;  // uniform: every wave of the workgroup leaves together
-    const uint2 wi = B.items[it];
-    const JobDev& J = B.jobs[wi.x];
-    if (J.kind == ENGINE_GATE) {
-      const uint32_t v = J.variant;
-      gate::with_tile_variant(v & 0xFFu, (v & gate::GATE_VARIANT_BYTE_ADDR) != 0, v >> gate::GATE_VARIANT_M_SHIFT,
-                              [&](auto w, auto ba, auto m) {
-                                gate::tile_body<decltype(w)::value, decltype(ba)::value, decltype(m)::value>(J.u.g, wi.y, lds);
-                              });
-    } else if (J.variant == word::WM_ALL_LDS) {
-      word::word_body<word::WM_ALL_LDS>(J.u.w, wi.y);
-    } else if (J.variant == word::WM_SCRATCH_LDS) {
-      word::word_body<word::WM_SCRATCH_LDS>(J.u.w, wi.y);
-    } else {
-      word::word_body<word::WM_HBM>(J.u.w, wi.y);
-    }
-    __syncthreads();  // LDS is reused by the next item
-  }
+// Source of the persistent kernel "rrtl_batch_jit" for the batch's jobs.
+std::string batch_kernel_source(const std::vector<BatchJob>& jobs, bool gate_budget) {
+  std::string src = "#include \"batch_device.hpp\"\n\n";
+  for (const BatchJob& J : jobs) src += J.code;
+  src += std::string("\nextern \"C\" __global__ void __launch_bounds__(256") +
+         (gate_budget ? ", " + std::to_string(gate::TILE_WAVES_PER_SIMD) : std::string()) +
+         ") rrtl_batch_jit(rrtl::batch::BatchArgs B) {\n"
+         "  extern __shared__ __align__(16) uint32_t lds[];\n"
+         "  rrtl::batch::batch_loop(B, [&](uint32_t job, uint32_t item, const rrtl::batch::JobDev::U& U) {\n"
+         "    switch (job) {\n";
+  for (size_t j = 0; j < jobs.size(); j++)
+    src += "      case " + std::to_string(j) + ": { " + jobs[j].call + " break; }\n";
+  src += "      default: break;\n    }\n  });\n}\n";
+  return src;
 }
 
 }  // namespace
@@ -115,24 +102,45 @@ class Batch {
 
   // One persistent launch for every design.
   void run_persistent() {
-    std::vector<JobDev> jobs(E.size());
+    // gate tiles first: their LDS is the budget the word designs' items must fit
+    std::vector<BatchJob> jobs(E.size());
+    size_t budget = 0;
+    bool any_gate = false;
+    for (size_t j = 0; j < E.size(); j++)
+      if (E[j]->kind() == ENGINE_GATE) {
+        jobs[j] = E[j]->batch_prepare(S[j], BatchRequest{});
+        budget = std::max(budget, jobs[j].lds_bytes);
+        any_gate = true;
+      }
+    if (!any_gate) budget = 64 * 1024;
+    for (size_t j = 0; j < E.size(); j++)
+      if (E[j]->kind() != ENGINE_GATE) {
+        BatchRequest req;
+        req.lds_budget = budget;
+        req.ns = "d" + std::to_string(j);
+        jobs[j] = E[j]->batch_prepare(S[j], req);
+      }
+    std::vector<JobDev> dev(E.size());
     std::vector<std::pair<double, uint2>> order;
     size_t lds = 0;
+    forms_.clear();
     for (size_t j = 0; j < E.size(); j++) {
-      BatchJob J = E[j]->batch_prepare(S[j]);
-      std::memset(&jobs[j], 0, sizeof(JobDev));
-      jobs[j].kind = J.kind; jobs[j].variant = J.variant;
-      std::memcpy(&jobs[j].u, J.args.data(), std::min(J.args.size(), sizeof(JobDev::U)));
+      const BatchJob& J = jobs[j];
+      std::memset(&dev[j], 0, sizeof(JobDev));
+      std::memcpy(&dev[j].u, J.args.data(), std::min(J.args.size(), sizeof(JobDev::U)));
       lds = std::max(lds, J.lds_bytes);
+      forms_.push_back(J.form);
       for (uint32_t i = 0; i < J.n_items; i++) order.push_back({J.item_cost, make_uint2(uint32_t(j), i)});
     }
-    if (lds > lds_max_) throw Error("E_HIP_TOO_LARGE: a batch item needs more LDS than a workgroup has");
+    if (lds > std::min<size_t>(lds_max_, 64 * 1024))
+      throw Error("E_HIP_TOO_LARGE: a batch item needs more than 64 KiB of LDS");
     std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
     std::vector<uint2> items(order.size());
     for (size_t k = 0; k < order.size(); k++) items[k] = order[k].second;
-    const bool same = items == last_items_ && jobs.size() == last_jobs_.size() &&
-                      std::memcmp(jobs.data(), last_jobs_.data(), jobs.size() * sizeof(JobDev)) == 0 && lds == last_lds_;
-    if (!same) rebuild(jobs, items, lds);
+    const std::string src = batch_kernel_source(jobs, any_gate);
+    const bool same = src == last_src_ && items == last_items_ && dev.size() == last_jobs_.size() &&
+                      std::memcmp(dev.data(), last_jobs_.data(), dev.size() * sizeof(JobDev)) == 0 && lds == last_lds_;
+    if (!same) rebuild(src, dev, items, lds);
     for (Engine* e : E) e->batch_reset(stream_);
     RRTL_HIP_CHECK(hipEventRecord(ev0_, stream_));
     RRTL_HIP_CHECK(hipGraphLaunch(exec_, stream_));
@@ -212,31 +220,33 @@ class Batch {
 
   const RunStats& stats() const { return stats_; }
   uint32_t grid() const { return grid_; }
+  uint32_t per_cu() const { return per_cu_; }
+  const std::vector<std::string>& forms() const { return forms_; }
 
  private:
-  void rebuild(const std::vector<JobDev>& jobs, const std::vector<uint2>& items, size_t lds) {
+  void rebuild(const std::string& src, const std::vector<JobDev>& jobs, const std::vector<uint2>& items, size_t lds) {
     if (exec_) { RRTL_HIP_CHECK(hipGraphExecDestroy(exec_)); exec_ = nullptr; }
     if (graph_) { RRTL_HIP_CHECK(hipGraphDestroy(graph_)); graph_ = nullptr; }
     if (jobs_) { RRTL_HIP_CHECK(hipFree(jobs_)); jobs_ = nullptr; }
     if (items_) { RRTL_HIP_CHECK(hipFree(items_)); items_ = nullptr; }
+    fn_ = rtc_kernel(src, "rrtl_batch_jit", device_);
     RRTL_HIP_CHECK(hipMalloc(&jobs_, jobs.size() * sizeof(JobDev)));
     RRTL_HIP_CHECK(hipMalloc(&items_, std::max<size_t>(1, items.size()) * sizeof(uint2)));
     RRTL_HIP_CHECK(hipMemcpy(jobs_, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice));
     if (!items.empty()) RRTL_HIP_CHECK(hipMemcpy(items_, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
-    if (lds > 65536)
-      RRTL_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_kernel),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     int per_cu = 0;
-    RRTL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, 256, lds));
-    grid_ = uint32_t(std::max(1, per_cu) * cus_);
+    RRTL_HIP_CHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn_, 256, lds));
+    per_cu_ = uint32_t(std::max(1, per_cu));
+    grid_ = per_cu_ * uint32_t(cus_);
     grid_ = std::min<uint32_t>(grid_, std::max<uint32_t>(1, uint32_t(items.size())));
     BatchArgs B{jobs_, items_, uint32_t(items.size()), counter_};
+    void* params[] = {&B};
     RRTL_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream_, counter_);
-    hipLaunchKernelGGL(batch_kernel, dim3(grid_), dim3(256), lds, stream_, B);
+    RRTL_HIP_CHECK(hipModuleLaunchKernel(fn_, grid_, 1, 1, 256, 1, 1, uint32_t(lds), stream_, params, nullptr));
     RRTL_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
     RRTL_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
-    last_jobs_ = jobs; last_items_ = items; last_lds_ = lds;
+    last_src_ = src; last_jobs_ = jobs; last_items_ = items; last_lds_ = lds;
   }
 
   void finish(double ms, uint64_t launches) {
@@ -265,7 +275,10 @@ class Batch {
   JobDev* jobs_ = nullptr;
   uint2* items_ = nullptr;
   uint32_t* counter_ = nullptr;
-  uint32_t grid_ = 0;
+  uint32_t grid_ = 0, per_cu_ = 0;
+  hipFunction_t fn_ = nullptr;          // the generated persistent kernel (module cached by source)
+  std::vector<std::string> forms_;      // per design: how its items run in the persistent kernel
+  std::string last_src_;
   std::vector<JobDev> last_jobs_;
   std::vector<uint2> last_items_;
   size_t last_lds_ = 0;
@@ -283,7 +296,12 @@ void batch_run(Batch* b, uint32_t mode) {
   else throw Error("E_HIP_BATCH: unknown batch mode");
 }
 const RunStats& batch_stats(const Batch* b) { return b->stats(); }
-uint32_t batch_grid(const Batch* b) { return b->grid(); }
+std::string batch_describe(const Batch* b) {
+  std::string t = "{\"grid\": " + std::to_string(b->grid()) + ", \"per_cu\": " + std::to_string(b->per_cu()) +
+                  ", \"forms\": [";
+  for (size_t j = 0; j < b->forms().size(); j++) t += (j ? ", \"" : "\"") + b->forms()[j] + "\"";
+  return t + "]}";
+}
 void batch_destroy(Batch* b) { delete b; }
 
 }  // namespace rrtl

@@ -17,6 +17,7 @@ class Batch;
 Batch* make_batch(std::vector<Engine*> engines, std::vector<RunSpec> specs);
 void batch_run(Batch* b, uint32_t mode);
 const RunStats& batch_stats(const Batch* b);
+std::string batch_describe(const Batch* b);
 void batch_destroy(Batch* b);
 std::string word_kernel_source(const Program& prog, int mode, uint32_t block);
 std::string word_split_kernel_source(const Program& prog, uint32_t lane_waves);
@@ -469,6 +470,17 @@ rrtl_status rrtl_hip_batch_stats(const rrtl_hip_batch* b, rrtl_hip_run_stats* ou
   if (!b || !b->impl || !out) return RRTL_E_INVALID;
   const rrtl::RunStats& s = rrtl::batch_stats(b->impl);
   out->kernel_ms = s.kernel_ms; out->launches = s.launches; out->lane_cycles = s.lane_cycles;
+  return RRTL_OK;
+}
+rrtl_status rrtl_hip_batch_describe(const rrtl_hip_batch* b, char* out, size_t cap, size_t* len) {
+  if (!b || !b->impl || !len || (!out && cap)) return RRTL_E_INVALID;
+  const std::string t = rrtl::batch_describe(b->impl);
+  *len = t.size();
+  if (out && cap) {
+    const size_t n = std::min(cap - 1, t.size());
+    std::memcpy(out, t.data(), n);
+    out[n] = 0;
+  }
   return RRTL_OK;
 }
 const char* rrtl_hip_batch_last_error(const rrtl_hip_batch* b) { return b ? b->err.c_str() : "E_HIP_NULL: null batch"; }

@@ -8,6 +8,8 @@
 #include <cstddef>
 #include <cstdint>
 #else
+typedef __hip_internal::int8_t int8_t;
+typedef __hip_internal::int32_t int32_t;
 typedef __hip_internal::uint32_t uint32_t;
 typedef __hip_internal::uint64_t uint64_t;
 typedef __hip_internal::size_t size_t;

@@ -26,16 +26,29 @@ struct EngineOptions {
 };
 enum : uint32_t { CODEGEN_AUTO = 0, CODEGEN_ON = 1, CODEGEN_OFF = 2 };
 
-// One design's share of a multi-design batch launch (batch.hip): the kernel
-// arguments of its run, how many work items (gate tiles or word lane blocks)
-// it has, their LDS need and an estimate of each item's cost for ordering.
+// One design's share of the persistent batch kernel (batch.hip), which is
+// generated per batch: the design's kernel arguments, its device code and the
+// statement that runs one of its work items (gate-engine tiles or word-engine
+// lane blocks), how many items it has, their LDS need and an estimate of each
+// item's cost for ordering.  In `call`, `U` names the job's argument union
+// (U.g: gate::TileArgs, U.w: word::WordArgs), `item` the item index and `lds`
+// the workgroup's dynamic LDS.
 struct BatchJob {
   uint32_t kind = 0;          // ENGINE_GATE or ENGINE_WORD
-  uint32_t variant = 0;       // gate: W | byte-address flag | M << 12; word: 1 if scratch lives in LDS
   uint32_t n_items = 0;
   size_t lds_bytes = 0;
   double item_cost = 0;
+  std::string code;           // definitions at namespace scope (the design's generated code), may be empty
+  std::string call;           // one statement
+  std::string form;           // how the items run, for launch reports
   std::vector<uint8_t> args;  // gate::TileArgs or word::WordArgs
+};
+// What the batch asks of a design's items: they must fit in `lds_budget`
+// bytes of LDS per 256-thread workgroup, and their definitions go in
+// namespace `ns` (unique within the batch).
+struct BatchRequest {
+  size_t lds_budget = 65536;
+  std::string ns;
 };
 
 class Engine {
@@ -83,9 +96,9 @@ class Engine {
     throw Error("E_HIP_UNSUPPORTED: trace replay with checks runs on the word engine");
   }
   virtual void synchronize() { RRTL_HIP_CHECK(hipStreamSynchronize(stream_)); }
-  // Multi-design batches: compile and upload a run, return its kernel
-  // arguments; reset its outputs before every launch; book-keep after it.
-  virtual BatchJob batch_prepare(const RunSpec& spec) = 0;
+  // Multi-design batches: upload a run and return its share of the generated
+  // batch kernel; reset its outputs before every launch; book-keep after it.
+  virtual BatchJob batch_prepare(const RunSpec& spec, const BatchRequest& req) = 0;
   virtual void batch_reset(hipStream_t s) = 0;
   virtual void batch_finish(const RunSpec& spec) = 0;
   // Concurrent batches: prepare a run of the engine's own kernels (uploads;

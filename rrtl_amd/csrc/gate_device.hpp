@@ -1,9 +1,12 @@
 // Device side of the bit-sliced gate engine: gate encoding, the tile program
 // arguments and the body of the LDS-resident tile kernel.  Shared by the gate
-// engine's own launches and by the multi-design batch kernel.
+// engine's own launches and by the multi-design batch kernel, which is
+// generated per batch and compiled at run time (so no host-only headers here).
 #pragma once
-#include "common.hpp"
+#include "device_defs.hpp"
+#ifndef __HIPCC_RTC__
 #include <type_traits>
+#endif
 
 namespace rrtl {
 namespace gate {
@@ -309,9 +312,6 @@ __device__ __forceinline__ void observe(const TileArgs& A, uint32_t* lds, const 
 
 constexpr uint32_t STAGE_AHEAD = 2;  // stages of descriptor look-ahead (3 or 4 measured no faster)
 constexpr uint32_t SINK_SLOTS = 64;     // per-tile slots written by lanes with no output to keep
-// batch job variant: tile words W | descriptors hold byte offsets | positions per stage M << 12
-constexpr uint32_t GATE_VARIANT_BYTE_ADDR = 0x100;
-constexpr uint32_t GATE_VARIANT_M_SHIFT = 12;
 
 constexpr int FAST_COMMITS_PER_THREAD = 8;
 constexpr int FAST_COMMIT_BATCH = 8;
@@ -580,6 +580,7 @@ __global__ void __launch_bounds__(TILE_THREADS, TILE_WAVES_PER_SIMD) tile_kernel
   tile_body<W, kBA, M>(A, blockIdx.x, lds);
 }
 
+#ifndef __HIPCC_RTC__
 // Calls f(W, byte_addr, M) with each as a std::integral_constant, for the
 // compiled variants: W in {1, 2, 4, 8}, M in {2, 3, 4}.
 template <class F>
@@ -600,6 +601,7 @@ __host__ __device__ inline void with_tile_variant(uint32_t W, bool byte_addr, ui
     default: b_(std::integral_constant<int, 8>{}); break;
   }
 }
+#endif
 
 // Work-queue schedule for runs with more tiles than the device holds at once.
 // Item i is chunk k = i / n_tiles of tile i % n_tiles: cycles [k * chunk,

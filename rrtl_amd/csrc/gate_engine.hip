@@ -1492,13 +1492,16 @@ class GateEngine final : public Engine {
     complete(async_);
   }
 
-  BatchJob batch_prepare(const RunSpec& spec) override {
+  BatchJob batch_prepare(const RunSpec& spec, const BatchRequest& req) override {
+    (void)req;  // a tile needs its LDS whatever the batch; the batch sizes its budget from the tiles
     Prepared R = prepare(spec, true);
     batch_digest_ = R.A.digest != nullptr;
     batch_detect_ = R.A.detect != nullptr;
     BatchJob J;
     J.kind = ENGINE_GATE;
-    J.variant = R.W | (R.byte_addr ? GATE_VARIANT_BYTE_ADDR : 0u) | (R.M << GATE_VARIANT_M_SHIFT);
+    J.call = "rrtl::gate::tile_body<" + std::to_string(R.W) + ", " + (R.byte_addr ? "true" : "false") + ", " +
+             std::to_string(R.M) + ">(U.g, item, lds);";
+    J.form = "gate tile W=" + std::to_string(R.W) + " M=" + std::to_string(R.M);
     J.n_items = R.tiles; J.lds_bytes = R.lds;
     // per tile and cycle: one barrier-separated pass per stage plus its gates
     J.item_cost = double(spec.cycles) * (double(R.A.n_cycle_stages + R.A.n_post_stages) * (48.0 + 8.0 * R.M) +

@@ -396,27 +396,31 @@ class WordJit {
   }
   uint32_t exchange_rows() const { return xslots_; }
   uint32_t phases(int f) const { return split_[f].phases; }
-  // Translation unit with the kernel "rrtl_word_jit" for one storage mode;
-  // after plan_waves(G) the kernel runs G waves per 64 lanes, exchanging
+  // The generated evaluations as device code in namespace rrtl::word::<ns>:
+  // jit_eval, jit_edge and the executor JitExec that word_body runs them
+  // through; after plan_waves(G) they run G waves per 64 lanes, exchanging
   // values through LDS rows from xrow0 on.
-  std::string source(int mode, uint32_t xrow0 = 0) const {
+  std::string definitions(uint32_t xrow0, const std::string& ns) const {
     const uint32_t G = waves_;
     std::string fn[2];
     for (int f = 0; f < 2; f++) fn[f] = G > 1 ? emit_split(fn_[f], split_[f], G, xrow0) : emit_serial(fn_[f]);
-    const std::string body =
-        "namespace rrtl {\nnamespace word {\nnamespace jit {\n\n"
-        "template <class Cx>\n__device__ __forceinline__ void jit_eval(const Cx& C) {\n" + fn[0] + "}\n\n"
-        "template <class Cx>\n__device__ __forceinline__ void jit_edge(const Cx& C) {\n" + fn[1] + "}\n\n"
-        "struct JitExec {\n"
-        "  template <class Cx>\n"
-        "  __device__ __forceinline__ static void run(const WordArgs&, uint32_t id, const Cx& C) {\n"
-        "    if (id == B_COMB) jit_eval(C);\n"
-        "    else if (id == B_COMMIT) jit_edge(C);\n"
-        "  }\n" + staging() +
-        "};\n\n}  // namespace jit\n}  // namespace word\n}  // namespace rrtl\n";
-    return "#include \"word_device.hpp\"\n\n" + body +
+    return "namespace rrtl {\nnamespace word {\nnamespace " + ns + " {\n\n"
+           "template <class Cx>\n__device__ __forceinline__ void jit_eval(const Cx& C) {\n" + fn[0] + "}\n\n"
+           "template <class Cx>\n__device__ __forceinline__ void jit_edge(const Cx& C) {\n" + fn[1] + "}\n\n"
+           "struct JitExec {\n"
+           "  template <class Cx>\n"
+           "  __device__ __forceinline__ static void run(const WordArgs&, uint32_t id, const Cx& C) {\n"
+           "    if (id == B_COMB) jit_eval(C);\n"
+           "    else if (id == B_COMMIT) jit_edge(C);\n"
+           "  }\n" + staging() +
+           "};\n\n}  // namespace " + ns + "\n}  // namespace word\n}  // namespace rrtl\n";
+  }
+  uint32_t waves() const { return waves_; }
+  // Translation unit with the kernel "rrtl_word_jit" for one storage mode.
+  std::string source(int mode, uint32_t xrow0 = 0) const {
+    return "#include \"word_device.hpp\"\n\n" + definitions(xrow0, "jit") +
            "\nextern \"C\" __global__ void __launch_bounds__(256) rrtl_word_jit(rrtl::word::WordArgs A) {\n"
-           "  rrtl::word::word_body<" + std::to_string(mode) + ", rrtl::word::jit::JitExec, " + std::to_string(G) +
+           "  rrtl::word::word_body<" + std::to_string(mode) + ", rrtl::word::jit::JitExec, " + std::to_string(waves_) +
            ">(A, blockIdx.x);\n}\n";
   }
 
@@ -1148,9 +1152,11 @@ class WordJit {
 
 // Memories a generated kernel stages in LDS (WM_ALL_LDS): small ones (a
 // register file, a FIFO's storage), smallest first, while a block of `block`
-// lanes keeps its scratch, signals and staged memories within 64 KiB.  Returns
+// lanes keeps its scratch, signals and staged memories within `limit` bytes
+// (64 KiB by default).  Returns
 // each memory's first LDS row (-1: stays in HBM); *used = staged words per lane.
-std::vector<int32_t> lds_memory_rows(const Program& P, uint32_t slots, uint32_t block, uint32_t* used) {
+std::vector<int32_t> lds_memory_rows(const Program& P, uint32_t slots, uint32_t block, uint32_t* used,
+                                     size_t limit = 64 * 1024) {
   std::vector<uint32_t> order(P.memories.size());
   std::iota(order.begin(), order.end(), 0u);
   auto words = [&](uint32_t m) { return uint64_t(P.memories[m].depth) * P.memories[m].limbs; };
@@ -1160,7 +1166,7 @@ std::vector<int32_t> lds_memory_rows(const Program& P, uint32_t slots, uint32_t 
   *used = 0;
   for (uint32_t m : order) {
     const uint64_t w = words(m);
-    if (w == 0 || w > 1024 || (fixed + *used + w) * block * 4 > 64 * 1024) continue;
+    if (w == 0 || w > 1024 || (fixed + *used + w) * block * 4 > limit) continue;
     row[m] = int32_t(*used);
     *used += uint32_t(w);
   }
@@ -1424,22 +1430,81 @@ class WordEngine final : public Engine {
   }
   void run(const RunSpec& spec) override { launch(run_flags(spec), spec.cycles, &spec, true); }
 
-  BatchJob batch_prepare(const RunSpec& spec) override {
-    // batch launches use 256-lane blocks; scratch (and signals) stay in LDS when they fit
-    // (batch items run the record interpreter, whatever the engine's own launches use)
-    const int mode = lds_mode(256, WP.scratch_slots);
-    if (mode == WM_HBM && !scratch_.p) { scratch_.alloc(size_t(WP.scratch_slots) * lanes_); scratch_.zero(stream_); }
+  // The design's items in the generated batch kernel, in the first form that
+  // fits the batch's LDS budget per 256-thread workgroup:
+  //   generated code, 256 lanes per item, signals (and small memories) in LDS;
+  //   generated code split between four waves per 64 lanes (values crossing
+  //     waves through LDS rows), small memories staged in LDS or not;
+  //   generated code with the state in HBM (scratch in LDS if it fits);
+  // and the record interpreter when the engine runs without generated code.
+  BatchJob batch_prepare(const RunSpec& spec, const BatchRequest& req) override {
     WordArgs A = make_args(run_flags(spec), spec.cycles, &spec, true);
-    A.n_scratch = WP.scratch_slots;
-    A.block_lanes = 0;  // 256 lanes per batch item
-    A.scratch = mode != WM_HBM ? nullptr : scratch_.p;
-    BatchJob J;
-    J.kind = ENGINE_WORD; J.variant = uint32_t(mode);
-    J.n_items = (lanes_ + 255) / 256;
-    J.lds_bytes = lds_bytes(mode, 256, WP.scratch_slots);
+    const size_t budget = std::min(req.lds_budget, lds_max_);
+    const uint32_t sig = P.total_signal_words;
     size_t recs = 0;
     for (auto& b : WP.blk) recs += b.size();
-    J.item_cost = double(spec.cycles) * double(recs) * 4.0;
+    BatchJob J;
+    J.kind = ENGINE_WORD;
+    int mode = WM_HBM;
+    uint32_t G = 1, per_item = 256, slots = 0;
+    std::string exec = "rrtl::word::InterpExec";
+    bool chosen = false;
+    if (jit_fn_) {
+      const bool scr = WordJit(WP).uses_scratch();
+      slots = scr ? WP.scratch_slots : 0;
+      if (size_t(slots + sig) * 256 * 4 <= budget) {
+        JitLayout lay;
+        uint32_t mw = 0;
+        lay.mem_row = lds_memory_rows(P, slots, 256, &mw, budget);
+        lay.stride = 256; lay.slots = slots; lay.sig_words = sig;
+        J.code = WordJit(WP, std::move(lay)).definitions(0, req.ns);
+        mode = WM_ALL_LDS;
+        J.lds_bytes = size_t(slots + sig + mw) * 256 * 4;
+        J.form = "word generated, 256 lanes per item, state in LDS";
+        chosen = true;
+      }
+      for (int stage = 1; !chosen && !scr && stage >= 0; stage--) {
+        JitLayout lay;
+        uint32_t mw = 0;
+        if (stage) lay.mem_row = lds_memory_rows(P, 0, 64, &mw, budget);
+        lay.stride = 64; lay.slots = 0; lay.sig_words = sig;
+        WordJit j(WP, std::move(lay));
+        if (j.plan_waves(4) <= 0) break;
+        const size_t rows = size_t(sig) + mw + j.exchange_rows();
+        if (rows * 64 * 4 > budget) continue;
+        J.code = j.definitions(sig + mw, req.ns);
+        mode = WM_ALL_LDS; G = 4; per_item = 64;
+        J.lds_bytes = rows * 64 * 4;
+        J.form = std::string("word generated, four waves per 64 lanes, ") +
+                 (mw ? "small memories staged in LDS" : "memories in HBM");
+        chosen = true;
+      }
+      if (!chosen) {
+        mode = size_t(slots) * 256 * 4 <= budget ? (slots ? WM_SCRATCH_LDS : WM_HBM) : WM_HBM;
+        J.code = WordJit(WP).definitions(0, req.ns);
+        J.lds_bytes = mode == WM_SCRATCH_LDS ? size_t(slots) * 256 * 4 : 0;
+        J.form = "word generated, 256 lanes per item, state in HBM";
+        chosen = true;
+      }
+      exec = "rrtl::word::" + req.ns + "::JitExec";
+    } else {
+      slots = WP.scratch_slots;
+      mode = size_t(slots + sig) * 256 * 4 <= budget ? WM_ALL_LDS
+             : size_t(slots) * 256 * 4 <= budget       ? WM_SCRATCH_LDS
+                                                        : WM_HBM;
+      J.lds_bytes = lds_bytes(mode, 256, slots);
+      J.form = "word interpreter, 256 lanes per item";
+    }
+    if (mode == WM_HBM && slots && !scratch_.p) { scratch_.alloc(size_t(WP.scratch_slots) * lanes_); scratch_.zero(stream_); }
+    A.n_scratch = slots;
+    A.block_lanes = 0;  // every thread of an item runs a lane (G = 1), or 64 lanes per item (G = 4)
+    A.scratch = mode == WM_HBM ? scratch_.p : nullptr;
+    J.call = "rrtl::word::word_body<" + std::to_string(mode) + ", " + exec + ", " + std::to_string(G) + ">(U.w, item);";
+    J.n_items = (lanes_ + per_item - 1) / per_item;
+    // cost of one item: its waves' record walks per cycle (a split item runs
+    // about twice as fast as one wave, state in HBM about half as fast)
+    const double waves = double(per_item) / 64.0 / (G > 1 ? 2.0 : 1.0) * (mode == WM_HBM ? 2.0 : 1.0);
+    J.item_cost = double(spec.cycles) * double(recs) * waves * (jit_fn_ ? 1.0 : 20.0);
     J.args.resize(sizeof(WordArgs));
     std::memcpy(J.args.data(), &A, sizeof(WordArgs));
     batch_spec_halt_ = has_halt_;
