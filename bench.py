@@ -506,7 +506,7 @@ def bench_faultsim(args) -> int:
                          "frac": pred["gemm_tflops"] / 2500.0, "traffic": None},
             "config": {"workload": "faultsim: 4000 flop stuck-at faults x 256 stimuli x 256 cycles on the 20k-gate "
                                    "netlist; surrogate trained on 512 other stimuli", "epochs": args.epochs},
-            "methods": {k: out[k] for k in ("plain", "triage_parity", "rules_dropping", "dropping")},
+            "methods": {k: out[k] for k in ("plain", "triage_parity", "triage_deferred", "rules_dropping", "dropping")},
             "triage_no_tail": out["triage"], "train": out["train"], "predict": pred,
             "features_ms": out["features_ms"], "eligible_candidates": out["eligible_candidates"]}
     print(json.dumps(line, default=float), flush=True)

@@ -32,6 +32,30 @@ def test_triage_small_netlist():
     assert out["train"]["loss"][-1] < out["train"]["loss"][0]
 
 
+def test_network_against_the_same_pipeline_without_it():
+    """The triaged pipeline with the network (scoring every candidate, or only
+    what a pool-order first stage leaves undetected) against the same exact
+    rules and stages without it, at the plain path's coverage: all three find
+    exactly the plain path's faults with zero unconfirmed detections; the
+    network's ordering finds them with no more simulated candidates than pool
+    order; the GPU times (features + scoring + exact stages) are printed
+    side by side, whichever wins."""
+    flush_print("case: triage with vs without the network, 2000-gate netlist")
+    p = lower.lower_module(designs.synthetic_netlist(gates=2000, levels=16, dffs=200, inputs=64, outputs=24, seed=11))
+    out = triage.run_experiment(p, n_train=128, n_eval=64, cycles=128, epochs=12, batch=2048, lr=1e-3,
+                                pmins=(1e-3,), log=flush_print)
+    plain = out["plain"]
+    rows = {k: out[k] for k in ("triage_parity", "triage_deferred", "rules_dropping")}
+    for k, row in rows.items():
+        assert row["unconfirmed_detections"] == 0, k
+        assert row["missed_faults"] == 0 and row["detected_faults"] == plain["detected_faults"], k
+    assert rows["triage_parity"]["candidates"] <= rows["rules_dropping"]["candidates"]
+    assert rows["triage_deferred"]["scored_candidates"] < plain["candidates"]
+    flush_print("GPU ms (features + scoring + exact): " + ", ".join(
+        f"{k} {r['gpu_ms']:.2f} (score {r['score_ms']:.2f}, {r['candidates']} simulated)" for k, r in rows.items()) +
+        f"; plain {plain['gpu_ms']:.2f}")
+
+
 def auc(score, label):
     """Probability that a detecting candidate outscores a non-detecting one."""
     order = np.argsort(score, kind="stable")
