@@ -95,7 +95,8 @@ rrtl_status rrtl_mlp_last_stats(const rrtl_mlp* m, rrtl_mlp_stats* out);
  * `iters` times on device copies of A and B; c (null: skip the copy back)
  * receives C; ms_per_launch the mean device time of one launch (split-K
  * reduction included).  tile: 0 automatic, 1 = 128 x 128 workgroup tiles,
- * 2 = 256 x 128 (M a multiple of 256). */
+ * 2 = 256 x 128 (M a multiple of 256), 3 = 256 x 256 (M and N multiples of
+ * 256). */
 rrtl_status rrtl_mlp_gemm_nt(rrtl_mlp* m, const uint16_t* a, const uint16_t* b, uint32_t M, uint32_t N, uint32_t K,
                              uint32_t tile, float* c, uint32_t iters, double* ms_per_launch);
 

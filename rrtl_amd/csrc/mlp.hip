@@ -62,6 +62,49 @@ struct GemmArgs {
   size_t split_stride;         // EPI_F32: elements between the y-slices' partial products
 };
 
+// Epilogue: the wave's TM x TN accumulator tiles, rows from mb, columns from
+// nb.  C tile layout: lane holds column (lane & 31) and rows 8 (r / 4) +
+// 4 (lane / 32) + r % 4 of each 32 x 32 tile.
+template <int EPI, int TM, int TN>
+__device__ __forceinline__ void store_acc(const GemmArgs& g, const f32x16 (&acc)[TM][TN], uint32_t mb, uint32_t nb,
+                                          uint32_t lane) {
+  float* out_f = g.out_f + (EPI == EPI_F32 ? blockIdx.y * g.split_stride : 0);
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      const uint32_t n = nb + j * 32u + (lane & 31u);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t mq = mb + i * 32u + 8u * q + 4u * (lane >> 5);
+        if constexpr (EPI == EPI_F32) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) out_f[size_t(mq + r) * g.ldf + n] = acc[i][j][4 * q + r];
+        } else {
+          u16 h[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            float v = acc[i][j][4 * q + r];
+            if constexpr (EPI == EPI_RELU) {
+              v = fmaxf(v + g.bias[n], 0.f);
+            } else {
+              const u16 a = g.act[size_t(mq + r) * g.ldo + n];
+              if ((a & 0x8000u) || a == 0) v = 0.f;
+            }
+            h[r] = f2bf(v);
+            g.out[size_t(mq + r) * g.ldo + n] = h[r];
+          }
+          if (g.out_t) {
+            uint2 pk;
+            pk.x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
+            pk.y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+            *reinterpret_cast<uint2*>(&g.out_t[size_t(n) * g.ldt + mq]) = pk;
+          }
+        }
+      }
+    }
+}
+
 template <int EPI, int TM, int TN>
 __global__ void __launch_bounds__(GT, 2) gemm_nt_kernel(GemmArgs g) {
   constexpr uint32_t BMt = 64 * TM, BNt = 64 * TN, NA = BMt / 32, NB = BNt / 32;
@@ -124,42 +167,138 @@ __global__ void __launch_bounds__(GT, 2) gemm_nt_kernel(GemmArgs g) {
         for (int j = 0; j < TN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-  // C tile layout: lane holds column (lane & 31) and rows 8 (r / 4) + 4 (lane / 32) + r % 4
-  float* out_f = g.out_f + (EPI == EPI_F32 ? blockIdx.y * g.split_stride : 0);
+  store_acc<EPI, TM, TN>(g, acc, m0 + wm * (32u * TM), n0 + wn * (32u * TN), lane);
+}
+
+// 256 x 256 workgroup tiles for large products (the surrogate's scoring of a
+// million candidates).  Four waves in a 2 x 2 grid, each owning 4 x 4 MFMA
+// 32 x 32 tiles: per 16-deep K slice a wave reads 8 fragments for 16 MFMAs
+// (half the LDS bytes per MFMA of the 256 x 128 tiles), and its 256 fp32
+// accumulators live in the AccVGPRs (one wave per SIMD).  K steps of 64 go
+// through two LDS buffers (dynamic LDS, 144 KiB: one workgroup per CU): at the
+// top of step k the operands of step k + 1, already in registers, are stored
+// into the other buffer and the global loads of step k + 2 are issued, then
+// the MFMAs of step k run from the current buffer; one barrier per step.
+constexpr uint32_t BIG = 256;
+constexpr size_t BIG_LDS = size_t(4) * BIG * LDK * sizeof(u16);  // two buffers of A and B tiles: 147,456 B
+
+// Stores the registers of one step's operand tiles into an LDS buffer: this
+// thread's 16-byte chunk of rows r, r + 32, ..., r + 224 of A, then of B.
+__device__ __forceinline__ void big_stash(u32x4* __restrict__ d, const u32x4 (&ra)[8], const u32x4 (&rb)[8]) {
+  constexpr uint32_t R32 = 32u * LDK / 8u;  // 32 LDS rows in uint4 units
 #pragma unroll
-  for (int i = 0; i < TM; i++)
+  for (uint32_t i = 0; i < 8; i++) d[i * R32] = ra[i];
 #pragma unroll
-    for (int j = 0; j < TN; j++) {
-      const uint32_t n = n0 + wn * (32u * TN) + j * 32u + (lane & 31u);
+  for (uint32_t i = 0; i < 8; i++) d[BIG * LDK / 8u + i * R32] = rb[i];
+}
+
+// One K step of gemm_nt_big_kernel: the MFMAs of the step in buffer fa / fb,
+// the next step's operands (in registers) stored through `st` into the other
+// buffer, and the global loads of step knext.  The restrict qualifiers tell
+// the compiler the fragment reads and the stores touch different buffers, so
+// it may interleave them.
+__device__ __forceinline__ void big_step(f32x16 (&acc)[4][4], const u16* __restrict__ fa, const u16* __restrict__ fb,
+                                         u32x4* __restrict__ st, u32x4 (&ra)[8], u32x4 (&rb)[8], const u16* pa,
+                                         const u16* pb, size_t sa, size_t sb, uint32_t knext) {
+  bf16x8 a[2][4], b[2][4];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const uint32_t mq = m0 + wm * (32u * TM) + i * 32u + 8u * q + 4u * (lane >> 5);
-        if constexpr (EPI == EPI_F32) {
+  for (int i = 0; i < 4; i++) {
+    a[0][i] = *reinterpret_cast<const bf16x8*>(fa + i * 32u * LDK);
+    b[0][i] = *reinterpret_cast<const bf16x8*>(fb + i * 32u * LDK);
+  }
+  big_stash(st, ra, rb);
 #pragma unroll
-          for (int r = 0; r < 4; r++) out_f[size_t(mq + r) * g.ldf + n] = acc[i][j][4 * q + r];
-        } else {
-          u16 h[4];
+  for (uint32_t i = 0; i < 8; i++) ra[i] = *reinterpret_cast<const u32x4*>(pa + i * sa + knext);
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            float v = acc[i][j][4 * q + r];
-            if constexpr (EPI == EPI_RELU) {
-              v = fmaxf(v + g.bias[n], 0.f);
-            } else {
-              const u16 a = g.act[size_t(mq + r) * g.ldo + n];
-              if ((a & 0x8000u) || a == 0) v = 0.f;
-            }
-            h[r] = f2bf(v);
-            g.out[size_t(mq + r) * g.ldo + n] = h[r];
-          }
-          if (g.out_t) {
-            uint2 pk;
-            pk.x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
-            pk.y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
-            *reinterpret_cast<uint2*>(&g.out_t[size_t(n) * g.ldt + mq]) = pk;
-          }
-        }
+  for (uint32_t i = 0; i < 8; i++) rb[i] = *reinterpret_cast<const u32x4*>(pb + i * sb + knext);
+#pragma unroll
+  for (uint32_t kk = 0; kk < BK / 16u; kk++) {
+    const uint32_t c = kk & 1u;
+    if (kk + 1 < BK / 16u) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        a[c ^ 1u][i] = *reinterpret_cast<const bf16x8*>(fa + i * 32u * LDK + (kk + 1) * 16u);
+        b[c ^ 1u][i] = *reinterpret_cast<const bf16x8*>(fb + i * 32u * LDK + (kk + 1) * 16u);
       }
     }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[c][i], b[c][j], acc[i][j], 0, 0, 0);
+  }
+  // Issue order: the first slice's 8 fragment reads, then its 16 MFMAs
+  // interleaved with the 16 stores into the other buffer, the 16 global loads
+  // and the next slice's 8 fragment reads; each later slice's MFMAs
+  // interleaved with the following slice's reads.  (Left to itself the
+  // scheduler reuses one fragment register and waits for each read right
+  // before its MFMAs.)
+  __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x200, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+  }
+  __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(GT, 1) gemm_nt_big_kernel(GemmArgs g) {
+  extern __shared__ __attribute__((aligned(16))) u16 big_lds[];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t tiles_n = g.N / BIG, nb = gridDim.x;
+  uint32_t bid = blockIdx.x;
+  // dealt round-robin over the 8 XCDs: each XCD takes a contiguous run of tiles sharing A rows
+  if (nb % 8u == 0) bid = (bid % 8u) * (nb / 8u) + bid / 8u;
+  const uint32_t m0 = (bid / tiles_n) * BIG, n0 = (bid % tiles_n) * BIG;
+  const uint32_t kbeg = blockIdx.y * g.k_chunk, kend = min(g.K, kbeg + g.k_chunk);
+  const uint32_t wm = wave >> 1, wn = wave & 1u;
+  // copy: 16-byte chunk (tid & 7) of rows (tid >> 3) + 32 i, i < 8, of each operand
+  const uint32_t crow = tid >> 3, cch = (tid & 7u) * 8u;
+  const u16* pa = g.A + size_t(m0 + crow) * g.lda + cch;
+  const u16* pb = g.B + size_t(n0 + crow) * g.ldb + cch;
+  const size_t sa = size_t(32) * g.lda, sb = size_t(32) * g.ldb;
+  u32x4 ra[8], rb[8];
+  auto load = [&](uint32_t k) {
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) ra[i] = *reinterpret_cast<const u32x4*>(pa + i * sa + k);
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) rb[i] = *reinterpret_cast<const u32x4*>(pb + i * sb + k);
+  };
+  f32x16 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+  // Each step is one basic block (big_step): the stores and loads for later
+  // steps run unconditionally (past the end they reload the last step into
+  // the buffer nobody reads again), so its scheduling groups can interleave
+  // them with the MFMAs.
+  const uint32_t klast = kend - BK;
+  load(kbeg);
+  big_stash(reinterpret_cast<u32x4*>(big_lds + crow * LDK + cch), ra, rb);
+  load(min(kbeg + BK, klast));
+  __syncthreads();
+  const uint32_t frow = lane & 31u, fk = (lane >> 5) * 8u;
+  const uint32_t fa_off = (wm * 128u + frow) * LDK + fk, fb_off = BIG * LDK + (wn * 128u + frow) * LDK + fk;
+  uint32_t cur = 0;
+  for (uint32_t k0 = kbeg; k0 < kend; k0 += BK) {
+    u16* const buf = big_lds + cur * (2 * BIG * LDK);
+    u16* const other = big_lds + (cur ^ 1u) * (2 * BIG * LDK);
+    big_step(acc, buf + fa_off, buf + fb_off, reinterpret_cast<u32x4*>(other + crow * LDK + cch), ra, rb, pa, pb, sa, sb,
+             min(k0 + 2 * BK, klast));
+    __syncthreads();  // this step's fragment reads and the next step's stores are done
+    cur ^= 1u;
+  }
+  store_acc<EPI, 4, 4>(g, acc, m0 + wm * 128u, n0 + wn * 128u, lane);
 }
 
 // out[i] = sum over s < splits of part[s * stride + i], i < n (n, stride multiples of 4)
@@ -350,25 +489,29 @@ __global__ void __launch_bounds__(256) sigmoid_kernel(const float* l, float* p, 
 }
 
 // ------------------------------------------------------------------ GEMM launch
-// Workgroup tile and K split of one product.  256 x 128 tiles when there are
-// enough of them to fill the device; fp32 products split K (powers of two)
-// while the tiles alone cannot occupy the CUs and every slice keeps at least
-// four K steps.  Shared by the network's own products and the stand-alone
-// entry points, so both run the same kernels in the same summation order.
+// Workgroup tile and K split of one product.  256 x 256 tiles when there are
+// enough of them for two rounds of one workgroup per CU, else 256 x 128 when
+// those fill the device, else 128 x 128; fp32 products split K (powers of
+// two) while the tiles alone cannot occupy the CUs and every slice keeps at
+// least four K steps.  Shared by the network's own products and the
+// stand-alone entry points, so both run the same kernels in the same
+// summation order.
 struct GemmPlan {
-  bool wide;
+  bool wide, big;
   uint32_t tiles, splits;
   size_t partial_floats(uint32_t M, uint32_t N) const { return splits > 1 ? size_t(splits) * M * N : 0; }
 };
 
 GemmPlan plan_gemm(int epi, uint32_t M, uint32_t N, uint32_t K, uint32_t tile) {
-  const bool wide = tile ? tile == 2 : (M % 256 == 0 && uint64_t(M / 256) * (N / 128) >= 1024);
+  const bool big = tile ? tile == 3 : (M % 256 == 0 && N % 256 == 0 && uint64_t(M / 256) * (N / 256) >= 512);
+  if (big && (M % 256 || N % 256)) throw Error("E_MLP_GEMM: 256 x 256 tiles need M and N to be multiples of 256");
+  const bool wide = !big && (tile ? tile == 2 : (M % 256 == 0 && uint64_t(M / 256) * (N / 128) >= 1024));
   if (wide && M % 256) throw Error("E_MLP_GEMM: 256-row tiles need M to be a multiple of 256");
-  const uint32_t tiles = wide ? (M / 256) * (N / 128) : (M / 128) * (N / 128);
+  const uint32_t tiles = big ? (M / 256) * (N / 256) : wide ? (M / 256) * (N / 128) : (M / 128) * (N / 128);
   uint32_t splits = 1;
   if (epi == EPI_F32)
     while (tiles * splits < 512 && K / (splits * 2) >= 4 * BK && (K / (splits * 2)) % BK == 0) splits *= 2;
-  return GemmPlan{wide, tiles, splits};
+  return GemmPlan{wide, big, tiles, splits};
 }
 
 // Launches the product on stream s.  With plan.splits > 1 every y-slice writes
@@ -381,7 +524,19 @@ void launch_gemm(int epi, const GemmPlan& plan, GemmArgs g, float* partials, hip
     g.out_f = partials; g.ldf = g.N; g.split_stride = size_t(g.M) * g.N;
   }
   const dim3 grid(plan.tiles, plan.splits);
-  if (plan.wide) {
+  if (plan.big) {
+    // dynamic LDS beyond 64 KiB is opted into once per kernel
+    static std::once_flag once[3];
+    std::call_once(once[epi], [&] {
+      const void* fn = epi == EPI_RELU    ? reinterpret_cast<const void*>(&gemm_nt_big_kernel<EPI_RELU>)
+                       : epi == EPI_DRELU ? reinterpret_cast<const void*>(&gemm_nt_big_kernel<EPI_DRELU>)
+                                          : reinterpret_cast<const void*>(&gemm_nt_big_kernel<EPI_F32>);
+      RRTL_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(BIG_LDS)));
+    });
+    if (epi == EPI_RELU) hipLaunchKernelGGL(gemm_nt_big_kernel<EPI_RELU>, grid, dim3(GT), BIG_LDS, s, g);
+    else if (epi == EPI_DRELU) hipLaunchKernelGGL(gemm_nt_big_kernel<EPI_DRELU>, grid, dim3(GT), BIG_LDS, s, g);
+    else hipLaunchKernelGGL(gemm_nt_big_kernel<EPI_F32>, grid, dim3(GT), BIG_LDS, s, g);
+  } else if (plan.wide) {
     if (epi == EPI_RELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_RELU, 4, 2>), grid, dim3(GT), 0, s, g);
     else if (epi == EPI_DRELU) hipLaunchKernelGGL((gemm_nt_kernel<EPI_DRELU, 4, 2>), grid, dim3(GT), 0, s, g);
     else hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 4, 2>), grid, dim3(GT), 0, s, g);
@@ -433,7 +588,7 @@ float* noalloc_scratch() {
 void gemm_nt_noalloc(const u16* a, const u16* b, uint32_t M, uint32_t N, uint32_t K, uint32_t tile, float* c,
                      hipStream_t s) {
   if (!a || !b || !c) throw Error("E_HIP_NULL: null device buffer");
-  if (tile > 2) throw Error("E_MLP_GEMM: tile must be 0 (automatic), 1 (128 x 128) or 2 (256 x 128)");
+  if (tile > 3) throw Error("E_MLP_GEMM: tile must be 0 (automatic), 1 (128 x 128), 2 (256 x 128) or 3 (256 x 256)");
   if (!M || !N || !K || M % 128 || N % 128 || K % BK) throw Error("E_MLP_GEMM: M and N must be multiples of 128, K of 64");
   const GemmPlan plan = plan_gemm(EPI_F32, M, N, K, tile);
   const size_t need = plan.partial_floats(M, N) * sizeof(float);
@@ -631,7 +786,7 @@ class Mlp {
   // launched iters times on device copies; returns the mean launch time.
   double gemm_nt(const u16* a, const u16* b, uint32_t M, uint32_t N, uint32_t K, float* c, uint32_t iters,
                  uint32_t tile) {
-    if (tile > 2) throw Error("E_MLP_GEMM: tile must be 0 (automatic), 1 (128 x 128) or 2 (256 x 128)");
+    if (tile > 3) throw Error("E_MLP_GEMM: tile must be 0 (automatic), 1 (128 x 128), 2 (256 x 128) or 3 (256 x 256)");
     if (!M || !N || !K || M % 128 || N % 128 || K % BK) throw Error("E_MLP_GEMM: M and N must be multiples of 128, K of 64");
     if (iters == 0) iters = 1;
     ga_.alloc(size_t(M) * K); gb_.alloc(size_t(N) * K); gc_.alloc(size_t(M) * N);

@@ -180,7 +180,7 @@ class Surrogate:
 
     def gemm_nt(self, a_bits: np.ndarray, b_bits: np.ndarray, iters: int = 1, want_c: bool = True, tile: int = 0):
         """C = A . B^T on the matrix cores (A [M][K], B [N][K] bf16 patterns).
-        tile: 0 automatic, 1 = 128 x 128, 2 = 256 x 128 workgroup tiles.
+        tile: 0 automatic, 1 = 128 x 128, 2 = 256 x 128, 3 = 256 x 256 workgroup tiles.
         Returns (C or None, mean ms per launch)."""
         a = np.ascontiguousarray(a_bits, dtype=np.uint16)
         b = np.ascontiguousarray(b_bits, dtype=np.uint16)

@@ -17,7 +17,9 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("M,N,K,tile", [(128, 128, 64, 0), (512, 1024, 512, 0), (1024, 512, 4096, 0),
-                                        (2048, 1024, 1024, 0), (512, 256, 1024, 2), (256, 128, 8192, 2)])
+                                        (2048, 1024, 1024, 0), (512, 256, 1024, 2), (256, 128, 8192, 2),
+                                        (256, 256, 64, 3), (512, 768, 192, 3), (131072, 1024, 512, 0),
+                                        (768, 256, 4096, 3)])
 def test_gemm_matches_float64_product(M, N, K, tile):
     """C = A . B^T of bf16 operands: every element within the fp32
     summation bound K * 2^-24 * sum |a_k b_k| of the float64 product."""
