@@ -98,6 +98,14 @@ typedef struct rrtl_hip_options {
    * 64-lane wave per SIMD and the split is estimated to pay; 1 = off.
    * Results are identical.  Read only when struct_size covers it. */
   uint32_t lane_waves;
+  /* Word engine running the record interpreter (no generated code): macro
+   * records, the counterpart of the reference's fuse_macros
+   * (crates/rrtl-gpu-sim/src/interp.rs:696-826).  Cones of one-limb
+   * operations whose intermediate values have one reader each run as one
+   * record with the intermediates in registers (at most 16 operations per
+   * record); they replace the fusion menu.  0 = automatic (on), 1 = on,
+   * 2 = off.  Results are identical.  Read only when struct_size covers it. */
+  uint32_t interp_macros;
 } rrtl_hip_options;
 
 enum { RRTL_CODEGEN_AUTO = 0, RRTL_CODEGEN_ON = 1, RRTL_CODEGEN_OFF = 2 };
@@ -265,6 +273,13 @@ rrtl_status rrtl_hip_emit_kernel(const void* program_blob, size_t blob_bytes, ui
  * cannot be split.  No counterpart in the reference.  Needs no device. */
 rrtl_status rrtl_hip_emit_split_kernel(const void* program_blob, size_t blob_bytes, uint32_t lane_waves, char* out,
                                        size_t cap, size_t* len);
+/* The record encoding the word engine's interpreter runs for a program, with
+ * macro records (interp_macros 1) or with the fusion menu (2), as counts:
+ * out4[0] records, out4[1] macro records, out4[2] operations inside macro
+ * records, out4[3] scratch slots per lane.  Needs no device.  No counterpart
+ * in the reference (its encoder reports nothing). */
+rrtl_status rrtl_hip_word_program_info(const void* program_blob, size_t blob_bytes, uint32_t interp_macros,
+                                       uint32_t* out4);
 /* Compiles HIP source for gfx950 at run time (hiprtc) against the library's
  * device headers, as the word engine does for its generated kernels; sets
  * *code_bytes to the code object's size.  Needs no device.  On failure the

@@ -47,6 +47,7 @@ EXPORTS = [
     "rrtl_hip_get_detect",
     "rrtl_hip_gate_tile_info", "rrtl_hip_word_launch_info", "rrtl_hip_batch_create", "rrtl_hip_batch_run", "rrtl_hip_batch_stats",
     "rrtl_hip_batch_describe", "rrtl_hip_batch_last_error", "rrtl_hip_batch_destroy", "rrtl_hip_emit_kernel", "rrtl_hip_emit_split_kernel", "rrtl_hip_compile_kernel",
+    "rrtl_hip_word_program_info",
     "rrtl_hip_replay",
 ]
 BATCH_MODES = {"persistent": 0, "per_design": 1, "concurrent": 2, "streams": 3}
@@ -77,7 +78,8 @@ class Options(ctypes.Structure):
                 ("block_threads", ctypes.c_uint32), ("tile_words", ctypes.c_uint32),
                 ("ticks_per_launch", ctypes.c_uint32), ("tile_chunks", ctypes.c_uint32),
                 ("stage_gates", ctypes.c_uint32), ("codegen", ctypes.c_uint32), ("wave_lanes", ctypes.c_uint32),
-                ("compact_lanes", ctypes.c_uint32), ("lane_waves", ctypes.c_uint32)]
+                ("compact_lanes", ctypes.c_uint32), ("lane_waves", ctypes.c_uint32),
+                ("interp_macros", ctypes.c_uint32)]
 
 
 class RunSpec(ctypes.Structure):
@@ -145,6 +147,7 @@ def load_library() -> ctypes.CDLL:
     L.rrtl_hip_batch_destroy.argtypes = [vp]
     L.rrtl_hip_emit_kernel.argtypes = [ctypes.c_char_p, sz, u32, u32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
     L.rrtl_hip_emit_split_kernel.argtypes = [ctypes.c_char_p, sz, u32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+    L.rrtl_hip_word_program_info.argtypes = [ctypes.c_char_p, sz, u32, ctypes.POINTER(u32)]
     L.rrtl_hip_compile_kernel.argtypes = [ctypes.c_char_p, ctypes.POINTER(sz)]
     L.rrtl_hip_replay.argtypes = [vp, ctypes.POINTER(ReplaySpec), ctypes.POINTER(u64), ctypes.POINTER(ReplayMismatch),
                                   ctypes.POINTER(u32)]
@@ -200,6 +203,18 @@ def emit_split_kernel(program: Program, lane_waves: int) -> str:
     return buf.value.decode()
 
 
+def word_program_info(program: Program, macros: bool = True) -> dict:
+    """Counts of the word engine's interpreter encoding of ``program`` (with
+    macro records or with the fusion menu).  Needs no GPU."""
+    L = load_library()
+    blob = program.to_blob()
+    out = (ctypes.c_uint32 * 4)()
+    st = L.rrtl_hip_word_program_info(blob, len(blob), 1 if macros else 2, out)
+    if st != 0:
+        raise HipError(st, L.rrtl_hip_create_error().decode())
+    return {"records": out[0], "macro_records": out[1], "macro_ops": out[2], "scratch_slots": out[3]}
+
+
 def compile_kernel(source: str) -> int:
     """Compiles HIP source for gfx950 with the library's run-time compiler;
     returns the code object's size.  Needs no GPU."""
@@ -217,13 +232,13 @@ class HipSimulator:
     def __init__(self, program: Program, lanes: int, engine: str = "auto", device: int = 0,
                  tile_words: int = 0, ticks_per_launch: int = 0, block_threads: int = 0, tile_chunks: int = 0,
                  stage_gates: int = 0, codegen: str = "auto", wave_lanes: int = 0, compact_lanes: bool = False,
-                 lane_waves: int = 0):
+                 lane_waves: int = 0, interp_macros: str = "auto"):
         self.L = load_library()
         self.program = program
         blob = program.to_blob()
         opt = Options(ctypes.sizeof(Options), ENGINE_NAMES[engine], device, block_threads, tile_words,
                       ticks_per_launch, tile_chunks, stage_gates, CODEGEN_NAMES[codegen], wave_lanes,
-                      int(bool(compact_lanes)), lane_waves)
+                      int(bool(compact_lanes)), lane_waves, {"auto": 0, "on": 1, "off": 2}[interp_macros])
         h = ctypes.c_void_p()
         st = self.L.rrtl_hip_create(blob, len(blob), lanes, ctypes.byref(opt), ctypes.byref(h))
         if st != 0:

@@ -21,6 +21,7 @@ std::string batch_describe(const Batch* b);
 void batch_destroy(Batch* b);
 std::string word_kernel_source(const Program& prog, int mode, uint32_t block);
 std::string word_split_kernel_source(const Program& prog, uint32_t lane_waves);
+void word_program_info(const Program& prog, bool macros, uint32_t* out4);
 }
 
 struct rrtl_hip_batch {
@@ -113,7 +114,10 @@ rrtl_status rrtl_hip_create(const void* blob, size_t bytes, uint32_t lanes, cons
         opt.compact_lanes = options->compact_lanes;
       if (options->struct_size >= offsetof(rrtl_hip_options, lane_waves) + sizeof(uint32_t))
         opt.lane_waves = options->lane_waves;
+      if (options->struct_size >= offsetof(rrtl_hip_options, interp_macros) + sizeof(uint32_t))
+        opt.interp_macros = options->interp_macros;
     }
+    if (opt.interp_macros > 2) throw rrtl::Error("E_HIP_OPTIONS: interp_macros must be 0, 1 or 2");
     if (opt.lane_waves > 4 || opt.lane_waves == 3)
       throw rrtl::Error("E_HIP_OPTIONS: lane_waves must be 0, 1, 2 or 4");
     if (opt.block_threads && (opt.block_threads % 64 || opt.block_threads > 256))
@@ -179,6 +183,19 @@ rrtl_status rrtl_hip_emit_kernel(const void* blob, size_t bytes, uint32_t storag
       std::memcpy(out, src.data(), n);
       out[n] = '\0';
     }
+    g_create_err.clear();
+    return RRTL_OK;
+  } catch (const std::exception& ex) {
+    g_create_err = ex.what();
+    return classify(g_create_err);
+  }
+}
+
+rrtl_status rrtl_hip_word_program_info(const void* blob, size_t bytes, uint32_t interp_macros, uint32_t* out4) {
+  try {
+    if (!blob || !out4) throw rrtl::Error("E_HIP_NULL: null argument");
+    if (interp_macros < 1 || interp_macros > 2) throw rrtl::Error("E_HIP_OPTIONS: interp_macros must be 1 or 2");
+    rrtl::word_program_info(rrtl::Program::parse(blob, bytes), interp_macros == 1, out4);
     g_create_err.clear();
     return RRTL_OK;
   } catch (const std::exception& ex) {

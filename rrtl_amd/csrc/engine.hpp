@@ -23,6 +23,7 @@ struct EngineOptions {
   uint32_t wave_lanes = 0;         // word engine: active lanes per 64-thread wave (16, 32, 64; 0: automatic)
   uint32_t compact_lanes = 0;      // word engine: pack the live lanes of halting runs between launches
   uint32_t lane_waves = 0;         // word engine, generated code: waves per 64 lanes (0: automatic, 1, 2, 4)
+  uint32_t interp_macros = 0;      // word engine, record interpreter: macro records (0: automatic = on, 1: on, 2: off)
 };
 enum : uint32_t { CODEGEN_AUTO = 0, CODEGEN_ON = 1, CODEGEN_OFF = 2 };
 

@@ -12,7 +12,20 @@ constexpr uint32_t SP_SCRATCH = 0u << 30, SP_SIGNAL = 1u << 30, SP_CONST = 2u <<
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 
 // record ops: machine ops keep their numbers, effects follow
-enum : uint32_t { R_STORE = 40, R_ASYNC_CAPTURE, R_CAPTURE, R_COMMIT, R_MEMWRITE, R_SIGLOAD };
+enum : uint32_t { R_STORE = 40, R_ASYNC_CAPTURE, R_CAPTURE, R_COMMIT, R_MEMWRITE, R_SIGLOAD, R_MACRO };
+// effect records (run by exec_effect); the others produce a value
+__host__ __device__ inline bool is_effect_op(uint32_t op) { return op >= R_STORE && op < R_SIGLOAD; }
+
+// Macro records (the interpreter's fused superoperator, the counterpart of the
+// reference's fuse_macros, crates/rrtl-gpu-sim/src/interp.rs:696-826): a cone
+// of one-limb operations whose intermediate values each have one reader runs
+// as one record, the intermediates in the record's own registers instead of
+// scratch slots.  w[2]: offset of the sub-operation records (8 words each) in
+// the constant pool, w[3]: their count (at most MACRO_REGS, the root last),
+// w[1]: the root's scratch slot.  A sub-operation writes register k (its
+// index, in w[1]); operand code SP_LOCAL | k reads it.
+constexpr uint32_t SP_LOCAL = 3u << 30;
+constexpr uint32_t MACRO_REGS = 16;
 // Fused one-limb records (the fusion menu of the reference's interpreter,
 // crates/rrtl-gpu-sim/src/interp.rs:351-495): an inner operation whose value
 // has no other reader folded into its reader, operands a, b, c in w[2..4] and
@@ -167,6 +180,17 @@ __device__ __forceinline__ uint64_t ld_usize(const Cx& C, uint32_t code, uint32_
   return out;
 }
 
+// Record i of an array.  Every lane runs the same record, and records never
+// change during a launch, so they are read through the constant address
+// space: scalar loads and scalar branches on the opcode.
+__device__ __forceinline__ Rec load_rec(const Rec* recs, uint32_t i) {
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(recs + i);
+  Rec R;
+#pragma unroll
+  for (int k = 0; k < 8; k++) R.w[k] = cload(p, k);
+  return R;
+}
+
 // One-limb results (every value of at most 32 bits): straight scalar code.
 template <class Cx>
 __device__ __forceinline__ bool exec_rec1(const Cx& C, const Rec& R) {
@@ -248,9 +272,38 @@ __device__ __forceinline__ bool exec_rec1(const Cx& C, const Rec& R) {
   return true;
 }
 
+// A macro record's view of the operands: SP_LOCAL codes name the record's
+// registers (a 16-entry vector indexed by wave-uniform record fields, so it
+// stays in VGPRs), everything else reads through the block's context; a
+// sub-operation's result goes to its register.
+typedef uint32_t MacroRegs __attribute__((ext_vector_type(MACRO_REGS)));
+template <class Cx>
+struct MacroCx {
+  const Cx& C;
+  MacroRegs& loc;
+  const uint32_t* cpool; const uint32_t* mem_table;
+  __device__ __forceinline__ uint32_t ld(uint32_t code, uint32_t limb) const {
+    if ((code & SP_MASK) == SP_LOCAL) return limb ? 0u : loc[code & (MACRO_REGS - 1)];
+    return C.ld(code, limb);
+  }
+  __device__ __forceinline__ void st(uint32_t reg, uint32_t, uint32_t v) const { loc[reg & (MACRO_REGS - 1)] = v; }
+  __device__ __forceinline__ uint32_t& mem(uint32_t base, uint64_t word) const { return C.mem(base, word); }
+};
+
+template <class Cx>
+__device__ __forceinline__ void exec_macro(const Cx& C, const Rec& R) {
+  MacroRegs loc;
+  const MacroCx<Cx> M{C, loc, C.cpool, C.mem_table};
+  const Rec* sub = reinterpret_cast<const Rec*>(C.cpool + R.w[2]);
+  const uint32_t n = R.w[3];
+  for (uint32_t k = 0; k < n; k++) exec_rec1(M, load_rec(sub, k));
+  C.st(R.w[1], 0, loc[(n - 1) & (MACRO_REGS - 1)]);
+}
+
 template <class Cx>
 __device__ __forceinline__ void exec_rec(const Cx& C, const Rec& R) {
   const uint32_t op = R.w[0] & 0xFF, L = (R.w[0] >> 8) & 0xFF, width = R.w[5];
+  if (op == R_MACRO) { exec_macro(C, R); return; }
   if (L == 1 && exec_rec1(C, R)) return;
   Limbs a, b, r;
 #pragma unroll
@@ -438,22 +491,12 @@ __device__ __forceinline__ void exec_effect(const Cx& C, const Rec& R) {
   }
 }
 
-// The interpreter loop over one block's records.  Every lane runs the same
-// record, and records never change during a launch, so they are read through
-// the constant address space: scalar loads and scalar branches on the opcode.
-__device__ __forceinline__ Rec load_rec(const Rec* recs, uint32_t i) {
-  const uint32_t* p = reinterpret_cast<const uint32_t*>(recs + i);
-  Rec R;
-#pragma unroll
-  for (int k = 0; k < 8; k++) R.w[k] = cload(p, k);
-  return R;
-}
-
+// The interpreter loop over one block's records.
 template <class Cx>
 __device__ __forceinline__ void run_records(const Rec* recs, uint32_t n, const Cx& C) {
   for (uint32_t i = 0; i < n; i++) {
     const Rec R = load_rec(recs, i);
-    if ((R.w[0] & 0xFF) >= R_STORE && (R.w[0] & 0xFF) != R_SIGLOAD) exec_effect(C, R);
+    if (is_effect_op(R.w[0] & 0xFF)) exec_effect(C, R);
     else exec_rec(C, R);
   }
 }

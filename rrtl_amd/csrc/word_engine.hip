@@ -52,7 +52,11 @@ struct Operand { uint32_t code; uint32_t limbs; uint32_t width; };
 
 class WordCompiler {
  public:
-  explicit WordCompiler(const Program& P) : P(P) {
+  // macros: cones of single-reader one-limb operations become macro records
+  // (interpreter only; replaces the fusion menu, as the reference runs
+  // fuse_macros on a menu-off encoding, interp.rs:691-695)
+  explicit WordCompiler(const Program& P, bool macros = false) : P(P), macros_(macros) {
+    fuse_ = !macros;
     for (auto& m : P.memories) out.mem_table.insert(out.mem_table.end(), {m.offset, m.depth, m.limbs, m.width});
   }
   WordProgram compile() {
@@ -166,13 +170,139 @@ class WordCompiler {
       }
   }
 
+  // Operand fields of the record of `in` (every operation but CONCAT and
+  // MEMREAD, which keep lists and memory indices): codes in w[2..4], limb
+  // counts in w[7], the width an operation reads at in w[6].
+  template <class F>
+  static void fill_operands(const Instr& in, Rec& r, F&& opnd) {
+    switch (in.op) {
+      case OP_NOT: case OP_ZEXT: case OP_TRUNC: case OP_CAST: case OP_SEXT:
+        r.w[2] = opnd(in.args[0]).code; r.w[7] = opnd(in.args[0]).limbs; r.w[6] = opnd(in.args[0]).width; break;
+      case OP_SLICE:
+        r.w[2] = opnd(in.args[0]).code; r.w[7] = opnd(in.args[0]).limbs; r.w[6] = in.args[1]; break;
+      case OP_LT:
+        r.w[2] = opnd(in.args[0]).code; r.w[3] = opnd(in.args[1]).code;
+        r.w[7] = opnd(in.args[0]).limbs | (opnd(in.args[1]).limbs << 8) | (in.args[2] ? 1u << 24 : 0);
+        r.w[6] = opnd(in.args[0]).width; break;
+      case OP_MUX:
+        r.w[2] = opnd(in.args[1]).code; r.w[3] = opnd(in.args[2]).code; r.w[4] = opnd(in.args[0]).code;
+        r.w[7] = opnd(in.args[1]).limbs | (opnd(in.args[2]).limbs << 8); break;
+      default:  // binary
+        r.w[2] = opnd(in.args[0]).code; r.w[3] = opnd(in.args[1]).code;
+        r.w[7] = opnd(in.args[0]).limbs | (opnd(in.args[1]).limbs << 8);
+        r.w[6] = std::max(opnd(in.args[0]).width, opnd(in.args[1]).width);
+        break;
+    }
+  }
+
+  // Operands a one-limb record of `in` reads, in record order.
+  static std::vector<uint32_t> rec_args(const Instr& in) {
+    switch (in.op) {
+      case OP_NOT: case OP_ZEXT: case OP_TRUNC: case OP_CAST: case OP_SEXT: case OP_SLICE: return {in.args[0]};
+      case OP_MUX: return {in.args[1], in.args[2], in.args[0]};
+      default: return {in.args[0], in.args[1]};
+    }
+  }
+  static bool macro_op(uint32_t op) {
+    switch (op) {
+      case OP_NOT: case OP_AND: case OP_OR: case OP_XOR: case OP_ADD: case OP_SUB: case OP_MUL: case OP_EQ:
+      case OP_NE: case OP_LT: case OP_MUX: case OP_SLICE: case OP_ZEXT: case OP_SEXT: case OP_TRUNC: case OP_CAST:
+        return true;
+      default: return false;
+    }
+  }
+
+  // Macro plan of a block: walking back from the last operation, every
+  // one-limb operation (of one-limb operands) not yet absorbed roots a cone
+  // that takes in each operand produced by such an operation with exactly one
+  // reader, depth first, up to MACRO_REGS operations.  The cone runs at the
+  // root's position, so an operand is taken in only if no store between its
+  // producer and the root writes a signal the producer reads; operands from
+  // outside the cone stay live until the root (ext_last).
+  void plan_macros(const Block& bk, int b, std::vector<uint8_t>& absorbed,
+                   std::unordered_map<uint32_t, std::vector<uint32_t>>& macros,
+                   std::unordered_map<uint32_t, int64_t>& ext_last) {
+    std::vector<const Instr*> def(bk.n_values, nullptr);
+    std::vector<uint32_t> uses(bk.n_values, 0);
+    std::vector<const Instr*> reader(bk.n_values, nullptr);
+    std::vector<int64_t> at(bk.n_values, -1);
+    std::unordered_map<uint32_t, std::vector<int64_t>> stores;  // signal -> positions of stores to it
+    std::vector<const Instr*> order;
+    int64_t pos = 0;
+    for (const Packet& pk : bk.packets) {
+      for (const Instr& in : pk.instrs) {
+        def[in.dst] = &in;
+        at[in.dst] = pos++;
+        order.push_back(&in);
+        if (in.op == OP_LIT || in.op == OP_SIGNAL) continue;
+        if (in.op == OP_MEMREAD) { uses[in.args[1]]++; continue; }
+        const size_t na = in.op == OP_SLICE ? 1 : in.op == OP_LT ? 2 : in.args.size();
+        for (size_t k = 0; k < na; k++) { uses[in.args[k]]++; reader[in.args[k]] = &in; }
+      }
+      for (const Effect& ef : pk.effects) {
+        if (ef.kind == EF_MEMWRITE) { uses[ef.enable] += 2; uses[ef.addr] += 2; uses[ef.data] += 2; }
+        else uses[ef.value] += 2;  // a value an effect reads is never taken into a cone
+        const bool writes = (ef.kind == EF_STORE && b != B_NEXT) ||
+                            (ef.kind == EF_CAPTURE && b == B_ASYNC && ef.has_reset && ef.reset.kind == 1);
+        if (writes) stores[ef.dst].push_back(pos);
+        pos++;
+      }
+    }
+    auto one_limb = [&](uint32_t v) { return def[v] && limbs_of(def[v]->width) == 1; };
+    auto simple = [&](const Instr& in) {
+      if (!macro_op(in.op) || limbs_of(in.width) != 1) return false;
+      for (uint32_t a : rec_args(in)) if (!one_limb(a)) return false;
+      return true;
+    };
+    auto hazard_free = [&](uint32_t v, int64_t root_at) {
+      for (uint32_t a : rec_args(*def[v])) {
+        if (def[a]->op != OP_SIGNAL) continue;
+        auto it = stores.find(def[a]->args[0]);
+        if (it == stores.end()) continue;
+        for (int64_t q : it->second) if (q > at[v] && q < root_at) return false;
+      }
+      return true;
+    };
+    for (auto it = order.rbegin(); it != order.rend(); ++it) {
+      const Instr& root = **it;
+      if (absorbed[root.dst] || !simple(root)) continue;
+      const int64_t root_at = at[root.dst];
+      std::vector<uint32_t> cone;
+      // depth: operations of the current path still waiting to be appended
+      std::function<void(uint32_t, uint32_t)> take = [&](uint32_t v, uint32_t depth) {
+        for (uint32_t a : rec_args(*def[v])) {
+          const bool inl = uses[a] == 1 && reader[a] == def[v] && simple(*def[a]) && !absorbed[a];
+          if (inl && cone.size() + depth + 1 <= MACRO_REGS && hazard_free(a, root_at)) take(a, depth + 1);
+        }
+        cone.push_back(v);
+      };
+      take(root.dst, 1);
+      if (cone.size() < 2) continue;
+      for (size_t k = 0; k + 1 < cone.size(); k++) {
+        absorbed[cone[k]] = 1;
+        for (uint32_t a : rec_args(*def[cone[k]]))
+          if (std::find(cone.begin(), cone.end(), a) == cone.end()) {
+            auto& e = ext_last[a];
+            e = std::max(e, root_at);
+          }
+      }
+      macros[root.dst] = std::move(cone);
+    }
+  }
+
   void lower(int b) {
     const Block& bk = P.blocks[b];
     std::vector<Operand> val(bk.n_values, {NONE, 0, 0});
     std::vector<uint8_t> absorbed;
     std::unordered_map<uint32_t, Fused> fused;
+    std::unordered_map<uint32_t, std::vector<uint32_t>> macros;
+    std::unordered_map<uint32_t, int64_t> ext_last;
     if (fuse_) plan_fusion(bk, absorbed, fused);
     else absorbed.assign(bk.n_values, 0);
+    if (macros_) plan_macros(bk, b, absorbed, macros, ext_last);
+    std::vector<const Instr*> defs(bk.n_values, nullptr);
+    for (const Packet& pk : bk.packets)
+      for (const Instr& in : pk.instrs) defs[in.dst] = &in;
     // liveness: last record index that reads each value
     std::vector<int64_t> last_use(bk.n_values, -1);
     int64_t pos = 0;
@@ -198,6 +328,7 @@ class WordCompiler {
         pos++;
       }
     }
+    for (const auto& e : ext_last) last_use[e.first] = std::max(last_use[e.first], e.second);
     // linear-scan slot allocation over limb ranges
     std::multimap<int64_t, std::pair<uint32_t, uint32_t>> live;  // last use -> (slot, limbs)
     std::vector<std::vector<uint32_t>> free_by_limbs(MAXL + 1);
@@ -254,18 +385,37 @@ class WordCompiler {
           pos++;
           continue;
         }
+        auto mc = macros.find(in.dst);
+        if (mc != macros.end()) {
+          // the cone's operations as 8-word records in the constant pool,
+          // each writing its own register; operands inside the cone read it
+          const std::vector<uint32_t>& cone = mc->second;
+          std::unordered_map<uint32_t, Operand> local;
+          const uint32_t at = uint32_t(out.cpool.size());
+          for (uint32_t k = 0; k < cone.size(); k++) {
+            const Instr& si = *defs[cone[k]];
+            auto sub_opnd = [&](uint32_t v) -> const Operand& {
+              auto lt = local.find(v);
+              return lt != local.end() ? lt->second : val[v];
+            };
+            Rec sr{};
+            sr.w[0] = si.op | (1u << 8);
+            sr.w[5] = si.width;
+            fill_operands(si, sr, sub_opnd);
+            sr.w[1] = k;
+            out.cpool.insert(out.cpool.end(), sr.w, sr.w + 8);
+            local[cone[k]] = {SP_LOCAL | k, 1, si.width};
+          }
+          r.w[0] = R_MACRO | (1u << 8);
+          r.w[2] = at; r.w[3] = uint32_t(cone.size());
+          uint32_t slot = alloc(L, pos, last_use[in.dst]);
+          r.w[1] = slot;
+          recs.push_back(r);
+          val[in.dst] = {SP_SCRATCH | slot, L, in.width};
+          pos++;
+          continue;
+        }
         switch (in.op) {
-          case OP_NOT: case OP_ZEXT: case OP_TRUNC: case OP_CAST: case OP_SEXT:
-            r.w[2] = opnd(in.args[0]).code; r.w[7] = opnd(in.args[0]).limbs; r.w[6] = opnd(in.args[0]).width; break;
-          case OP_SLICE:
-            r.w[2] = opnd(in.args[0]).code; r.w[7] = opnd(in.args[0]).limbs; r.w[6] = in.args[1]; break;
-          case OP_LT:
-            r.w[2] = opnd(in.args[0]).code; r.w[3] = opnd(in.args[1]).code;
-            r.w[7] = opnd(in.args[0]).limbs | (opnd(in.args[1]).limbs << 8) | (in.args[2] ? 1u << 24 : 0);
-            r.w[6] = opnd(in.args[0]).width; break;
-          case OP_MUX:
-            r.w[2] = opnd(in.args[1]).code; r.w[3] = opnd(in.args[2]).code; r.w[4] = opnd(in.args[0]).code;
-            r.w[7] = opnd(in.args[1]).limbs | (opnd(in.args[2]).limbs << 8); break;
           case OP_CONCAT: {
             uint32_t at = uint32_t(out.cpool.size());
             for (uint32_t a : in.args) { out.cpool.push_back(opnd(a).code); out.cpool.push_back(opnd(a).width | (opnd(a).limbs << 16)); }
@@ -273,10 +423,8 @@ class WordCompiler {
           }
           case OP_MEMREAD:
             r.w[6] = in.args[0]; r.w[2] = opnd(in.args[1]).code; r.w[7] = opnd(in.args[1]).limbs; break;
-          default:  // binary
-            r.w[2] = opnd(in.args[0]).code; r.w[3] = opnd(in.args[1]).code;
-            r.w[7] = opnd(in.args[0]).limbs | (opnd(in.args[1]).limbs << 8);
-            r.w[6] = std::max(opnd(in.args[0]).width, opnd(in.args[1]).width);
+          default:
+            fill_operands(in, r, opnd);
             break;
         }
         uint32_t slot = alloc(L, pos, last_use[in.dst]);
@@ -329,7 +477,8 @@ class WordCompiler {
 
   const Program& P;
   WordProgram out;
-  bool fuse_ = true;  // fused records (plan_fusion)
+  bool macros_ = false;  // macro records (plan_macros)
+  bool fuse_ = true;     // fused records (plan_fusion)
   std::map<std::pair<uint32_t, std::vector<uint32_t>>, uint32_t> cmemo_;
   std::vector<Rec> commits_;
   uint32_t capture_limbs_ = 0, max_slot_ = 0;
@@ -1130,7 +1279,8 @@ class WordJit {
   void emit_block(int b) {
     for (const Rec& R : WP.blk[b]) {
       const uint32_t op = R.w[0] & 0xFF, L = (R.w[0] >> 8) & 0xFF;
-      if (op >= R_STORE && op != R_SIGLOAD) { emit_effect(R); continue; }
+      if (op == R_MACRO) throw Error("E_HIP_INTERNAL: macro records are for the interpreter only");
+      if (is_effect_op(op)) { emit_effect(R); continue; }
       std::string e = L == 1 ? expr1(R) : "";
       if (e.empty()) { emit_generic(R); continue; }
       scr_[R.w[1]] = def('v', masked(e, R.w[5]));
@@ -1269,6 +1419,8 @@ class WordEngine final : public Engine {
     std::unique_ptr<WordJit> jit;
     if (opt.codegen == CODEGEN_ON || (opt.codegen == CODEGEN_AUTO && n_rec <= 4000)) jit.reset(new WordJit(WP));
     if (jit && opt.codegen == CODEGEN_AUTO && jit->uses_scratch() && WP.scratch_slots > 128) jit.reset();
+    // the interpreter runs macro records in place of the fusion menu
+    if (!jit && opt.interp_macros != 2) WP = WordCompiler(P, true).compile();
     run_slots_ = jit && !jit->uses_scratch() ? 0 : WP.scratch_slots;
     RRTL_HIP_CHECK(hipSetDevice(opt.device));
     hipDeviceProp_t prop;
@@ -1897,6 +2049,17 @@ std::string word_split_kernel_source(const Program& prog, uint32_t lane_waves) {
   snprintf(head, sizeof head, "// split: %u waves, estimated speed-up %.3f, exchange rows %u, phases %u/%u\n",
            lane_waves, gain, j.exchange_rows(), j.phases(0), j.phases(1));
   return head + j.source(WM_ALL_LDS, prog.total_signal_words + mem_words);
+}
+
+void word_program_info(const Program& prog, bool macros, uint32_t* out4) {
+  const WordProgram wp = WordCompiler(prog, macros).compile();
+  uint32_t recs = 0, mrec = 0, mops = 0;
+  for (auto& b : wp.blk)
+    for (const Rec& r : b) {
+      recs++;
+      if ((r.w[0] & 0xFF) == R_MACRO) { mrec++; mops += r.w[3]; }
+    }
+  out4[0] = recs; out4[1] = mrec; out4[2] = mops; out4[3] = wp.scratch_slots;
 }
 
 Engine* make_word_engine(Program prog, uint32_t lanes, const EngineOptions& opt) {

@@ -67,15 +67,17 @@ def test_counter_scalar_oracle_lanes():
             sc.tick()
 
 
-@pytest.mark.parametrize("codegen", ["on", "off"])
+@pytest.mark.parametrize("codegen", ["on", "off", "off-menu"])
 @pytest.mark.parametrize("seed", range(12))
 def test_random_designs_match_packed_oracle(seed, codegen):
-    """Generated kernel ("on") and record interpreter ("off") alike."""
+    """Generated kernel ("on"), record interpreter with macro records ("off")
+    and with the fusion menu instead ("off-menu") alike."""
     flush_print(f"case: random design seed={seed} codegen={codegen}")
     m = random_design(seed)
     p = lower.lower_module(m)
     lanes = 96
-    sim = HipSimulator(p, lanes, engine="word", codegen=codegen)
+    sim = HipSimulator(p, lanes, engine="word", codegen=codegen.split("-")[0],
+                       interp_macros="off" if codegen == "off-menu" else "on")
     ref = oracle.PackedRef(p, lanes)
     rng = random.Random(seed)
     inputs = [i for i, s in enumerate(p.signals) if s.kind == KIND_INPUT]
@@ -277,15 +279,17 @@ def test_errors_are_reported():
 
 
 @pytest.mark.parametrize("width", [8, 32])
-@pytest.mark.parametrize("codegen", ["on", "off"])
+@pytest.mark.parametrize("codegen", ["on", "off", "off-macros"])
 def test_fused_records_match_packed_oracle(width, codegen):
     """Every fused record form (and the forms that must stay unfused) against
-    the packed oracle, in the record interpreter and the generated kernel."""
+    the packed oracle, in the record interpreter (with the fusion menu, or
+    with macro records in its place) and the generated kernel."""
     from designgen import fusion_design
     flush_print(f"case: fused records width={width} codegen={codegen}")
     p = lower.lower_module(fusion_design(width))
     lanes = 128
-    sim = HipSimulator(p, lanes, engine="word", codegen=codegen)
+    sim = HipSimulator(p, lanes, engine="word", codegen=codegen.split("-")[0],
+                       interp_macros="on" if codegen == "off-macros" else "off")
     ref = oracle.PackedRef(p, lanes)
     rng = random.Random(width)
     ins = [i for i, s in enumerate(p.signals) if s.kind == KIND_INPUT]

@@ -240,3 +240,23 @@ def test_split_plans_verified_on_designs():
             except backend.HipError as e:
                 assert e.code == "E_HIP_OPTIONS", str(e)
     assert split >= 10
+
+
+def test_macro_records_form_on_designs():
+    """The interpreter's macro records (cones of single-reader one-limb
+    operations with their intermediates in registers) form on picorv32, the
+    FIFO and random designs: fewer records than the fusion-menu encoding,
+    every macro holds 2..16 operations, and fewer scratch slots are live."""
+    from rrtl_amd import backend, picorv32 as pv
+    from designgen import random_design
+    cases = [pv.program(), lower.lower_module(designs.ready_valid_fifo())]
+    cases += [lower.lower_module(random_design(seed, async_resets=True)) for seed in range(6)]
+    for p in cases:
+        mac = backend.word_program_info(p, macros=True)
+        menu = backend.word_program_info(p, macros=False)
+        assert menu["macro_records"] == 0 and menu["macro_ops"] == 0
+        assert mac["macro_records"] > 0
+        assert 2 * mac["macro_records"] <= mac["macro_ops"] <= 16 * mac["macro_records"]
+        assert mac["records"] < menu["records"] and mac["scratch_slots"] <= menu["scratch_slots"]
+    pico = backend.word_program_info(pv.program(), macros=True)
+    assert pico["records"] < 0.6 * backend.word_program_info(pv.program(), macros=False)["records"]
