@@ -17,8 +17,13 @@ WIDTHS = [1, 3, 8, 16, 31, 32, 33, 63, 64, 65, 96, 127, 128]
 
 
 def random_design(seed: int, n_inputs: int = 4, n_regs: int = 4, n_wires: int = 8, depth: int = 3,
-                  widths=WIDTHS, with_memory: bool = True, async_resets: bool = True) -> ir.Module:
+                  widths=WIDTHS, with_memory: bool = True, async_resets: bool = True,
+                  one_limb_memory: bool = False) -> ir.Module:
+    """``one_limb_memory``: the memory's words and read addresses fit one limb
+    (by default a 40-bit memory and 33-bit addresses, with the saturating
+    usize decode, are possible)."""
     rng = random.Random(seed)
+    addr_widths = [2, 4, 5] if one_limb_memory else [2, 4, 5, 33]
     m = ir.Module(f"Rand{seed}")
     rst = m.input("rst", 1)
     rstn = m.input("rstn", 1)
@@ -26,7 +31,7 @@ def random_design(seed: int, n_inputs: int = 4, n_regs: int = 4, n_wires: int = 
     regs = [m.reg(f"r{i}", BitType(rng.choice(widths), rng.random() < 0.3)) for i in range(n_regs)]
     mem = None
     if with_memory:
-        mem = m.mem("ram", rng.choice([8, 32, 40]), rng.choice([4, 16]))
+        mem = m.mem("ram", rng.choice([8, 32] if one_limb_memory else [8, 32, 40]), rng.choice([4, 16]))
     pool: List[ir.Expr] = [s.value() for s in ins + regs]
 
     def leaf(ty: BitType) -> ir.Expr:
@@ -75,7 +80,7 @@ def random_design(seed: int, n_inputs: int = 4, n_regs: int = 4, n_wires: int = 
             src = gen(BitType(rng.choice(widths), False), d - 1)
             return fit(src, ty)
         if k == 11 and mem is not None:
-            a = gen(uint(rng.choice([2, 4, 5, 33])), d - 1)
+            a = gen(uint(rng.choice(addr_widths)), d - 1)
             return fit(mem.read(a), ty)
         return leaf(ty)
 

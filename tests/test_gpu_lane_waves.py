@@ -39,7 +39,8 @@ def make_split(p, lanes, waves, **kw):
 def test_random_designs_split_match_packed_oracle(seed, waves):
     flush_print(f"case: split random design seed={seed} waves={waves}")
     # one-limb values only: wider ones keep interpreted records, which are not split
-    p = lower.lower_module(random_design(seed, async_resets=False, widths=(1, 4, 8, 13, 16, 31, 32)))
+    p = lower.lower_module(random_design(seed, async_resets=False, widths=(1, 4, 8, 13, 16, 31, 32),
+                                         one_limb_memory=True))
     lanes = 160
     sim = make_split(p, lanes, waves)
     if sim is None:
