@@ -63,6 +63,8 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=6, help="faultsim: surrogate training epochs")
     ap.add_argument("--no-autotune", action="store_true", help="mixed: default launch shapes instead of the autotuner's")
     ap.add_argument("--skip-persistent", action="store_true", help="mixed: leave out the persistent work-queue kernel")
+    ap.add_argument("--wave-lanes", type=int, default=0,
+                    help="rv32i: active lanes per wave (0: automatic)")
     ap.add_argument("--lane-waves", type=int, default=0,
                     help="rv32i: waves per 64 lanes for the generated kernel (0: automatic, 1: off, 2, 4)")
     ap.add_argument("--codegen", choices=("auto", "on", "off"), default="auto",
@@ -361,9 +363,9 @@ def bench_rv32i(args) -> int:
     else:
         job = workloads.rv32i_job(16384, args.cycles, args.seed)
         design = "single-cycle rv32i core"
-    sim = job.make(codegen=args.codegen, lane_waves=args.lane_waves)
+    sim = job.make(codegen=args.codegen, lane_waves=args.lane_waves, wave_lanes=args.wave_lanes)
     sim.run(**dict(job.run, cycles=1))   # compile the run plan
-    sim = job.make(codegen=args.codegen, lane_waves=args.lane_waves)
+    sim = job.make(codegen=args.codegen, lane_waves=args.lane_waves, wave_lanes=args.wave_lanes)
     launch = sim.launch_info()
     t0 = time.perf_counter()
     sim.run(**job.run)

@@ -601,7 +601,7 @@ constexpr uint32_t STEP_OBS = 15;  // sequence entry: observe (else a block id)
 template <int kMode, class Exec = InterpExec, int kWaves = 1>
 __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
   const uint32_t col = kWaves > 1 ? threadIdx.x & 63u : threadIdx.x;
-  const uint32_t per_block = kWaves > 1 ? 64u : (A.block_lanes ? A.block_lanes : blockDim.x);
+  const uint32_t per_block = A.block_lanes ? A.block_lanes : (kWaves > 1 ? 64u : blockDim.x);
   const uint32_t slot = block * per_block + col;
   if (col >= per_block) return;
   uint32_t lane = slot;
@@ -615,7 +615,7 @@ __device__ __forceinline__ void word_body(const WordArgs& A, uint32_t block) {
   C.values = A.values; C.mems = A.mems; C.cpool = A.cpool; C.mem_table = A.mem_table;
   C.lanes = A.lanes; C.lane = lane;
   C.scr = A.scratch + lane; C.sstride = A.lanes;
-  const uint32_t stride = kWaves > 1 ? 64u : blockDim.x;
+  const uint32_t stride = kWaves > 1 ? per_block : blockDim.x;
   C.col = col; C.stride = stride; C.sig0 = A.n_scratch * stride;
   C.mem0 = C.sig0 + A.n_sig_words * stride;
   C.wave = kWaves > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0u;

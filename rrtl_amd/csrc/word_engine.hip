@@ -1245,7 +1245,7 @@ class WordJit {
       }
       if (!S[i].def.empty()) def_of[S[i].def] = uint32_t(i);
     }
-    auto xref = [&](uint32_t q) { return "L[" + num((xrow0 + uint32_t(W.xslot[q])) * 64u) + "]"; };
+    auto xref = [&](uint32_t q) { return "L[" + num((xrow0 + uint32_t(W.xslot[q])) * lay_.stride) + "]"; };
     std::string out = header();
     for (uint32_t g = 0; g < G; g++) {
       out += "  if (C.wave == " + num(g) + ") {\n";
@@ -1451,17 +1451,21 @@ class WordEngine final : public Engine {
       uint32_t G = opt.lane_waves;
       if (!G) G = blocks64 * 4 <= simds ? 4 : (blocks64 * 2 <= simds ? 2 : 1);
       if (G > 1) {
+        // lanes per workgroup (the LDS columns the waves share): 64, or fewer
+        // when asked for (wave_lanes), which puts more workgroups on a CU
+        const uint32_t S = opt.wave_lanes ? opt.wave_lanes : 64u;
         JitLayout lay;
         uint32_t mem_words = 0;
-        lay.mem_row = lds_memory_rows(P, 0, 64, &mem_words);
-        lay.stride = 64; lay.slots = 0; lay.sig_words = P.total_signal_words;
+        lay.mem_row = lds_memory_rows(P, 0, S, &mem_words);
+        lay.stride = S; lay.slots = 0; lay.sig_words = P.total_signal_words;
         std::unique_ptr<WordJit> sj(new WordJit(WP, std::move(lay)));
         const double gain = sj->plan_waves(G);
         const size_t rows = size_t(P.total_signal_words) + mem_words + sj->exchange_rows();
-        const bool fits = rows * 64 * 4 <= std::min<size_t>(lds_max_, 64 * 1024);
+        const bool fits = rows * S * 4 <= std::min<size_t>(lds_max_, 64 * 1024);
         if (gain > 0 && fits && (opt.lane_waves || gain >= 1.2)) {
           lane_waves_ = G; lds_mem_words_ = mem_words; split_jit = std::move(sj);
-          split_lds_ = rows * 64 * 4;
+          split_lds_ = rows * S * 4;
+          split_lanes_ = S;
         } else if (opt.lane_waves) {
           throw Error(gain > 0 ? "E_HIP_OPTIONS: lane_waves: the split design does not fit in LDS"
                                : "E_HIP_OPTIONS: lane_waves: the generated code cannot be split between waves");
@@ -1478,7 +1482,7 @@ class WordEngine final : public Engine {
         wave_lanes_ /= 2;
     }
     if (lane_waves_ > 1) {
-      wave_lanes_ = 64;
+      wave_lanes_ = split_lanes_;
       block_ = 64 * lane_waves_;
     } else if (wave_lanes_ < 64) {
       block_ = 64;
@@ -1765,7 +1769,7 @@ class WordEngine final : public Engine {
   // LDS bytes a block of `block` lanes with `slots` scratch slots (and, in
   // WM_ALL_LDS, `mem_words` words of LDS-staged memories) needs in `mode`.
   // Lanes one workgroup runs.
-  uint32_t lanes_per_block() const { return lane_waves_ > 1 ? 64u : (wave_lanes_ < 64 ? wave_lanes_ : block_); }
+  uint32_t lanes_per_block() const { return lane_waves_ > 1 ? split_lanes_ : (wave_lanes_ < 64 ? wave_lanes_ : block_); }
   size_t lds_bytes(int mode, uint32_t block, uint32_t slots, uint32_t mem_words = 0) const {
     if (mode == WM_HBM) return 0;
     const size_t words = slots + (mode == WM_ALL_LDS ? size_t(P.total_signal_words) + mem_words : 0);
@@ -2002,6 +2006,7 @@ class WordEngine final : public Engine {
   uint32_t block_ = 256;
   uint32_t wave_lanes_ = 64;            // active lanes per 64-thread wave (16, 32 or 64)
   uint32_t lane_waves_ = 1;             // waves per 64 lanes (generated code split between them)
+  uint32_t split_lanes_ = 64;           // split kernel: lanes per workgroup (its LDS columns)
   size_t split_lds_ = 0;                // LDS per workgroup of the split kernel
   int mode_ = WM_SCRATCH_LDS;
   bool use_lds_ = true, has_digest_ = false, has_halt_ = false, batch_spec_halt_ = false, batch_digest_ = false;

@@ -71,20 +71,22 @@ def test_random_designs_split_match_packed_oracle(seed, waves):
             assert np.array_equal(sim.get_memory_limbs(k), ref.get_memory_limbs(k)), (seed, c, "memory")
 
 
-@pytest.mark.parametrize("waves", [2, 4])
-def test_picorv32_split_every_signal_every_cycle(waves):
+@pytest.mark.parametrize("waves,wave_lanes", [(2, 0), (4, 0), (4, 32), (4, 16), (2, 32)])
+def test_picorv32_split_every_signal_every_cycle(waves, wave_lanes):
     """300 lanes (a partial last workgroup), a different 512-instruction
     program per lane, 3000 cycles: per-lane digests of every signal at every
-    cycle and the final memories equal the host engine's."""
-    flush_print(f"case: split picorv32 digest, 300 lanes x 3000 cycles, waves={waves}")
+    cycle and the final memories equal the host engine's; with 64 lanes per
+    workgroup, or (wave_lanes) 32 or 16, the waves' other threads idle."""
+    flush_print(f"case: split picorv32 digest, 300 lanes x 3000 cycles, waves={waves} wave_lanes={wave_lanes}")
     p = pv.program()
     lanes, cycles = 300, 3000
     rng = random.Random(17)
     progs = [pv.random_program(rng, n=512) for _ in range(lanes)]
     ins = pv.inputs(p)
     obs = [i for i, s in enumerate(p.signals) if s.kind != KIND_INPUT]
-    sim = make_split(p, lanes, waves)
+    sim = make_split(p, lanes, waves, wave_lanes=wave_lanes)
     assert sim is not None
+    assert sim.launch_info()["wave_lanes"] == (wave_lanes or 64)
     sim.set_memory_limbs("mem", pv.images(progs))
     sim.run(cycles, seed=3, inputs=ins, observe=obs)
     cpu = oracle.SimdCpu(p, lanes, threads=8)
