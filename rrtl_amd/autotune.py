@@ -8,13 +8,14 @@ measured on the device, the fastest becomes a recommendation keyed by
 exactly or by the nearest (lanes, steps) with ties broken by the measured
 metric.
 
-Knobs: the gate engine's tile width (u32 lane-words per workgroup, i.e. how
-many lanes share one LDS-resident copy of the live gate values, which sets
-LDS per workgroup and so workgroups per CU), gate positions per thread per
-stage and cycles per launch; the word engine's workgroup size, active lanes
-per wave (lanes per thread below one wave's width), waves per 64 lanes (the
-generated code split between several threads per lane), cycles per launch
-and, for runs that halt, lane compaction between launches.  Shapes a design
+Knobs: the gate engine's tile width (u32 lane-words per thread, i.e. 32 to
+256 lanes per thread sharing one LDS-resident copy of the live gate values,
+which sets LDS per workgroup and so workgroups per CU), gate positions per
+thread per stage and cycles per launch; the word engine's workgroup size,
+active lanes per wave (below one wave's width), waves per 64 lanes (the
+generated code split between several threads per lane) and, with them, lanes
+per split workgroup, cycles per launch and, for runs that halt, lane
+compaction between launches.  Shapes a design
 cannot take (a split of code with interpreted records) are recorded with
 their error and skipped.
 """
@@ -62,7 +63,7 @@ def _candidates(engine: str, steps: int, probe: int, halting: bool = False) -> L
             for b in (64, 128, 256) for t in tpl] +
            [{"block_threads": 64, "wave_lanes": w, "lane_waves": 1, "ticks_per_launch": t}
             for w in (8, 16, 32) for t in tpl] +
-           [{"lane_waves": g, "ticks_per_launch": t} for g in (2, 4) for t in tpl])
+           [{"lane_waves": g, "wave_lanes": w, "ticks_per_launch": t} for g in (2, 4) for w in (0, 32) for t in tpl])
     if halting:
         ct = [t for t in (256, 1024) if 2 * t <= probe] or [max(1, probe // 2)]
         out += [{"block_threads": 0, "wave_lanes": 0, "ticks_per_launch": t, "compact_lanes": True} for t in ct]
