@@ -43,7 +43,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lanes", type=int, default=65536)
-    ap.add_argument("--cycles", type=int, default=10000)
+    ap.add_argument("--cycles", type=int, default=0,
+                    help="cycles per run (gates: 10,000; rv32i: the cycle cap, 100,000)")
     ap.add_argument("--tile-words", type=int, default=0)
     ap.add_argument("--stage-gates", type=int, default=0, help="gate positions per thread per stage (0: automatic)")
     ap.add_argument("--seed", type=int, default=1)
@@ -70,6 +71,8 @@ def main() -> int:
     ap.add_argument("--codegen", choices=("auto", "on", "off"), default="auto",
                     help="rv32i: generated per-design kernel (auto/on) or the record interpreter (off)")
     args = ap.parse_args()
+    if not args.cycles:
+        args.cycles = 100000 if args.workload in ("rv32i", "rv32i-core") else 10000
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.workload == "mixed":
@@ -385,6 +388,7 @@ def bench_rv32i(args) -> int:
     cpu = None
     if not args.no_cpu_baseline:
         cpu = rv32i_cpu_baseline(job, args)
+    uneven = rv32i_uneven_halting(args) if args.workload == "rv32i" else None
     line = {"metric": "RV32I per-lane programs: simulated lane-cycles/s (halted lanes stop)",
             "value": value, "unit": "lane-cycles/s", "n_gpus": 1,
             "higher_is_better": True, "data": "synthetic (random RV32I programs)", "dtype": "u32",
@@ -395,9 +399,41 @@ def bench_rv32i(args) -> int:
                        "lanes_at_cap": int((halt >= args.cycles).sum())},
             "roofline": {"bound": "valu", "achieved": achieved, "peak": valu_peak, "unit": "Tops/s",
                          "frac": achieved / valu_peak, "traffic": None, "ops_per_lane_cycle": ops},
-            "cpu_baseline": cpu}
+            "cpu_baseline": cpu, "uneven_halting": uneven}
     print(json.dumps(line), flush=True)
     return 0
+
+
+def rv32i_uneven_halting(args) -> dict:
+    """picorv32 with programs whose branches mostly jump forward (5 % of
+    in-range targets backward), so that most lanes reach ecall at uneven
+    cycles: the same 16,384-lane run under the cycle cap without and with lane
+    compaction between launches; halt cycles must agree."""
+    from rrtl_amd import workloads
+    out = {"workload": f"picorv32, 16384 lanes, 512-instruction programs with 5 % backward branches, cap {args.cycles}"}
+    halts = {}
+    for compact in (False, True):
+        job = workloads.picorv32_job(16384, args.cycles, args.seed, backward=0.05)
+        kw = dict(codegen=args.codegen, lane_waves=args.lane_waves, wave_lanes=args.wave_lanes)
+        if compact:
+            kw.update(compact_lanes=True, ticks_per_launch=1024)
+        sim = job.make(**kw)
+        sim.run(**dict(job.run, cycles=1))   # compile the run plan
+        sim = job.make(**kw)
+        sim.run(**job.run)
+        st = sim.stats()
+        h = sim.halt_cycles()
+        halts[compact] = h
+        key = "compacted" if compact else "uncompacted"
+        out[key] = {"kernel_ms": st["kernel_ms"], "lane_cycles_per_s": st["lane_cycles"] / (st["kernel_ms"] / 1e3),
+                    "launches": st["launches"]}
+        sim.close()
+    h = halts[False]
+    out.update(halt_cycles_equal=bool(np.array_equal(halts[False], halts[True])),
+               halt_cycles_min=int(h.min()), halt_cycles_median=int(np.median(h)), halt_cycles_max=int(h.max()),
+               lanes_at_cap=int((h >= args.cycles).sum()),
+               compaction_speedup=out["uncompacted"]["kernel_ms"] / out["compacted"]["kernel_ms"])
+    return out
 
 
 def bench_divergent(args) -> int:
