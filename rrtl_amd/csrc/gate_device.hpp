@@ -41,7 +41,7 @@ __host__ __device__ inline uint32_t gate_eval(uint32_t op, uint32_t a, uint32_t 
 // ------------------------------------------------------------------ kernels
 struct TileArgs {
   uint32_t* state; uint64_t nwords;             // HBM rows [node][nwords]
-  const uint2* desc;                            // [stage][position][thread], STAGE_AHEAD padding stages
+  const uint2* desc;                            // [stage][thread][position], STAGE_AHEAD padding stages
   uint32_t n_cycle_stages, n_post_stages, any_fix;  // any_fix: some gate applies its output inversion
   const uint32_t* load_slot; const uint32_t* load_node; uint32_t n_load;
   const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
@@ -138,10 +138,21 @@ __device__ __forceinline__ uint32_t op3_xor_or_and(uint32_t A, uint32_t B, uint3
 
 template <int M> struct StageDesc { uint2 d[M]; };
 
+// A thread's M descriptors of a stage are contiguous (8 M bytes), so the fetch
+// is one 16-byte load per two positions.
 template <int M>
 __device__ __forceinline__ void load_stage(StageDesc<M>& D, const uint2* row) {
+  const uint2* p = row + threadIdx.x * uint32_t(M);
+  if constexpr (M % 2 == 0) {
 #pragma unroll
-  for (int k = 0; k < M; k++) D.d[k] = row[k * TILE_THREADS + threadIdx.x];
+    for (int k = 0; k < M; k += 2) {
+      const uint4 q = *reinterpret_cast<const uint4*>(p + k);
+      D.d[k] = make_uint2(q.x, q.y); D.d[k + 1] = make_uint2(q.z, q.w);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; k++) D.d[k] = p[k];
+  }
 }
 
 // One stage of this thread: M gate positions over every lane-word of the tile,

@@ -1252,8 +1252,10 @@ class GateEngine final : public Engine {
     const uint32_t tiles = uint32_t((nwords_ + W - 1) / W);
     TileArgs A{};
     A.state = state_.p; A.nwords = nwords_;
-    // descriptors [stage][position][thread] as compiled, STAGE_AHEAD padding
-    // stages at the end; every field is checked against the tile's slots
+    // descriptors compiled as [stage][position][thread], uploaded as
+    // [stage][thread][position] (a thread's positions in one fetch), with
+    // STAGE_AHEAD padding stages at the end; every field is checked against
+    // the tile's slots
     const uint32_t KS = T.n_cycle_stages + T.n_post_stages;
     const size_t stride = size_t(M) * TILE_THREADS;
     auto enc = [&](uint32_t slot) {
@@ -1261,6 +1263,9 @@ class GateEngine final : public Engine {
       return byte_addr ? slot * W * 4 : slot;
     };
     std::vector<uint2> desc(size_t(KS + STAGE_AHEAD) * stride);
+    auto at = [&](size_t i) -> uint2& {  // compiled index -> uploaded index
+      return desc[i - i % stride + (i % TILE_THREADS) * M + (i % stride) / TILE_THREADS];
+    };
     for (size_t i = 0; i < desc.size(); i++) {
       if (i < size_t(KS) * stride) {
         const uint32_t x = T.desc[2 * i], y = T.desc[2 * i + 1];
@@ -1279,13 +1284,13 @@ class GateEngine final : public Engine {
               low |= (((oq >> 3) & 1u) | (((oq >> 2) & 1u) << 1)) << (2 * q);
             }
           }
-          desc[i] = make_uint2(low | (enc(x >> 16) << 16),
-                               enc(y & 0xFFFFu) | ((op & GI_A) ? 0xFF0000u : 0u) | ((op & GI_B) ? 0xFF000000u : 0u));
+          at(i) = make_uint2(low | (enc(x >> 16) << 16),
+                             enc(y & 0xFFFFu) | ((op & GI_A) ? 0xFF0000u : 0u) | ((op & GI_B) ? 0xFF000000u : 0u));
         } else {
-          desc[i] = make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16));
+          at(i) = make_uint2(enc(x & 0xFFFFu) | (enc(x >> 16) << 16), enc(y & 0xFFFFu) | (op << 16));
         }
       } else {
-        desc[i] = make_uint2(enc(T.c0) | (enc(T.c0) << 16), enc(sink + uint32_t(i % 32)));
+        at(i) = make_uint2(enc(T.c0) | (enc(T.c0) << 16), enc(sink + uint32_t(i % 32)));
       }
     }
     A.desc = upload_prog(0, desc);
