@@ -276,6 +276,38 @@ StageSchedule stage_schedule(size_t n, const std::vector<std::vector<uint32_t>>&
   return S;
 }
 
+// Order of per-thread table entries (entry i runs on thread i % 256 in round
+// i / 256) for the per-cycle tail: each half-wave's 64-bit reads land on
+// distinct slot residues mod 32 and each quarter-wave's 64-bit writes on
+// distinct residues mod 16 where the entries allow it (the bank groups of
+// gfx950's LDS for those accesses).  Greedy, position by position: the entry
+// whose residues are least used so far in its half-wave and quarter-wave,
+// writes weighted as their higher cost.  rslot empty: the entries only write.
+std::vector<uint32_t> bank_order(const std::vector<uint32_t>& rslot, const std::vector<uint32_t>& wslot) {
+  const size_t n = wslot.size();
+  const bool reads = !rslot.empty();
+  std::vector<std::vector<uint32_t>> bucket(32 * 16);  // [read residue][write residue]
+  for (size_t i = n; i-- > 0;) bucket[(reads ? rslot[i] % 32 : 0) * 16 + wslot[i] % 16].push_back(uint32_t(i));
+  std::vector<uint32_t> order;
+  order.reserve(n);
+  uint32_t rc[32] = {}, wc[16] = {};
+  for (size_t p = 0; p < n; p++) {
+    if (p % 32 == 0) std::fill(rc, rc + 32, 0u);
+    if (p % 16 == 0) std::fill(wc, wc + 16, 0u);
+    size_t best = 0;
+    uint32_t best_cost = ~0u;
+    for (size_t b = 0; b < bucket.size(); b++) {
+      if (bucket[b].empty()) continue;
+      const uint32_t cost = 3 * wc[b % 16] + rc[b / 16];
+      if (cost < best_cost) { best_cost = cost; best = b; }
+    }
+    order.push_back(bucket[best].back());
+    bucket[best].pop_back();
+    rc[best / 16]++; wc[best % 16]++;
+  }
+  return order;
+}
+
 TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec, const std::vector<uint8_t>& forced,
                          uint32_t M) {
   TileProgram T;
@@ -822,6 +854,11 @@ TileProgram compile_tile(const GateNet& N, const Program& P, const RunSpec& spec
     T.stim_stream.push_back(spec.in_sig[i] << 8);
     T.stim_arg.push_back(spec.in_arg.empty() ? 0 : spec.in_arg[i]);
   }
+  {  // which thread drives which input: conflict-free writes (entries are independent)
+    const std::vector<uint32_t> ord = bank_order({}, T.stim_slot);
+    auto permute = [&](auto& v) { auto c = v; for (size_t i = 0; i < ord.size(); i++) v[i] = c[ord[i]]; };
+    permute(T.stim_slot); permute(T.stim_mode); permute(T.stim_stream); permute(T.stim_arg);
+  }
   for (uint32_t s : spec.obs_sig) T.obs_slot.push_back(slot_of(s, spec.observe_post) | (value_pol(s, spec.observe_post) << 31));
   return T;
 }
@@ -1306,11 +1343,24 @@ class GateEngine final : public Engine {
     std::vector<uint4> cm;
     std::vector<uint32_t> cf;
     const size_t fast_cap = size_t(FAST_COMMITS_PER_THREAD) * TILE_THREADS;
+    std::vector<uint32_t> cf_reg, cf_val, cm_reg, cm_val;
     for (size_t k = 0; k < T.commit.size() / 4; k++) {
       const uint32_t reg = T.commit[4 * k], val = T.commit[4 * k + 1], rs = T.commit[4 * k + 2], fl = T.commit[4 * k + 3];
-      if (byte_addr && !(fl & 4u) && cf.size() < fast_cap) cf.push_back(enc(reg) | ((enc(val) | ((fl >> 3) & 1u)) << 16));
-      else cm.push_back(make_uint4(reg, val, rs, fl));
+      if (byte_addr && !(fl & 4u) && cf.size() < fast_cap) {
+        cf.push_back(enc(reg) | ((enc(val) | ((fl >> 3) & 1u)) << 16));
+        cf_reg.push_back(reg); cf_val.push_back(val);
+      } else {
+        cm.push_back(make_uint4(reg, val, rs, fl));
+        cm_reg.push_back(reg); cm_val.push_back(val);
+      }
     }
+    // commits of one list are independent (no value slot is a committed
+    // register's), so their order is free: spread them over the LDS banks
+    auto reorder = [](auto& v, const std::vector<uint32_t>& ord) {
+      auto c = v; for (size_t i = 0; i < ord.size(); i++) v[i] = c[ord[i]];
+    };
+    reorder(cf, bank_order(cf_val, cf_reg));
+    reorder(cm, bank_order(cm_val, cm_reg));
     A.commit = upload_prog(8, cm); A.n_commit = uint32_t(cm.size());
     A.commit_fast = upload_prog(13, cf); A.n_commit_fast = uint32_t(cf.size());
     A.obs_slot = upload_prog(9, T.obs_slot); A.n_obs = uint32_t(T.obs_slot.size());
