@@ -448,6 +448,81 @@ __device__ __forceinline__ void drive_inputs(const TileArgs& A, uint32_t* lds, c
     st_words<W>(lds, A.stim_slot[s], stimulus_of<W>(A, A.stim_mode[s], A.stim_stream[s], A.stim_arg[s], cyc, w0, gkey));
 }
 
+// The merged per-cycle tail (byte-addressed tiles without an output trace):
+// the PRE observation, the reset-free commits and the next cycle's inputs in
+// one barrier phase, ordered so that every LDS read of the phase (committed
+// values, observed values) is issued first and the input words of cycle
+// cyc + 1 are hashed while they are in flight; then the commit writes, the
+// observation's folds and the input writes.  The host enables the merged tail
+// only when observed slots, committed registers and input slots are disjoint
+// and no committed value is read from an input slot, so the reads may all go
+// ahead of the writes.
+template <int W>
+__device__ __forceinline__ void merged_tail(const TileArgs& A, uint32_t* lds, const ObsRegs& R, const FastCommits& F,
+                                            const StimReg& SR, Words<W>& sreg, Words<W>& det, uint64_t cyc, bool more,
+                                            uint64_t w0, const uint64_t* gkey) {
+  const uint32_t tid = threadIdx.x;
+  Words<W> cv[FAST_COMMITS_PER_THREAD];
+#pragma unroll
+  for (int j = 0; j < FAST_COMMITS_PER_THREAD; j++)
+    if (uint32_t(j) < F.n) cv[j] = ld_words_p<W>(lds_at<W, true>(lds, (F.e[j] >> 16) & ~1u));
+  const bool dig = R.dig_on && A.digest && A.n_obs && tid < 64;
+  Words<W> ov[OBS_REGS];
+  if (R.det_on || dig) {
+#pragma unroll
+    for (int k = 0; k < OBS_REGS; k++)
+      if (R.o[k] != NONE) ov[k] = ld_words<W>(lds, R.o[k] & 0x7FFFFFFFu);
+  }
+  const bool drive = more && SR.slot != NONE;
+  Words<W> iv;
+  if (drive) iv = stimulus_of<W>(A, SR.stream_mode & 0xFFu, SR.stream_mode & ~0xFFu, SR.arg, cyc + 1, w0, gkey);
+#pragma unroll
+  for (int j = 0; j < FAST_COMMITS_PER_THREAD; j++)
+    if (uint32_t(j) < F.n) {
+      const uint32_t e = F.e[j], inv = 0u - ((e >> 16) & 1u);
+#pragma unroll
+      for (int q = 0; q < W; q++) cv[j].v[q] ^= inv;
+      uint32_t* o = lds_at<W, true>(lds, e & 0xFFFFu);
+      if constexpr (W == 1) { o[0] = cv[j].v[0]; }
+      else if constexpr (W == 2) { *reinterpret_cast<uint2*>(o) = make_uint2(cv[j].v[0], cv[j].v[1]); }
+      else {
+#pragma unroll
+        for (int q = 0; q < W; q += 4) *reinterpret_cast<uint4*>(o + q) = make_uint4(cv[j].v[q], cv[j].v[q + 1], cv[j].v[q + 2], cv[j].v[q + 3]);
+      }
+    }
+  if (R.det_on) {
+#pragma unroll
+    for (int k = 0; k < OBS_REGS; k++)
+      if (R.o[k] != NONE)
+#pragma unroll
+        for (int q = 0; q < W; q++) det.v[q] |= ov[k].v[q] ^ (0u - (ov[k].v[q & ~1] & 1u));
+  }
+  if (dig) {
+    // lane q folds signals j = q (mod 64); S[p] takes them from lane (p - cyc) mod 64
+    Words<W> x;
+#pragma unroll
+    for (int q = 0; q < W; q++) x.v[q] = 0u;
+#pragma unroll
+    for (int k = 0; k < OBS_REGS; k++)
+      if (R.o[k] != NONE)
+#pragma unroll
+        for (int q = 0; q < W; q++) x.v[q] ^= ov[k].v[q] ^ (0u - (R.o[k] >> 31));
+    const int src = int((tid - uint32_t(cyc & 63)) & 63);
+#pragma unroll
+    for (int q = 0; q < W; q++) sreg.v[q] ^= uint32_t(__shfl(int(x.v[q]), src, 64));
+    const int pa = int((tid + 17) & 63), pb = int((tid + 41) & 63);
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      const uint32_t a = uint32_t(__shfl(int(sreg.v[k]), pa, 64)), b = uint32_t(__shfl(int(sreg.v[k]), pb, 64));
+      sreg.v[k] ^= a & ~b;
+    }
+  }
+  if (drive) st_words<W>(lds, SR.slot, iv);
+  if (more)
+    for (uint32_t s = tid + TILE_THREADS; s < A.n_stim; s += TILE_THREADS)
+      st_words<W>(lds, A.stim_slot[s], stimulus_of<W>(A, A.stim_mode[s], A.stim_stream[s], A.stim_arg[s], cyc + 1, w0, gkey));
+}
+
 // One stage: this thread's positions (descriptors fetched STAGE_AHEAD stages
 // earlier), the fetch for STAGE_AHEAD stages later into the same registers
 // (the table carries STAGE_AHEAD padding stages, so the fetch needs no bound
@@ -534,6 +609,9 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
   else F.n = 0;
   const StimReg SR = load_stim(A);
   const ObsRegs OR = load_obs(A);
+  // observations the merged tail covers: detection or the signature from
+  // register-held slots, no output trace
+  const bool fast_tail = !A.trace && (A.n_obs == 0 || ((!A.detect || OR.det_on) && (!A.digest || OR.dig_on)));
   for (uint64_t c = 0; c < A.cycles; c++) {
     const uint64_t cyc = A.cycle_base + c;
     // primary inputs for this cycle (thread per input, all words of the tile);
@@ -547,11 +625,17 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
     if (A.merge_tail) {
       // observed slots, committed registers and input slots are disjoint, and
       // no committed value is read from an input slot: one barrier phase
-      if (A.n_obs) observe_values<W>(A, lds, OR, sreg, c, cyc, w0, det);
-      if constexpr (kBA) commit_fast<W>(lds, F);
-      commit_general<W>(A, lds);
-      __builtin_amdgcn_s_setprio(3);  // as in the stages: reads issued, on to the barrier
-      if (c + 1 < A.cycles) drive_inputs<W>(A, lds, SR, cyc + 1, w0, gkey);
+      if (kBA && fast_tail) {
+        merged_tail<W>(A, lds, OR, F, SR, sreg, det, cyc, c + 1 < A.cycles, w0, gkey);
+        commit_general<W>(A, lds);
+        __builtin_amdgcn_s_setprio(3);  // as in the stages: reads issued, on to the barrier
+      } else {
+        if (A.n_obs) observe_values<W>(A, lds, OR, sreg, c, cyc, w0, det);
+        if constexpr (kBA) commit_fast<W>(lds, F);
+        commit_general<W>(A, lds);
+        __builtin_amdgcn_s_setprio(3);  // as in the stages: reads issued, on to the barrier
+        if (c + 1 < A.cycles) drive_inputs<W>(A, lds, SR, cyc + 1, w0, gkey);
+      }
       __syncthreads();
       __builtin_amdgcn_s_setprio(0);
       if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
