@@ -286,6 +286,15 @@ StageSchedule stage_schedule(size_t n, const std::vector<std::vector<uint32_t>>&
 std::vector<uint32_t> bank_order(const std::vector<uint32_t>& rslot, const std::vector<uint32_t>& wslot) {
   const size_t n = wslot.size();
   const bool reads = !rslot.empty();
+  {  // two entries writing one slot keep their order (the later one wins)
+    std::vector<uint32_t> w(wslot);
+    std::sort(w.begin(), w.end());
+    if (std::adjacent_find(w.begin(), w.end()) != w.end()) {
+      std::vector<uint32_t> id(n);
+      for (size_t i = 0; i < n; i++) id[i] = uint32_t(i);
+      return id;
+    }
+  }
   std::vector<std::vector<uint32_t>> bucket(32 * 16);  // [read residue][write residue]
   for (size_t i = n; i-- > 0;) bucket[(reads ? rslot[i] % 32 : 0) * 16 + wslot[i] % 16].push_back(uint32_t(i));
   std::vector<uint32_t> order;
