@@ -286,7 +286,7 @@ StageSchedule stage_schedule(size_t n, const std::vector<std::vector<uint32_t>>&
 std::vector<uint32_t> bank_order(const std::vector<uint32_t>& rslot, const std::vector<uint32_t>& wslot) {
   const size_t n = wslot.size();
   const bool reads = !rslot.empty();
-  {  // two entries writing one slot keep their order (the later one wins)
+  {  // entries that write one slot twice keep the table order
     std::vector<uint32_t> w(wslot);
     std::sort(w.begin(), w.end());
     if (std::adjacent_find(w.begin(), w.end()) != w.end()) {
