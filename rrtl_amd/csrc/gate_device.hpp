@@ -524,12 +524,14 @@ __device__ __forceinline__ void merged_tail(const TileArgs& A, uint32_t* lds, co
 }
 
 // One stage: this thread's positions (descriptors fetched STAGE_AHEAD stages
-// earlier), the fetch for STAGE_AHEAD stages later into the same registers
-// (the table carries STAGE_AHEAD padding stages, so the fetch needs no bound
-// check), the stage barrier and, when forces exist, their clamps.  No other
-// global access happens in this loop, so the fetches stay in flight.  The
-// fetch address is the uniform row of stage S + STAGE_AHEAD plus the thread's
-// fixed offset.  kFix: some stage has gates that apply their output inversion.
+// earlier), the fetch for STAGE_AHEAD stages later into the same registers,
+// the stage barrier and, when forces exist, their clamps.  No other global
+// access happens in this loop, so the fetches stay in flight.  The fetch
+// address is the uniform row of stage S + STAGE_AHEAD plus the thread's fixed
+// offset.  The table's STAGE_AHEAD rows past the last stage repeat its first
+// rows, so the fetches of a cycle's last stages bring in the next cycle's
+// first descriptors (and need no bound check).  kFix: some stage has gates
+// that apply their output inversion.
 template <int W, bool kBA, int M, bool kClamp, bool kFix>
 __device__ __forceinline__ void stage_step(const TileArgs& A, uint32_t* lds, uint32_t S, StageDesc<M>& D, uint32_t fb,
                                            uint32_t fe) {
@@ -540,33 +542,31 @@ __device__ __forceinline__ void stage_step(const TileArgs& A, uint32_t* lds, uin
   if (kClamp) { apply_clamps<W>(A, lds, S + 1, fb, fe); __syncthreads(); }
 }
 
-// Stages [S0, S1): unrolled by STAGE_AHEAD so each descriptor register set is
-// loaded and consumed in a fixed step (no register rotation).
+// Stages [S0, S1).  Stage S runs on descriptor register set D[S % 2] (two
+// sets: STAGE_AHEAD = 2), which its fetch then fills with stage S + 2; the
+// sets live across the run's cycles (tile_body).
+static_assert(STAGE_AHEAD == 2, "descriptor sets are indexed by stage parity");
 template <int W, bool kBA, int M, bool kClamp, bool kFix>
 __device__ __forceinline__ void run_stages(const TileArgs& A, uint32_t* lds, uint32_t S0, uint32_t S1, uint32_t fb,
-                                           uint32_t fe) {
-  StageDesc<M> D[STAGE_AHEAD];
-#pragma unroll
-  for (uint32_t q = 0; q < STAGE_AHEAD; q++) load_stage<M>(D[q], A.desc + size_t(S0 + q) * (uint32_t(M) * TILE_THREADS));
+                                           uint32_t fe, StageDesc<M> (&D)[2]) {
   uint32_t S = S0;
-  for (; S + STAGE_AHEAD <= S1; S += STAGE_AHEAD) {
-#pragma unroll
-    for (uint32_t q = 0; q < STAGE_AHEAD; q++) stage_step<W, kBA, M, kClamp, kFix>(A, lds, S + q, D[q], fb, fe);
+  if ((S & 1u) && S < S1) { stage_step<W, kBA, M, kClamp, kFix>(A, lds, S, D[1], fb, fe); S++; }
+  for (; S + 2 <= S1; S += 2) {
+    stage_step<W, kBA, M, kClamp, kFix>(A, lds, S, D[0], fb, fe);
+    stage_step<W, kBA, M, kClamp, kFix>(A, lds, S + 1, D[1], fb, fe);
   }
-#pragma unroll
-  for (uint32_t q = 0; q < STAGE_AHEAD; q++)
-    if (S + q < S1) stage_step<W, kBA, M, kClamp, kFix>(A, lds, S + q, D[q], fb, fe);
+  if (S < S1) stage_step<W, kBA, M, kClamp, kFix>(A, lds, S, D[0], fb, fe);
 }
 
 template <int W, bool kBA, int M>
 __device__ __forceinline__ void run_program(const TileArgs& A, uint32_t* lds, uint32_t S0, uint32_t S1, uint32_t fb,
-                                            uint32_t fe) {
+                                            uint32_t fe, StageDesc<M> (&D)[2]) {
   if (fe > fb) {
-    if (A.any_fix) run_stages<W, kBA, M, true, true>(A, lds, S0, S1, fb, fe);
-    else run_stages<W, kBA, M, true, false>(A, lds, S0, S1, fb, fe);
+    if (A.any_fix) run_stages<W, kBA, M, true, true>(A, lds, S0, S1, fb, fe, D);
+    else run_stages<W, kBA, M, true, false>(A, lds, S0, S1, fb, fe, D);
   } else {
-    if (A.any_fix) run_stages<W, kBA, M, false, true>(A, lds, S0, S1, fb, fe);
-    else run_stages<W, kBA, M, false, false>(A, lds, S0, S1, fb, fe);
+    if (A.any_fix) run_stages<W, kBA, M, false, true>(A, lds, S0, S1, fb, fe, D);
+    else run_stages<W, kBA, M, false, false>(A, lds, S0, S1, fb, fe, D);
   }
 }
 
@@ -612,6 +612,11 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
   // observations the merged tail covers: detection or the signature from
   // register-held slots, no output trace
   const bool fast_tail = !A.trace && (A.n_obs == 0 || ((!A.detect || OR.det_on) && (!A.digest || OR.dig_on)));
+  // descriptor sets of stages 0 and 1; each cycle's last fetches refill them
+  // with the repeated rows, in the order of the stage count's parity
+  StageDesc<M> D[2];
+  load_stage<M>(D[0], A.desc);
+  load_stage<M>(D[1], A.desc + size_t(M) * TILE_THREADS);
   for (uint64_t c = 0; c < A.cycles; c++) {
     const uint64_t cyc = A.cycle_base + c;
     // primary inputs for this cycle (thread per input, all words of the tile);
@@ -621,7 +626,7 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
       __syncthreads();
     }
     // cycle program, observe before the edge, commit, post program
-    run_program<W, kBA, M>(A, lds, 0, A.n_cycle_stages, fb, fm);
+    run_program<W, kBA, M>(A, lds, 0, A.n_cycle_stages, fb, fm, D);
     if (A.merge_tail) {
       // observed slots, committed registers and input slots are disjoint, and
       // no committed value is read from an input slot: one barrier phase
@@ -645,7 +650,7 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
       commit_general<W>(A, lds);
       __syncthreads();
       if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
-      if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fm);
+      if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fm, D);
       if (A.observe_post && A.n_obs) observe<W>(A, lds, OR, sreg, c, cyc, w0, det);
     }
   }

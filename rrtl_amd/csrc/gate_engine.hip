@@ -1300,7 +1300,7 @@ class GateEngine final : public Engine {
     A.state = state_.p; A.nwords = nwords_;
     // descriptors compiled as [stage][position][thread], uploaded as
     // [stage][thread][position] (a thread's positions in one fetch), with
-    // STAGE_AHEAD padding stages at the end; every field is checked against
+    // STAGE_AHEAD rows past the last stage; every field is checked against
     // the tile's slots
     const uint32_t KS = T.n_cycle_stages + T.n_post_stages;
     const size_t stride = size_t(M) * TILE_THREADS;
@@ -1338,6 +1338,12 @@ class GateEngine final : public Engine {
       } else {
         at(i) = make_uint2(enc(T.c0) | (enc(T.c0) << 16), enc(sink + uint32_t(i % 32)));
       }
+    }
+    // the rows past the last stage repeat stage (row % 2): a cycle's last
+    // fetches then leave the next cycle's first two descriptor sets in place
+    for (uint32_t q = 0; q < STAGE_AHEAD; q++) {
+      const size_t src = (KS + q) % 2;
+      if (src < KS) std::copy(desc.begin() + src * stride, desc.begin() + (src + 1) * stride, desc.begin() + (KS + q) * stride);
     }
     A.desc = upload_prog(0, desc);
     A.any_fix = 0;
