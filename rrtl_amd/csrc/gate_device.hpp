@@ -217,6 +217,32 @@ __device__ __forceinline__ void apply_clamps(const TileArgs& A, uint32_t* lds, u
   }
 }
 
+// The tile's commit clamps (forced registers, entries [fm, fe), all at
+// position CLAMP_COMMIT) held one per thread in registers for the run when
+// there are at most 256 of them, so the per-cycle clamp reads no table.
+struct CommitClamp { uint32_t off, andm, orm; bool on, held; };
+
+template <int W>
+__device__ __forceinline__ CommitClamp load_commit_clamp(const TileArgs& A, uint32_t fm, uint32_t fe) {
+  CommitClamp c{0u, ~0u, 0u, false, fe - fm <= TILE_THREADS};
+  const uint32_t i = fm + threadIdx.x;
+  if (c.held && i < fe) {
+    const uint32_t* f = &A.force[size_t(i) * 5];
+    c.off = f[1] * W + f[2]; c.andm = f[3]; c.orm = f[4]; c.on = true;
+  }
+  return c;
+}
+
+template <int W>
+__device__ __forceinline__ void apply_commit_clamps(const TileArgs& A, uint32_t* lds, const CommitClamp& c,
+                                                    uint32_t fm, uint32_t fe) {
+  if (c.held) {
+    if (c.on) lds[c.off] = (lds[c.off] & c.andm) | c.orm;
+  } else {
+    apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe);
+  }
+}
+
 // Signature of the observed outputs: per cycle c, S[(j + c) & 63] ^= v_j for every
 // observed word v_j, then one nonlinear round S'[p] = S[p] ^ (S[p+17] & ~S[p+41]).
 // Bitwise, so every lane keeps its own 64-bit signature.  Lane p of wave 0
@@ -614,6 +640,7 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
   const bool fast_tail = !A.trace && (A.n_obs == 0 || ((!A.detect || OR.det_on) && (!A.digest || OR.dig_on)));
   // descriptor sets of stages 0 and 1; each cycle's last fetches refill them
   // with the repeated rows, in the order of the stage count's parity
+  const CommitClamp CC = load_commit_clamp<W>(A, fm, fe);
   StageDesc<M> D[2];
   load_stage<M>(D[0], A.desc);
   load_stage<M>(D[1], A.desc + size_t(M) * TILE_THREADS);
@@ -643,13 +670,13 @@ __device__ void tile_body(const TileArgs& A, uint32_t tile, uint32_t* lds) {
       }
       __syncthreads();
       __builtin_amdgcn_s_setprio(0);
-      if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
+      if (fe > fm) { apply_commit_clamps<W>(A, lds, CC, fm, fe); __syncthreads(); }
     } else {
       if (!A.observe_post && A.n_obs) observe<W>(A, lds, OR, sreg, c, cyc, w0, det);
       if constexpr (kBA) commit_fast<W>(lds, F);
       commit_general<W>(A, lds);
       __syncthreads();
-      if (fe > fm) { apply_clamps<W>(A, lds, CLAMP_COMMIT, fm, fe); __syncthreads(); }
+      if (fe > fm) { apply_commit_clamps<W>(A, lds, CC, fm, fe); __syncthreads(); }
       if (A.n_post_stages) run_program<W, kBA, M>(A, lds, A.n_cycle_stages, nst, fb, fm, D);
       if (A.observe_post && A.n_obs) observe<W>(A, lds, OR, sreg, c, cyc, w0, det);
     }
