@@ -65,10 +65,18 @@ struct GemmArgs {
 // Epilogue: the wave's TM x TN accumulator tiles, rows from mb, columns from
 // nb.  C tile layout: lane holds column (lane & 31) and rows 8 (r / 4) +
 // 4 (lane / 32) + r % 4 of each 32 x 32 tile.
+// The bias values of the lane's TN columns are read before any store: a load
+// issued after stores makes its wait count the stores too (the vector-memory
+// counter is in order), which stalled every group on the stores before it.
 template <int EPI, int TM, int TN>
 __device__ __forceinline__ void store_acc(const GemmArgs& g, const f32x16 (&acc)[TM][TN], uint32_t mb, uint32_t nb,
                                           uint32_t lane) {
   float* out_f = g.out_f + (EPI == EPI_F32 ? blockIdx.y * g.split_stride : 0);
+  float bias[TN];
+  if constexpr (EPI == EPI_RELU) {
+#pragma unroll
+    for (int j = 0; j < TN; j++) bias[j] = g.bias[nb + j * 32u + (lane & 31u)];
+  }
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -86,7 +94,7 @@ __device__ __forceinline__ void store_acc(const GemmArgs& g, const f32x16 (&acc)
           for (int r = 0; r < 4; r++) {
             float v = acc[i][j][4 * q + r];
             if constexpr (EPI == EPI_RELU) {
-              v = fmaxf(v + g.bias[n], 0.f);
+              v = fmaxf(v + bias[j], 0.f);
             } else {
               const u16 a = g.act[size_t(mq + r) * g.ldo + n];
               if ((a & 0x8000u) || a == 0) v = 0.f;
