@@ -77,6 +77,26 @@ __device__ __forceinline__ void store_acc(const GemmArgs& g, const f32x16 (&acc)
 #pragma unroll
     for (int j = 0; j < TN; j++) bias[j] = g.bias[nb + j * 32u + (lane & 31u)];
   }
+  // ReLU derivative: which of the lane's 16 values per tile keep their
+  // gradient (forward activation > 0), one bit each, also read before any store
+  uint32_t keep[TM][TN];
+  if constexpr (EPI == EPI_DRELU) {
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+      for (int j = 0; j < TN; j++) {
+        const uint32_t n = nb + j * 32u + (lane & 31u);
+        uint32_t m = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const u16 a = g.act[size_t(mb + i * 32u + 8u * q + 4u * (lane >> 5) + r) * g.ldo + n];
+            m |= ((a & 0x8000u) || a == 0) ? 0u : (1u << (4 * q + r));
+          }
+        keep[i][j] = m;
+      }
+  }
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -96,8 +116,7 @@ __device__ __forceinline__ void store_acc(const GemmArgs& g, const f32x16 (&acc)
             if constexpr (EPI == EPI_RELU) {
               v = fmaxf(v + bias[j], 0.f);
             } else {
-              const u16 a = g.act[size_t(mq + r) * g.ldo + n];
-              if ((a & 0x8000u) || a == 0) v = 0.f;
+              if (!((keep[i][j] >> (4 * q + r)) & 1u)) v = 0.f;
             }
             h[r] = f2bf(v);
             g.out[size_t(mq + r) * g.ldo + n] = h[r];
