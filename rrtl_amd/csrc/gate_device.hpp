@@ -41,7 +41,7 @@ __host__ __device__ inline uint32_t gate_eval(uint32_t op, uint32_t a, uint32_t 
 // ------------------------------------------------------------------ kernels
 struct TileArgs {
   uint32_t* state; uint64_t nwords;             // HBM rows [node][nwords]
-  const uint2* desc;                            // [stage][thread][position], STAGE_AHEAD padding stages
+  const uint2* desc;                            // [stage][thread][position]; STAGE_AHEAD rows past the end repeat the first stages
   uint32_t n_cycle_stages, n_post_stages, any_fix;  // any_fix: some gate applies its output inversion
   const uint32_t* load_slot; const uint32_t* load_node; uint32_t n_load;
   const uint32_t* stim_slot; const uint32_t* stim_mode; const uint32_t* stim_stream; const uint64_t* stim_arg; uint32_t n_stim;
