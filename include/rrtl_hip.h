@@ -74,11 +74,9 @@ typedef struct rrtl_hip_options {
   /* Word engine: active lanes per 64-thread wave (4, 8, 16, 32 or 64).  Fewer
    * lanes per wave put more waves on each SIMD, which hides memory latency for
    * a batch too small to do so at 64 lanes per wave, at the price of issuing
-   * each instruction for fewer lanes.  0 = automatic (programs under 600
-   * records: the largest of 64, 32, 16 that still gives four waves per CU,
-   * else 16; longer programs: 64).  With lane_waves above 1 it is the lanes
-   * of one workgroup (the columns its waves share; automatic: 64).  Read only
-   * when struct_size covers it. */
+   * each instruction for fewer lanes.  0 = automatic: 64.  With lane_waves
+   * above 1 it is the lanes of one workgroup (the columns its waves share;
+   * automatic: 64).  Read only when struct_size covers it. */
   uint32_t wave_lanes;
   /* Word engine runs with a halt signal: 1 = compact the lanes between
    * launches.  The run is cut into launches of ticks_per_launch cycles (1024

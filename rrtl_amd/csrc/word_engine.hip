@@ -1430,14 +1430,12 @@ class WordEngine final : public Engine {
     values_.alloc(size_t(P.total_signal_words) * lanes); values_.zero(stream_);
     mems_.alloc(size_t(P.total_memory_words) * lanes); mems_.zero(stream_);
     // One lane per thread: blocks of 256 lanes, smaller (down to one wave)
-    // when that is what it takes to give every CU a block.  A batch too small
-    // to give every SIMD (four per CU) a wave runs 32 or 16 lanes per wave
-    // instead when the per-cycle program is short: a wave's time per cycle is
-    // then a dependent chain of memory accesses that barely depends on how
-    // many of its lanes are active, so more, thinner waves finish the batch
-    // sooner.  Long programs (600 records or more, e.g. picorv32) are bound by
-    // the instructions each wave issues and keep full waves (measured: 3.5e9
-    // against 2.4e9 lane-cycles/s at 16 lanes per wave).
+    // when that is what it takes to give every CU a block.  Waves stay full (64
+    // lanes) unless the options ask for fewer: thinner waves for batches that
+    // leave SIMDs idle lost on every design measured with the current kernels
+    // (the README counter at 4,096 lanes 8 %, the RV32I core at 16,384 lanes
+    // 3.5 %, the mixed batch's counter and FIFO 20-34 %, picorv32 at 16 lanes
+    // per wave 2.4e9 against 3.5e9 lane-cycles/s).
     const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
     cus_ = cus;
     // Several waves per 64 lanes when the batch gives fewer than one 64-lane
@@ -1474,13 +1472,7 @@ class WordEngine final : public Engine {
     } else if (opt.lane_waves > 1) {
       throw Error("E_HIP_OPTIONS: lane_waves needs generated code without interpreted records");
     }
-    if (opt.wave_lanes) {
-      wave_lanes_ = opt.wave_lanes;  // 4, 8, 16, 32 or 64
-    } else {
-      wave_lanes_ = 64;
-      while (n_rec < 600 && wave_lanes_ > 16 && (uint64_t(lanes) + wave_lanes_ - 1) / wave_lanes_ < 4ull * cus)
-        wave_lanes_ /= 2;
-    }
+    wave_lanes_ = opt.wave_lanes ? opt.wave_lanes : 64u;  // 4, 8, 16, 32 or 64
     if (lane_waves_ > 1) {
       wave_lanes_ = split_lanes_;
       block_ = 64 * lane_waves_;
