@@ -460,7 +460,8 @@ def rv32i_uneven_halting(args) -> dict:
     """picorv32 with programs whose branches mostly jump forward (5 % of
     in-range targets backward), so that most lanes reach ecall at uneven
     cycles: the same 16,384-lane run under the cycle cap without and with lane
-    compaction between launches; halt cycles must agree."""
+    compaction between launches (automatic launch lengths: 1,024 cycles, then
+    each twice the last); halt cycles must agree."""
     from rrtl_amd import workloads
     out = {"workload": f"picorv32, 16384 lanes, 512-instruction programs with 5 % backward branches, cap {args.cycles}"}
     halts = {}
@@ -468,7 +469,7 @@ def rv32i_uneven_halting(args) -> dict:
         job = workloads.picorv32_job(16384, args.cycles, args.seed, backward=0.05)
         kw = dict(codegen=args.codegen, lane_waves=args.lane_waves, wave_lanes=args.wave_lanes)
         if compact:
-            kw.update(compact_lanes=True, ticks_per_launch=1024)
+            kw.update(compact_lanes=True)
         sim = job.make(**kw)
         sim.run(**dict(job.run, cycles=1))   # compile the run plan
         sim = job.make(**kw)

@@ -79,8 +79,9 @@ typedef struct rrtl_hip_options {
    * automatic: 64).  Read only when struct_size covers it. */
   uint32_t wave_lanes;
   /* Word engine runs with a halt signal: 1 = compact the lanes between
-   * launches.  The run is cut into launches of ticks_per_launch cycles (1024
-   * when that is 0); after each launch the lanes still running are packed into
+   * launches.  The run is cut into launches of ticks_per_launch cycles (when
+   * that is 0: 1024, then each launch twice the last); after each launch the
+   * lanes still running are packed into
    * the lowest threads, so lanes that halted free their waves and the next
    * launch runs full waves of live lanes only (the GPU counterpart of the
    * reference's work-stealing schedule over halting instances,

@@ -1897,7 +1897,10 @@ class WordEngine final : public Engine {
     // lane compaction: halting runs cut into launches, live lanes packed between them
     const bool compact = opt_.compact_lanes && outputs && A.halt_offset != NONE && A.halt_cycles;
     uint64_t chunk = (opt_.ticks_per_launch && outputs) ? opt_.ticks_per_launch : std::max<uint64_t>(cycles, 1);
-    if (compact && !opt_.ticks_per_launch) chunk = 1024;
+    // automatic launches of a compacting run: 1,024 cycles, then each twice the
+    // last, so a long tail of a few live lanes costs a handful of launches
+    const bool doubling = compact && !opt_.ticks_per_launch;
+    if (doubling) chunk = 1024;
     const uint32_t per_block = lanes_per_block();
     const uint32_t grid = (lanes_ + per_block - 1) / per_block;
     // the engine's own launches (batch items size theirs in batch_prepare)
@@ -1947,6 +1950,7 @@ class WordEngine final : public Engine {
       }
       launches++;
       done += B.cycles;
+      if (doubling) chunk *= 2;
     } while (done < cycles);
     return launches;
   }
