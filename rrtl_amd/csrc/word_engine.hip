@@ -1608,7 +1608,18 @@ class WordEngine final : public Engine {
         J.code = WordJit(WP, std::move(lay)).definitions(0, req.ns);
         mode = WM_ALL_LDS;
         J.lds_bytes = size_t(slots + sig + mw) * 256 * 4;
-        J.form = "word generated, 256 lanes per item, state in LDS";
+        // a design whose lanes fit one 64-lane wave per CU runs one wave per
+        // item instead: four times the items, each on a CU of its own, and a
+        // long run no longer shares a CU's issue slots with three more of its
+        // waves (with more lanes, one-wave items would leave workgroup slots
+        // holding a single wave: the mixed batch's 63,654-lane counter ran
+        // 1.7x slower that way)
+        if ((uint64_t(lanes_) + 63) / 64 <= cus_) {
+          per_item = 64;
+          J.form = "word generated, 64 lanes per item, state in LDS";
+        } else {
+          J.form = "word generated, 256 lanes per item, state in LDS";
+        }
         chosen = true;
       }
       for (int stage = 1; !chosen && !scr && stage >= 0; stage--) {
@@ -1645,7 +1656,9 @@ class WordEngine final : public Engine {
     }
     if (mode == WM_HBM && slots && !scratch_.p) { scratch_.alloc(size_t(WP.scratch_slots) * lanes_); scratch_.zero(stream_); }
     A.n_scratch = slots;
-    A.block_lanes = 0;  // every thread of an item runs a lane (G = 1), or 64 lanes per item (G = 4)
+    // every thread of an item runs a lane (G = 1, 256 lanes), one wave of it
+    // (G = 1, 64 lanes), or 64 lanes per item (G = 4)
+    A.block_lanes = (G == 1 && per_item == 64) ? 64u : 0u;
     A.scratch = mode == WM_HBM ? scratch_.p : nullptr;
     J.call = "rrtl::word::word_body<" + std::to_string(mode) + ", " + exec + ", " + std::to_string(G) + ">(U.w, item);";
     J.n_items = (lanes_ + per_item - 1) / per_item;
